@@ -1,0 +1,189 @@
+#!/usr/bin/env python3
+"""Benchmark: photons propagated per second on one MI355X.
+
+Workload (one step = one launch of the propagation kernel over the batch):
+  Cherenkov cascade — a 1e4 GeV electron cascade (ExtendedCherenkovEmitter,
+  longitudinal Γ profile a = 2.01849 + 1.45469·log10 E, b = 0.63207,
+  X0 = 0.3561 m) emitting 1e7 photons (300-800 nm Frank-Tamm table, 30
+  knots), propagated through Cascadia water (HG g = 0.95) to a string of 20
+  MultiPMTDetector modules (r = 0.21 m, 16 PMTs) at 50 m vertical spacing,
+  20 m from the cascade axis.  Synthetic inputs, all device-resident before
+  the timed region; hits go to device-resident columns.
+
+Prints one JSON line.  `roofline` is for the propagation kernel (FP32 VALU
+bound): algorithmic FLOP per launch from the kernel's own step/test counters
+÷ the kernel time measured with HIP events on its stream.  `cpu_baseline`
+times the CPU oracle (the reference algorithm restated in C, OpenMP) on a
+bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+FP32_PEAK_TFLOPS = 157.3        # MI355X published FP32 vector peak
+PMT_AREA = (75e-3 / 2) ** 2 * math.pi
+
+# Algorithmic work (counting rule: add/mul/cmp/select 1, fma 2, div 1,
+# transcendental 1): per scattering step 61 FLOP + 4 transcendentals besides
+# the sphere tests, 18 FLOP per sphere test; per photon initialisation 49 FLOP
+# + 9 transcendentals (isotropic) and 95 FLOP + 20 transcendentals more for an
+# extended Cherenkov emitter.
+FLOP_STEP = 61 + 4
+FLOP_TEST = 18
+FLOP_INIT = {"isotropic": 49 + 9, "cascade": 49 + 9 + 95 + 20}
+
+
+def build_workload(name: str, photons: int):
+    import neutrinotelescopes_amd as nt
+    med = nt.make_cascadia_medium_properties(0.95)
+    if name == "cascade":
+        spec = nt.CherenkovSpectrum((300.0, 800.0), 30, med)
+        lp = nt.LongitudinalParameterisation.from_energy(1e4, med, nt.PEMinus)
+        src = nt.ExtendedCherenkovEmitter(np.zeros(3, np.float32), np.array([1, 0, 0], np.float32),
+                                          0.0, photons, lp)
+        coords = nt.make_pom_pmt_coordinates(np.float32)
+        tgts = [nt.MultiPMTDetector(np.array([20.0, 0.0, -475.0 + 50.0 * k], np.float32), 0.21,
+                                    PMT_AREA, coords, k + 1) for k in range(20)]
+    elif name == "isotropic":
+        spec = nt.CherenkovSpectrum((300.0, 600.0), 30, med)
+        src = nt.PointlikeIsotropicEmitter(np.zeros(3, np.float32), 0.0, photons)
+        tgts = [nt.DetectionSphere(np.array([0, 0, 10.0], np.float32), 0.21, 16, PMT_AREA, 1)]
+    else:
+        raise ValueError(name)
+    return nt.PhotonPropSetup([src], tgts, med, spec)
+
+
+def cpu_baseline(workload: str, target_s: float = 12.0):
+    """Oracle photons/s on the host cores for a bounded sample."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import _oracle
+    threads = min(len(os.sched_getaffinity(0)), 16)
+    n = 20000
+    for _ in range(6):
+        s = build_workload(workload, n)
+        t0 = time.perf_counter()
+        _oracle.propagate(s, seed=1, threads=threads, capacity=1 << 20)
+        dt = time.perf_counter() - t0
+        if dt > 1.0:
+            break
+        n = int(n * min(50.0, 1.5 / max(dt, 1e-3)))
+    n_final = int(n * max(1.0, target_s / max(dt, 1e-6)))
+    s = build_workload(workload, n_final)
+    t0 = time.perf_counter()
+    _oracle.propagate(s, seed=2, threads=threads, capacity=1 << 22)
+    dt = time.perf_counter() - t0
+    return {"value": n_final / dt, "unit": "photons/s", "cores": threads, "kind": "port",
+            "sample": f"{n_final} photons of the {workload} workload, {dt:.1f} s on {threads} threads "
+                      f"(CPU oracle, g++ -O3, OpenMP)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="cascade", choices=["cascade", "isotropic"])
+    ap.add_argument("--photons", type=int, default=10_000_000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU per process; multi-GPU runs are not "
+                                   "part of this build (--gpus must be 1)"}))
+        sys.exit(2)
+
+    import torch
+    from neutrinotelescopes_amd import _abi
+    from neutrinotelescopes_amd.propagation import PropagationEngine
+
+    dev = torch.device("cuda:0")
+    setup = build_workload(args.workload, args.photons)
+    eng = PropagationEngine(0)
+    eng.setup(setup)
+    cap = 1 << 22
+    cols = dict(position=torch.empty(cap, 3, device=dev), direction=torch.empty(cap, 3, device=dev),
+                initial_direction=torch.empty(cap, 3, device=dev), wavelength=torch.empty(cap, device=dev),
+                time=torch.empty(cap, device=dev), dist_travelled=torch.empty(cap, device=dev),
+                module_id=torch.empty(cap, dtype=torch.int16, device=dev),
+                source_index=torch.empty(cap, dtype=torch.int32, device=dev),
+                photon_id=torch.empty(cap, dtype=torch.int64, device=dev))
+    counters = torch.zeros(4, dtype=torch.int64, device=dev)
+    h = _abi.nt_hits()
+    for k, v in cols.items():
+        setattr(h, k, v.data_ptr())
+    h.capacity = cap
+    stream = torch.cuda.current_stream()
+
+    def step(seed):
+        _abi.check(eng.lib.nt_launch(eng.handle, C.c_uint64(seed), C.byref(h), C.c_void_p(counters.data_ptr()),
+                                     C.c_void_p(stream.cuda_stream)), "nt_launch")
+
+    for i in range(args.warmup):
+        step(1000 + i)
+    torch.cuda.synchronize()
+    c = counters.cpu().numpy()
+    hits_per_step = int(c[0])
+    tests_per_step = int(c[2])
+
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ev[i][0].record(stream)
+        step(i)
+        ev[i][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = [a.elapsed_time(b) for a, b in ev]
+    c = counters.cpu().numpy()
+    assert int(c[1]) == args.photons, c
+
+    n_targets = len(setup.targets)
+    steps_total = tests_per_step / max(n_targets, 1)
+    flop = args.photons * FLOP_INIT[args.workload] + steps_total * FLOP_STEP + tests_per_step * FLOP_TEST
+    k_ms = float(np.mean(kernel_ms))
+    achieved = flop / (k_ms * 1e-3) / 1e12
+    value = args.photons * args.steps / wall
+
+    out = {
+        "metric": "photons propagated/s (1 GPU)",
+        "value": value,
+        "unit": "photons/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "cherenkov_cascade_1e4GeV_1e7ph_20x16pmt_string" if args.workload == "cascade"
+                   else "isotropic_point_source_1_sphere_10m", "photons_per_step": args.photons,
+                   "targets": n_targets, "hits_per_step": hits_per_step,
+                   "scattering_steps_per_photon": steps_total / args.photons,
+                   "sphere_tests_per_step": tests_per_step / max(steps_total, 1)},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "propagate_kernel", "kernel_ms": k_ms,
+                     "flop_per_launch": flop},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.workload)
+        out["gpu_vs_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,227 @@
+/*
+ * ntprop.h — C ABI of the MI355X photon-propagation library (libntprop.so).
+ *
+ * The library is a drop-in for the photon-propagation API of the Julia package
+ * NeutrinoTelescopes (module PhotonPropagationCuda).  Every entry point below
+ * names the reference interface it replaces (paths relative to the reference
+ * repository root).  Plain C types only: pointers, sizes, POD structs.
+ *
+ * Conventions
+ *  - Every function returns an int32 status (NT_OK = 0); no exception, device
+ *    trap or abort crosses the ABI.  The reference traps the kernel on hit
+ *    overflow (src/photon_propagation/photon_prop_cuda.jl:484); here overflow is
+ *    NT_ERR_HIT_OVERFLOW with the first `capacity` hits kept and the number of
+ *    lost hits reported.
+ *  - A context (nt_ctx) owns device-resident state: medium, spectrum table,
+ *    targets, sources, counters.  set_* calls are single-writer.
+ *  - Random numbers: Philox4x32-10 (Salmon et al., SC'11).  key = (seed_lo32,
+ *    seed_hi32); counter = (block, photon_lo32, photon_hi32, source_index),
+ *    where photon is the photon's index inside its source.  Block 0 holds the
+ *    initial-state draws, block 1+k the draws of scattering step k, block
+ *    0x80000000+t the draws of Gamma rejection trial t.  A photon's history is
+ *    therefore a pure function of (seed, source_index, photon index), whatever
+ *    the launch shape or the other sources in the batch.  The reference seeds
+ *    CUDA.jl's device RNG with seed + thread index (photon_prop_cuda.jl:423).
+ */
+#ifndef NTPROP_H
+#define NTPROP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NTPROP_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    NT_OK = 0,
+    NT_ERR_ARG = 1,          /* malformed argument (reference: error() in constructors) */
+    NT_ERR_OOM = 2,          /* device allocation failed */
+    NT_ERR_HIT_OVERFLOW = 3, /* more hits than capacity; first `capacity` kept */
+    NT_ERR_DEVICE = 4,       /* HIP runtime error */
+    NT_ERR_BUDGET = 5,       /* request does not fit the HBM budget (serving) */
+    NT_ERR_STATE = 6,        /* call out of order (e.g. propagate before set_medium) */
+    NT_ERR_NOT_READY = 7     /* asynchronous request still running */
+};
+
+/* ---- sources: src/photon_propagation/lightyield.jl:421-496 ------------- */
+enum {
+    NT_SRC_POINTLIKE_ISOTROPIC = 0,  /* PointlikeIsotropicEmitter  lightyield.jl:436-440 */
+    NT_SRC_POINTLIKE_TIME_RANGE = 1, /* PointlikeTimeRangeEmitter  lightyield.jl:449-453 */
+    NT_SRC_AXICONE = 2,              /* AxiconeEmitter             lightyield.jl:421-427 */
+    NT_SRC_PENCIL = 3,               /* PencilEmitter              lightyield.jl:429-434 */
+    NT_SRC_EXTENDED_CHERENKOV = 4,   /* ExtendedCherenkovEmitter   lightyield.jl:463-469 */
+    NT_SRC_POINTLIKE_CHERENKOV = 5   /* PointlikeCherenkovEmitter  lightyield.jl:491-496 */
+    /* CherenkovTrackEmitter (lightyield.jl:519-525) is not supported: its
+       device initialiser assigns a scalar to a direction
+       (photon_prop_cuda.jl:197) and cannot run in the reference either. */
+};
+
+typedef struct nt_source {           /* 64 bytes */
+    int32_t kind;                    /* NT_SRC_* */
+    int32_t reserved;
+    int64_t photons;                 /* >= 0 */
+    float position[3];               /* m */
+    float direction[3];              /* unit vector (all kinds except isotropic/time-range) */
+    float time;                      /* ns; time_range[0] for TIME_RANGE */
+    float time_end;                  /* ns; time_range[1] for TIME_RANGE */
+    float angle;                     /* rad; AXICONE opening angle */
+    float long_a;                    /* LongitudinalParameterisation a  (lightyield.jl:139-143) */
+    float long_b;                    /* LongitudinalParameterisation b */
+    float long_lrad;                 /* radiation length in m */
+} nt_source;
+
+/* ---- medium: WaterProperties, src/photon_propagation/medium.jl:70-125 --- */
+enum {
+    NT_DISPERSION_REF = 0,       /* dispersion as written in medium.jl:293-298 + group
+                                    velocity with λ0 = n·λ (medium.jl:313-318) */
+    NT_DISPERSION_QUAN_FRY = 1   /* derivative of the Quan-Fry n(λ) (extension) */
+};
+
+typedef struct nt_medium {           /* 48 bytes */
+    float quan_fry[4];               /* a01, a2, a3, a4 (medium.jl:174-205) */
+    float vol_conc_small_part;       /* ppm */
+    float vol_conc_large_part;       /* ppm */
+    float mean_scattering_angle;     /* g of Henyey-Greenstein (medium.jl:164-165) */
+    float sl_fraction;               /* Simplified-Liu mixture weight; 0 = reference */
+    float sca_len_scale;             /* multiplies the scattering length; 1 = reference */
+    int32_t dispersion_mode;         /* NT_DISPERSION_* */
+    int32_t max_steps;               /* scattering steps per photon; 15 = reference
+                                        (photon_prop_cuda.jl:449) */
+    int32_t reserved;
+} nt_medium;
+
+/* ---- spectrum: src/photon_propagation/spectrum.jl:21-108 --------------- */
+enum {
+    NT_SPEC_MONOCHROMATIC = 0,   /* Monochromatic (spectrum.jl:21-23) */
+    NT_SPEC_TABLE = 1,           /* CherenkovSpectrum: inverse-CDF knots sampled like
+                                    the reference's 1-D texture (spectrum.jl:97-105) */
+    NT_SPEC_INV_LAMBDA2 = 2      /* closed-form λ ∝ 1/λ² on wl_range (extension) */
+};
+enum {
+    NT_TEX_TEXEL_CENTER_CLAMP = 0,    /* normalized coords, linear filter, clamp:
+                                         x = u·N − 0.5, lerp(knots[i], knots[i+1]) */
+    NT_TEX_TEXEL_CENTER_CLAMP_Q8 = 1, /* same with the filter weight rounded to 1/256 */
+    NT_TEX_ENDPOINT = 2               /* lerp with knots at u = i/(N−1) */
+};
+
+typedef struct nt_spectrum {
+    int32_t kind;                    /* NT_SPEC_* */
+    int32_t addressing;              /* NT_TEX_* (TABLE only) */
+    float wavelength;                /* nm (MONOCHROMATIC) */
+    float wl_range[2];               /* nm (INV_LAMBDA2, informational for TABLE) */
+    int32_t n_knots;                 /* TABLE: 2..NT_MAX_KNOTS */
+    const float* knots;              /* TABLE: host pointer, wavelengths in nm */
+} nt_spectrum;
+
+#define NT_MAX_KNOTS 1024
+
+/* ---- targets: src/photon_propagation/detection.jl:29-46 ---------------- */
+enum {
+    NT_TARGET_SPHERE = 0,            /* DetectionSphere  detection.jl:29-35 */
+    NT_TARGET_MULTI_PMT = 1          /* MultiPMTDetector detection.jl:39-45 */
+};
+
+#define NT_MAX_PMTS 16
+#define NT_MAX_TARGETS 4096
+
+typedef struct nt_target {           /* 160 bytes */
+    float position[3];               /* m */
+    float radius;                    /* m */
+    float pmt_area;                  /* m^2 */
+    int32_t n_pmts;
+    int32_t kind;                    /* NT_TARGET_* */
+    uint16_t module_id;
+    uint16_t reserved;
+    float pmt_coordinates[2 * NT_MAX_PMTS]; /* (θ, φ) per PMT, column-major like
+                                               SMatrix{2,N} */
+} nt_target;
+
+/* ---- hit records: PhotonHit, photon_prop_cuda.jl:391-399 --------------- */
+typedef struct nt_hits {
+    float* position;                 /* [capacity][3]  SVector{3,Float32} column */
+    float* direction;                /* [capacity][3]  direction at detection */
+    float* initial_direction;        /* [capacity][3] */
+    float* wavelength;               /* [capacity] nm */
+    float* time;                     /* [capacity] ns */
+    float* dist_travelled;           /* [capacity] m */
+    uint16_t* module_id;             /* [capacity] */
+    uint32_t* source_index;          /* [capacity] optional (NULL: not written) */
+    uint64_t* photon_id;             /* [capacity] optional (NULL: not written) */
+    int64_t capacity;
+} nt_hits;
+
+/* ---- context ----------------------------------------------------------- */
+typedef struct nt_ctx nt_ctx;
+
+int32_t nt_abi_version(void);
+const char* nt_status_string(int32_t status);
+
+/* Create a context on HIP device `device`. */
+int32_t nt_create(int32_t device, nt_ctx** out_ctx);
+void nt_destroy(nt_ctx* ctx);
+
+/* Replaces the `medium` argument of run_photon_prop_no_local_cache
+   (photon_prop_cuda.jl:726-731). */
+int32_t nt_set_medium(nt_ctx* ctx, const nt_medium* medium);
+
+/* Replaces the `spectrum` argument (a CherenkovSpectrum texture handle,
+   spectrum.jl:97-105, or Monochromatic). */
+int32_t nt_set_spectrum(nt_ctx* ctx, const nt_spectrum* spectrum);
+
+/* Replaces `targets` (photon_prop_cuda.jl:740: CuVector(targets)).  Order is
+   kept: the first listed intersecting target wins (photon_prop_cuda.jl:460-468). */
+int32_t nt_set_targets(nt_ctx* ctx, const nt_target* targets, int32_t n_targets);
+
+/* Replaces `sources` (one kernel launch per source in photon_prop_cuda.jl:749-753;
+   here one launch for all). */
+int32_t nt_set_sources(nt_ctx* ctx, const nt_source* sources, int32_t n_sources);
+
+/* Synchronous drop-in for run_photon_prop_no_local_cache
+   (photon_prop_cuda.jl:726-759): propagate every photon of the sources set by
+   nt_set_sources and copy the hits into caller-owned HOST columns.
+   *n_hits = hits stored (<= capacity), *n_simulated = photons simulated
+   (out_n_ph_simulated), *n_dropped = hits lost to capacity. */
+int32_t nt_propagate(nt_ctx* ctx, uint64_t seed, const nt_hits* out_host,
+                     int64_t* n_hits, int64_t* n_simulated, int64_t* n_dropped);
+
+/* Asynchronous device form: hits go to caller-owned DEVICE columns, counters to
+   `d_counters` (device int64[4]: total hits found, photons simulated, sphere
+   tests, reserved), which the call clears first.  `stream` is a hipStream_t
+   (NULL = default stream).  Hits beyond capacity are counted but not stored. */
+int32_t nt_launch(nt_ctx* ctx, uint64_t seed, const nt_hits* out_dev,
+                  int64_t* d_counters, void* stream);
+
+/* Grid used by nt_launch (blocks, threads per block). */
+int32_t nt_launch_shape(nt_ctx* ctx, int32_t* blocks, int32_t* threads);
+
+/* ---- device-function evaluation (test hook) ----------------------------
+   Runs one device function of the propagation kernel on n inputs (host
+   arrays; row-major, `in_width` floats per row) and writes `out_width` floats
+   per row.  Used to check the kernel's math against FP64 evaluations. */
+enum {
+    NT_FN_INTERSECT = 0,        /* pos3 dir3 step tpos3 r -> isec d  (photon_prop_cuda.jl:233-271) */
+    NT_FN_HG_COS = 1,           /* g eta -> cos                     (photon_prop_cuda.jl:70-77) */
+    NT_FN_SL_COS = 2,           /* g xi -> cos                      (Simplified-Liu, extension) */
+    NT_FN_ROT_FROM_EZ = 3,      /* a3 op3 -> v3                     (utils.jl:228-243) */
+    NT_FN_SPH_TO_CART = 4,      /* theta phi -> v3                  (utils.jl:105-115) */
+    NT_FN_ISO_DIR = 5,          /* u_cos u_phi -> v3                (photon_prop_cuda.jl:100-104) */
+    NT_FN_REFRACTIVE_INDEX = 6, /* wl -> n                          (medium.jl:234-244) */
+    NT_FN_GROUP_VELOCITY = 7,   /* wl -> c_grp                      (medium.jl:313-318) */
+    NT_FN_SCA_LEN = 8,          /* wl -> L_sca                      (medium.jl:331-347) */
+    NT_FN_CHERENKOV_DIR = 9,    /* wl u_phi u_tcos u_tphi dir3 -> v3 (photon_prop_cuda.jl:107-125) */
+    NT_FN_UPDATE_DIR = 10,      /* dir3 eta u_phi u_mix -> v3       (photon_prop_cuda.jl:204-222) */
+    NT_FN_SPECTRUM = 11,        /* u -> wl                          (photon_prop_cuda.jl:128-130) */
+    NT_FN_PHILOX = 12,          /* ctr4 key2 (uint32 bits) -> out4  */
+    NT_FN_GAMMA = 13,           /* a b lrad seed_lo seed_hi photon (bits) -> depth trials */
+    NT_FN_COUNT = 14
+};
+int32_t nt_eval(nt_ctx* ctx, int32_t fn, const float* in, int32_t in_width,
+                float* out, int32_t out_width, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NTPROP_H */

@@ -1,0 +1,192 @@
+"""ctypes binding of libntprop.so (include/ntprop.h).
+
+The library is the product: every propagation call goes through it.  Loading
+fails loudly when the shared library has not been built; there is no CPU
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = Path(os.environ.get("NTPROP_LIBRARY", PKG_DIR / "libntprop.so"))
+
+NT_OK = 0
+NT_ERR_ARG = 1
+NT_ERR_OOM = 2
+NT_ERR_HIT_OVERFLOW = 3
+NT_ERR_DEVICE = 4
+NT_ERR_BUDGET = 5
+NT_ERR_STATE = 6
+NT_ERR_NOT_READY = 7
+
+NT_SRC_POINTLIKE_ISOTROPIC = 0
+NT_SRC_POINTLIKE_TIME_RANGE = 1
+NT_SRC_AXICONE = 2
+NT_SRC_PENCIL = 3
+NT_SRC_EXTENDED_CHERENKOV = 4
+NT_SRC_POINTLIKE_CHERENKOV = 5
+
+NT_DISPERSION_REF = 0
+NT_DISPERSION_QUAN_FRY = 1
+
+NT_SPEC_MONOCHROMATIC = 0
+NT_SPEC_TABLE = 1
+NT_SPEC_INV_LAMBDA2 = 2
+NT_TEX_TEXEL_CENTER_CLAMP = 0
+NT_TEX_TEXEL_CENTER_CLAMP_Q8 = 1
+NT_TEX_ENDPOINT = 2
+
+NT_TARGET_SPHERE = 0
+NT_TARGET_MULTI_PMT = 1
+NT_MAX_PMTS = 16
+NT_MAX_TARGETS = 4096
+NT_MAX_KNOTS = 1024
+
+(NT_FN_INTERSECT, NT_FN_HG_COS, NT_FN_SL_COS, NT_FN_ROT_FROM_EZ, NT_FN_SPH_TO_CART,
+ NT_FN_ISO_DIR, NT_FN_REFRACTIVE_INDEX, NT_FN_GROUP_VELOCITY, NT_FN_SCA_LEN,
+ NT_FN_CHERENKOV_DIR, NT_FN_UPDATE_DIR, NT_FN_SPECTRUM, NT_FN_PHILOX, NT_FN_GAMMA) = range(14)
+
+
+class nt_source(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("reserved", C.c_int32),
+        ("photons", C.c_int64),
+        ("position", C.c_float * 3),
+        ("direction", C.c_float * 3),
+        ("time", C.c_float),
+        ("time_end", C.c_float),
+        ("angle", C.c_float),
+        ("long_a", C.c_float),
+        ("long_b", C.c_float),
+        ("long_lrad", C.c_float),
+    ]
+
+
+class nt_medium(C.Structure):
+    _fields_ = [
+        ("quan_fry", C.c_float * 4),
+        ("vol_conc_small_part", C.c_float),
+        ("vol_conc_large_part", C.c_float),
+        ("mean_scattering_angle", C.c_float),
+        ("sl_fraction", C.c_float),
+        ("sca_len_scale", C.c_float),
+        ("dispersion_mode", C.c_int32),
+        ("max_steps", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class nt_spectrum(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("addressing", C.c_int32),
+        ("wavelength", C.c_float),
+        ("wl_range", C.c_float * 2),
+        ("n_knots", C.c_int32),
+        ("knots", C.POINTER(C.c_float)),
+    ]
+
+
+class nt_target(C.Structure):
+    _fields_ = [
+        ("position", C.c_float * 3),
+        ("radius", C.c_float),
+        ("pmt_area", C.c_float),
+        ("n_pmts", C.c_int32),
+        ("kind", C.c_int32),
+        ("module_id", C.c_uint16),
+        ("reserved", C.c_uint16),
+        ("pmt_coordinates", C.c_float * (2 * NT_MAX_PMTS)),
+    ]
+
+
+class nt_hits(C.Structure):
+    _fields_ = [
+        ("position", C.c_void_p),
+        ("direction", C.c_void_p),
+        ("initial_direction", C.c_void_p),
+        ("wavelength", C.c_void_p),
+        ("time", C.c_void_p),
+        ("dist_travelled", C.c_void_p),
+        ("module_id", C.c_void_p),
+        ("source_index", C.c_void_p),
+        ("photon_id", C.c_void_p),
+        ("capacity", C.c_int64),
+    ]
+
+
+assert C.sizeof(nt_source) == 64
+assert C.sizeof(nt_medium) == 48
+assert C.sizeof(nt_target) == 160
+
+# exported symbol -> (restype, argtypes)
+SIGNATURES = {
+    "nt_abi_version": (C.c_int32, []),
+    "nt_status_string": (C.c_char_p, [C.c_int32]),
+    "nt_create": (C.c_int32, [C.c_int32, C.POINTER(C.c_void_p)]),
+    "nt_destroy": (None, [C.c_void_p]),
+    "nt_set_medium": (C.c_int32, [C.c_void_p, C.POINTER(nt_medium)]),
+    "nt_set_spectrum": (C.c_int32, [C.c_void_p, C.POINTER(nt_spectrum)]),
+    "nt_set_targets": (C.c_int32, [C.c_void_p, C.POINTER(nt_target), C.c_int32]),
+    "nt_set_sources": (C.c_int32, [C.c_void_p, C.POINTER(nt_source), C.c_int32]),
+    "nt_propagate": (C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(nt_hits), C.POINTER(C.c_int64),
+                                 C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "nt_launch": (C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(nt_hits), C.c_void_p, C.c_void_p]),
+    "nt_launch_shape": (C.c_int32, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "nt_eval": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
+                            C.c_int64]),
+}
+
+_lib = None
+
+
+def load_library(path: Path | None = None) -> C.CDLL:
+    """Load libntprop.so; raise if it is missing (no fallback path exists)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(
+            f"{p} is not built: run neutrinotelescopes_amd._build.build_library() "
+            "(or __graft_entry__.build()) first")
+    lib = C.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.nt_abi_version() != 1:
+        raise RuntimeError("libntprop ABI version mismatch")
+    if path is None:
+        _lib = lib
+    return lib
+
+
+class NTError(RuntimeError):
+    def __init__(self, status: int, where: str):
+        lib = load_library()
+        msg = lib.nt_status_string(status).decode(errors="replace")
+        super().__init__(f"{where}: status {status} ({msg})")
+        self.status = status
+
+
+def check(status: int, where: str) -> None:
+    if status != NT_OK:
+        raise NTError(status, where)
+
+
+def ptr(a: np.ndarray | None) -> int | None:
+    return None if a is None else a.ctypes.data
+
+
+def as_c_array(ctype, items):
+    arr = (ctype * max(len(items), 1))()
+    for i, it in enumerate(items):
+        arr[i] = it
+    return arr

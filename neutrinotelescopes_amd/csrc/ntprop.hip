@@ -1,0 +1,777 @@
+// ntprop.hip — persistent photon-propagation kernel for gfx950 and the C ABI
+// declared in include/ntprop.h.
+//
+// Kernel: one photon per lane, a persistent grid that strides over the
+// concatenated photon index space of all sources.  Per photon: initial state
+// (photon_prop_cuda.jl:435-446), then at most max_steps scattering steps
+// (photon_prop_cuda.jl:449-495), each testing every target sphere in list
+// order.  Target spheres are wave-uniform data read through scalar loads, the
+// spectrum knots sit in LDS, and hits are compacted per wavefront with a
+// ballot + mbcnt prefix into an LDS staging area, then written to HBM as
+// contiguous column segments with one 64-bit atomic per flush.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "ntprop_device.h"
+
+using namespace ntp;
+
+namespace {
+
+constexpr int kThreads = 256;          // 4 wavefronts per workgroup
+constexpr int kWaves = kThreads / 64;
+constexpr int kStageFields = 16;       // 32-bit words per staged hit
+constexpr int kStageSlots = 64;        // staged hits per wavefront
+
+// Targets are read with wave-uniform indices through the constant address
+// space, which the compiler turns into scalar (SGPR) loads: one s_load per
+// target per step for the whole wavefront, no VGPRs and no LDS traffic.
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(4))) f4* const_f4_ptr;
+
+struct DevSource {                     // 80 bytes, one per source
+    int32_t kind;
+    uint32_t index;                    // Philox counter word 3
+    int64_t offset;                    // first global photon index
+    int64_t photons;
+    float pos[3];
+    float dir[3];
+    float t0, t1, angle;
+    float gamma_d, gamma_c, gamma_scale, lrad;
+    float pad;
+};
+
+struct HitCols {
+    float* pos;
+    float* dir;
+    float* idir;
+    float* wl;
+    float* time;
+    float* dist;
+    uint16_t* module_id;
+    uint32_t* source_index;
+    uint64_t* photon_id;
+    int64_t capacity;
+};
+
+struct KParams {
+    const DevSource* __restrict__ sources;
+    const int64_t* __restrict__ offsets;     // n_sources + 1
+    const float4* __restrict__ targets;      // x, y, z, r^2
+    const uint32_t* __restrict__ module_ids;
+    const float* __restrict__ knots;
+    int64_t* counters;                       // [0] hits found, [1] photons, [2] sphere tests
+    HitCols hits;
+    int64_t total;
+    Medium medium;
+    Spectrum spectrum;
+    int n_sources;
+    int n_targets;
+    uint32_t seed_lo, seed_hi;
+};
+
+__device__ __forceinline__ int find_source(const int64_t* __restrict__ offs, int n, int64_t g) {
+    int lo = 0, hi = n - 1;            // largest s with offs[s] <= g
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (offs[mid] <= g) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Write the staged hits of this wavefront to the output columns.
+__device__ __forceinline__ void flush_stage(const KParams& p, const uint32_t* __restrict__ st, int n,
+                                            int lane) {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd((unsigned long long*)&p.counters[0], (unsigned long long)n);
+    base = __shfl(base, 0);
+    const HitCols& h = p.hits;
+    for (int i = lane; i < n; i += 64) {
+        const int64_t idx = (int64_t)base + i;
+        if (idx >= h.capacity) continue;
+        const float* f = reinterpret_cast<const float*>(st);
+        h.pos[3 * idx + 0] = f[0 * kStageSlots + i];
+        h.pos[3 * idx + 1] = f[1 * kStageSlots + i];
+        h.pos[3 * idx + 2] = f[2 * kStageSlots + i];
+        h.dir[3 * idx + 0] = f[3 * kStageSlots + i];
+        h.dir[3 * idx + 1] = f[4 * kStageSlots + i];
+        h.dir[3 * idx + 2] = f[5 * kStageSlots + i];
+        h.idir[3 * idx + 0] = f[6 * kStageSlots + i];
+        h.idir[3 * idx + 1] = f[7 * kStageSlots + i];
+        h.idir[3 * idx + 2] = f[8 * kStageSlots + i];
+        h.wl[idx] = f[9 * kStageSlots + i];
+        h.time[idx] = f[10 * kStageSlots + i];
+        h.dist[idx] = f[11 * kStageSlots + i];
+        h.module_id[idx] = (uint16_t)st[12 * kStageSlots + i];
+        if (h.source_index) h.source_index[idx] = st[13 * kStageSlots + i];
+        if (h.photon_id)
+            h.photon_id[idx] = (uint64_t)st[14 * kStageSlots + i] |
+                               ((uint64_t)st[15 * kStageSlots + i] << 32);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__global__ __launch_bounds__(kThreads) void propagate_kernel(KParams p) {
+    extern __shared__ float lds[];
+    float* s_knots = lds;                                          // n_knots (padded to 4)
+    uint32_t* s_stage = reinterpret_cast<uint32_t*>(lds + ((p.spectrum.n + 3) & ~3));
+
+    for (int i = threadIdx.x; i < p.spectrum.n; i += kThreads) s_knots[i] = p.knots[i];
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    uint32_t* st = s_stage + wave * (kStageFields * kStageSlots);
+    int staged = 0;                                                // wave-uniform
+    int64_t simulated = 0;
+    int64_t tests = 0;
+
+    const Medium& m = p.medium;
+    const int64_t stride = (int64_t)gridDim.x * kThreads;
+    int src = -1;
+    int64_t src_lo = 0, src_hi = 0;
+
+    for (int64_t base = (int64_t)blockIdx.x * kThreads + wave * 64; base < p.total; base += stride) {
+        const int64_t g = base + lane;
+        bool hit = false;
+        float3 pos, dir, idir;
+        float wl = 0.f, time = 0.f, dist = 0.f;
+        uint32_t mid = 0, sidx = 0;
+        uint64_t pid = 0;
+        if (g < p.total) {
+            if (g < src_lo || g >= src_hi) {
+                src = find_source(p.offsets, p.n_sources, g);
+                src_lo = p.offsets[src];
+                src_hi = p.offsets[src + 1];
+            }
+            const DevSource& s = p.sources[src];
+            pid = (uint64_t)(g - src_lo);
+            sidx = s.index;
+            const Rng rng{p.seed_lo, p.seed_hi, (uint32_t)pid, (uint32_t)(pid >> 32), sidx};
+
+            // initialize_photon_state, photon_prop_cuda.jl:133-188
+            const uint4 r0 = rng.block(0u);
+            wl = sample_spectrum(p.spectrum, s_knots, u01(r0.x));
+            pos = make_float3(s.pos[0], s.pos[1], s.pos[2]);
+            const float3 sdir = make_float3(s.dir[0], s.dir[1], s.dir[2]);
+            time = s.t0;
+            switch (s.kind) {
+                case NT_SRC_POINTLIKE_ISOTROPIC:
+                    dir = iso_dir(u01(r0.y), u01(r0.z));
+                    break;
+                case NT_SRC_POINTLIKE_TIME_RANGE:
+                    dir = iso_dir(u01(r0.y), u01(r0.z));
+                    time = s.t0 + u01(r0.w) * (s.t1 - s.t0);
+                    break;
+                case NT_SRC_AXICONE:
+                    dir = rot_from_ez(sdir, sph_to_cart(s.angle, 0.0f + u01(r0.z) * kTwoPi));
+                    break;
+                case NT_SRC_PENCIL:
+                    dir = sdir;
+                    break;
+                case NT_SRC_EXTENDED_CHERENKOV: {
+                    const float lp = rand_gamma(s.gamma_d, s.gamma_c, s.gamma_scale, rng, nullptr) * s.lrad;
+                    pos = make_float3(pos.x + lp * sdir.x, pos.y + lp * sdir.y, pos.z + lp * sdir.z);
+                    time = s.t0 + lp / kCVacMNs;
+                    dir = cherenkov_dir(wl, u01(r0.y), u01(r0.z), u01(r0.w), sdir, m);
+                    break;
+                }
+                default:  // NT_SRC_POINTLIKE_CHERENKOV
+                    dir = cherenkov_dir(wl, u01(r0.y), u01(r0.z), u01(r0.w), sdir, m);
+                    break;
+            }
+            idir = dir;
+            const float sca_len = scattering_length(wl, m);
+            const float c_grp = group_velocity(wl, m);
+
+            for (int k = 0; k < m.max_steps; ++k) {
+                const uint4 r = rng.block(1u + (uint32_t)k);
+                const float step = -logf(u01_oc(r.x)) * sca_len;
+                // first listed target hit within this step (photon_prop_cuda.jl:457-468)
+                int hit_t = -1;
+                float dt = 0.0f;
+                const const_f4_ptr tgt = (const_f4_ptr)(const void*)p.targets;
+                for (int t = 0; t < p.n_targets; ++t) {
+                    const f4 tv = tgt[t];
+                    const float4 T = make_float4(tv.x, tv.y, tv.z, tv.w);
+                    float a;
+                    const float b = isect_b(pos, dir, T, a);
+                    if (b >= 0.0f && hit_t < 0) {
+                        const float d = -a - sqrtf(b);
+                        if ((d > 0.0f) & (d < step)) {
+                            hit_t = t;
+                            dt = d;
+                        }
+                    }
+                }
+                tests += p.n_targets;
+                if (hit_t < 0) {
+                    pos = make_float3(pos.x + dir.x * step, pos.y + dir.y * step, pos.z + dir.z * step);
+                    dist += step;
+                    time += step / c_grp;
+                    dir = update_direction(dir, u01(r.y), u01(r.z), u01(r.w), m);
+                    continue;
+                }
+                pos = make_float3(pos.x + dir.x * dt, pos.y + dir.y * dt, pos.z + dir.z * dt);
+                dist += dt;
+                time += dt / c_grp;
+                mid = p.module_ids[hit_t];
+                hit = true;
+                break;
+            }
+            ++simulated;
+        }
+
+        // wavefront compaction of this round's hits into the LDS stage
+        const uint64_t mask = __ballot(hit);
+        if (mask) {
+            const int cnt = __popcll(mask);
+            if (staged + cnt > kStageSlots) {
+                flush_stage(p, st, staged, lane);
+                staged = 0;
+            }
+            if (hit) {
+                const int slot = staged + (int)__builtin_amdgcn_mbcnt_hi(
+                                              (uint32_t)(mask >> 32),
+                                              __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+                float* f = reinterpret_cast<float*>(st);
+                f[0 * kStageSlots + slot] = pos.x;
+                f[1 * kStageSlots + slot] = pos.y;
+                f[2 * kStageSlots + slot] = pos.z;
+                f[3 * kStageSlots + slot] = dir.x;
+                f[4 * kStageSlots + slot] = dir.y;
+                f[5 * kStageSlots + slot] = dir.z;
+                f[6 * kStageSlots + slot] = idir.x;
+                f[7 * kStageSlots + slot] = idir.y;
+                f[8 * kStageSlots + slot] = idir.z;
+                f[9 * kStageSlots + slot] = wl;
+                f[10 * kStageSlots + slot] = time;
+                f[11 * kStageSlots + slot] = dist;
+                st[12 * kStageSlots + slot] = mid;
+                st[13 * kStageSlots + slot] = sidx;
+                st[14 * kStageSlots + slot] = (uint32_t)pid;
+                st[15 * kStageSlots + slot] = (uint32_t)(pid >> 32);
+            }
+            staged += cnt;
+        }
+    }
+    if (staged) flush_stage(p, st, staged, lane);
+
+    // one atomic per wavefront for the photon and test counters
+    for (int off = 32; off > 0; off >>= 1) {
+        simulated += __shfl_down(simulated, off);
+        tests += __shfl_down(tests, off);
+    }
+    if (lane == 0) {
+        atomicAdd((unsigned long long*)&p.counters[1], (unsigned long long)simulated);
+        atomicAdd((unsigned long long*)&p.counters[2], (unsigned long long)tests);
+    }
+}
+
+__global__ void clear_counters(int64_t* c) {
+    if (threadIdx.x < 4) c[threadIdx.x] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Device-function evaluation kernel (test hook)
+// ---------------------------------------------------------------------------
+__global__ void eval_kernel(int fn, const float* __restrict__ in, int iw, float* __restrict__ out,
+                            int ow, int64_t n, Medium m, Spectrum s, const float* __restrict__ knots) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* x = in + i * iw;
+    float* y = out + i * ow;
+    switch (fn) {
+        case NT_FN_INTERSECT: {
+            const float3 pos = make_float3(x[0], x[1], x[2]);
+            const float3 dir = make_float3(x[3], x[4], x[5]);
+            const float4 T = make_float4(x[7], x[8], x[9], x[10] * x[10]);
+            float a;
+            const float b = isect_b(pos, dir, T, a);
+            float isec = 0.0f, d = NAN;
+            if (b >= 0.0f) {
+                const float dd = -a - sqrtf(b);
+                if ((dd > 0.0f) & (dd < x[6])) {
+                    isec = 1.0f;
+                    d = dd;
+                }
+            }
+            y[0] = isec;
+            y[1] = d;
+            break;
+        }
+        case NT_FN_HG_COS: y[0] = hg_cos(x[0], x[1]); break;
+        case NT_FN_SL_COS: y[0] = sl_cos(x[0], x[1]); break;
+        case NT_FN_ROT_FROM_EZ: {
+            const float3 v = rot_from_ez(make_float3(x[0], x[1], x[2]), make_float3(x[3], x[4], x[5]));
+            y[0] = v.x; y[1] = v.y; y[2] = v.z;
+            break;
+        }
+        case NT_FN_SPH_TO_CART: {
+            const float3 v = sph_to_cart(x[0], x[1]);
+            y[0] = v.x; y[1] = v.y; y[2] = v.z;
+            break;
+        }
+        case NT_FN_ISO_DIR: {
+            const float3 v = iso_dir(x[0], x[1]);
+            y[0] = v.x; y[1] = v.y; y[2] = v.z;
+            break;
+        }
+        case NT_FN_REFRACTIVE_INDEX: y[0] = refractive_index(x[0], m); break;
+        case NT_FN_GROUP_VELOCITY: y[0] = group_velocity(x[0], m); break;
+        case NT_FN_SCA_LEN: y[0] = scattering_length(x[0], m); break;
+        case NT_FN_CHERENKOV_DIR: {
+            const float3 v = cherenkov_dir(x[0], x[1], x[2], x[3], make_float3(x[4], x[5], x[6]), m);
+            y[0] = v.x; y[1] = v.y; y[2] = v.z;
+            break;
+        }
+        case NT_FN_UPDATE_DIR: {
+            const float3 v = update_direction(make_float3(x[0], x[1], x[2]), x[3], x[4], x[5], m);
+            y[0] = v.x; y[1] = v.y; y[2] = v.z;
+            break;
+        }
+        case NT_FN_SPECTRUM: y[0] = sample_spectrum(s, knots, x[0]); break;
+        case NT_FN_PHILOX: {
+            const uint32_t* u = reinterpret_cast<const uint32_t*>(x);
+            const uint4 r = philox(make_uint4(u[0], u[1], u[2], u[3]), u[4], u[5]);
+            uint32_t* o = reinterpret_cast<uint32_t*>(y);
+            o[0] = r.x; o[1] = r.y; o[2] = r.z; o[3] = r.w;
+            break;
+        }
+        case NT_FN_GAMMA: {
+            // in: a, b, lrad, seed_lo, seed_hi, photon_lo, photon_hi, source (last five as bits)
+            const uint32_t* u = reinterpret_cast<const uint32_t*>(x);
+            const float d = (float)((double)x[0] - 1.0 / 3.0);
+            const float c = 1.0f / sqrtf(9.0f * d);
+            const Rng rng{u[3], u[4], u[5], u[6], u[7]};
+            int trials = 0;
+            y[0] = rand_gamma(d, c, 1.0f / x[1], rng, &trials) * x[2];
+            y[1] = (float)trials;
+            break;
+        }
+        default: break;
+    }
+}
+
+}  // namespace
+
+// ===========================================================================
+// Host side: context and C ABI
+// ===========================================================================
+struct nt_ctx {
+    int device = 0;
+    int num_cus = 0;
+    hipStream_t stream = nullptr;
+    int blocks = 0;
+    std::mutex mu;
+
+    bool has_medium = false, has_spectrum = false, has_targets = false, has_sources = false;
+    Medium medium{};
+    Spectrum spectrum{};
+    float* d_knots = nullptr;
+    int knots_cap = 0;
+
+    int n_targets = 0;
+    float4* d_targets = nullptr;
+    uint32_t* d_module_ids = nullptr;
+    int targets_cap = 0;
+
+    int n_sources = 0;
+    DevSource* d_sources = nullptr;
+    int64_t* d_offsets = nullptr;
+    int sources_cap = 0;
+    int64_t total = 0;
+
+    int64_t* d_counters = nullptr;
+    // hit columns owned by the synchronous API
+    HitCols d_hits{};
+    int64_t hits_cap = 0;
+};
+
+namespace {
+
+thread_local char g_err[256];
+
+int32_t fail_hip(hipError_t e, const char* what) {
+    snprintf(g_err, sizeof g_err, "%s: %s", what, hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? NT_ERR_OOM : NT_ERR_DEVICE;
+}
+
+#define HIP_TRY(x)                                   \
+    do {                                             \
+        hipError_t _e = (x);                         \
+        if (_e != hipSuccess) return fail_hip(_e, #x); \
+    } while (0)
+
+template <typename T>
+int32_t grow(T** p, int64_t need, int64_t* cap_items) {
+    if (need <= *cap_items && *p) return NT_OK;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    int64_t n = std::max<int64_t>(need, 1);
+    HIP_TRY(hipMalloc((void**)p, sizeof(T) * n));
+    *cap_items = n;
+    return NT_OK;
+}
+
+void free_hits(HitCols& h) {
+    void* ptrs[] = {h.pos, h.dir, h.idir, h.wl, h.time, h.dist, h.module_id, h.source_index, h.photon_id};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    h = HitCols{};
+}
+
+size_t lds_bytes(int n_knots) {
+    return sizeof(float) * ((n_knots + 3) & ~3) + sizeof(uint32_t) * kWaves * kStageFields * kStageSlots;
+}
+
+int32_t launch(nt_ctx* c, uint64_t seed, const HitCols& cols, int64_t* d_counters, hipStream_t s) {
+    if (!c->has_medium || !c->has_spectrum || !c->has_targets || !c->has_sources) {
+        snprintf(g_err, sizeof g_err, "medium, spectrum, targets and sources must be set first");
+        return NT_ERR_STATE;
+    }
+    hipLaunchKernelGGL(clear_counters, dim3(1), dim3(64), 0, s, d_counters);
+    if (c->total == 0) return NT_OK;
+    KParams kp{};
+    kp.sources = c->d_sources;
+    kp.offsets = c->d_offsets;
+    kp.targets = c->d_targets;
+    kp.module_ids = c->d_module_ids;
+    kp.knots = c->d_knots;
+    kp.counters = d_counters;
+    kp.hits = cols;
+    kp.total = c->total;
+    kp.medium = c->medium;
+    kp.spectrum = c->spectrum;
+    kp.n_sources = c->n_sources;
+    kp.n_targets = c->n_targets;
+    kp.seed_lo = (uint32_t)seed;
+    kp.seed_hi = (uint32_t)(seed >> 32);
+    int64_t need_blocks = (c->total + kThreads - 1) / kThreads;
+    int blocks = (int)std::min<int64_t>(c->blocks, need_blocks);
+    hipLaunchKernelGGL(propagate_kernel, dim3(blocks), dim3(kThreads), lds_bytes(c->spectrum.n), s, kp);
+    HIP_TRY(hipGetLastError());
+    return NT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t nt_abi_version(void) { return NTPROP_ABI_VERSION; }
+
+const char* nt_status_string(int32_t st) {
+    switch (st) {
+        case NT_OK: return "ok";
+        case NT_ERR_ARG: return g_err[0] ? g_err : "invalid argument";
+        case NT_ERR_OOM: return g_err[0] ? g_err : "out of device memory";
+        case NT_ERR_HIT_OVERFLOW: return "hit capacity exceeded";
+        case NT_ERR_DEVICE: return g_err[0] ? g_err : "device error";
+        case NT_ERR_BUDGET: return g_err[0] ? g_err : "request exceeds the memory budget";
+        case NT_ERR_STATE: return g_err[0] ? g_err : "call out of order";
+        case NT_ERR_NOT_READY: return "not ready";
+        default: return "unknown status";
+    }
+}
+
+int32_t nt_create(int32_t device, nt_ctx** out) {
+    if (!out) return NT_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) {
+        snprintf(g_err, sizeof g_err, "device %d out of range (%d devices)", device, n);
+        return NT_ERR_ARG;
+    }
+    HIP_TRY(hipSetDevice(device));
+    nt_ctx* c = new nt_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete c; return NT_ERR_DEVICE; }
+    c->num_cus = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return NT_ERR_DEVICE; }
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, propagate_kernel, kThreads,
+                                                     lds_bytes(NT_MAX_KNOTS)) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    c->blocks = c->num_cus * per_cu;
+    if (hipMalloc((void**)&c->d_counters, 4 * sizeof(int64_t)) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return NT_ERR_OOM;
+    }
+    *out = c;
+    return NT_OK;
+}
+
+void nt_destroy(nt_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    void* ptrs[] = {c->d_knots, c->d_targets, c->d_module_ids, c->d_sources, c->d_offsets, c->d_counters};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    free_hits(c->d_hits);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int32_t nt_set_medium(nt_ctx* c, const nt_medium* m) {
+    if (!c || !m) return NT_ERR_ARG;
+    if (!(m->mean_scattering_angle > 0.0f && m->mean_scattering_angle < 1.0f) ||
+        m->max_steps < 0 || !(m->sl_fraction >= 0.0f && m->sl_fraction <= 1.0f) ||
+        !(m->sca_len_scale > 0.0f) ||
+        (m->dispersion_mode != NT_DISPERSION_REF && m->dispersion_mode != NT_DISPERSION_QUAN_FRY)) {
+        snprintf(g_err, sizeof g_err, "invalid medium parameters");
+        return NT_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    Medium& d = c->medium;
+    d.a01 = m->quan_fry[0]; d.a2 = m->quan_fry[1]; d.a3 = m->quan_fry[2]; d.a4 = m->quan_fry[3];
+    d.vcs = m->vol_conc_small_part;
+    d.vcl = m->vol_conc_large_part;
+    d.g = m->mean_scattering_angle;
+    d.sl_fraction = m->sl_fraction;
+    d.sca_len_scale = m->sca_len_scale;
+    d.dispersion_mode = m->dispersion_mode;
+    d.max_steps = m->max_steps;
+    c->has_medium = true;
+    return NT_OK;
+}
+
+int32_t nt_set_spectrum(nt_ctx* c, const nt_spectrum* s) {
+    if (!c || !s) return NT_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    Spectrum d{};
+    d.kind = s->kind;
+    d.addressing = s->addressing;
+    d.wl = s->wavelength;
+    d.lo = s->wl_range[0];
+    d.hi = s->wl_range[1];
+    d.n = 0;
+    if (s->kind == NT_SPEC_TABLE) {
+        if (s->n_knots < 2 || s->n_knots > NT_MAX_KNOTS || !s->knots ||
+            s->addressing < NT_TEX_TEXEL_CENTER_CLAMP || s->addressing > NT_TEX_ENDPOINT) {
+            snprintf(g_err, sizeof g_err, "spectrum table needs 2..%d knots and a valid addressing", NT_MAX_KNOTS);
+            return NT_ERR_ARG;
+        }
+        int64_t cap = c->knots_cap;
+        int32_t st = grow(&c->d_knots, s->n_knots, &cap);
+        if (st) return st;
+        c->knots_cap = (int)cap;
+        HIP_TRY(hipMemcpy(c->d_knots, s->knots, sizeof(float) * s->n_knots, hipMemcpyHostToDevice));
+        d.n = s->n_knots;
+    } else if (s->kind == NT_SPEC_MONOCHROMATIC) {
+        if (!(s->wavelength > 0.0f)) { snprintf(g_err, sizeof g_err, "wavelength must be > 0"); return NT_ERR_ARG; }
+    } else if (s->kind == NT_SPEC_INV_LAMBDA2) {
+        if (!(s->wl_range[0] > 0.0f && s->wl_range[1] > s->wl_range[0])) {
+            snprintf(g_err, sizeof g_err, "invalid wavelength range");
+            return NT_ERR_ARG;
+        }
+    } else {
+        snprintf(g_err, sizeof g_err, "unknown spectrum kind %d", s->kind);
+        return NT_ERR_ARG;
+    }
+    if (!c->d_knots) {                      // keep a valid pointer for the kernel
+        int64_t cap = 0;
+        int32_t st = grow(&c->d_knots, 1, &cap);
+        if (st) return st;
+        c->knots_cap = (int)cap;
+    }
+    c->spectrum = d;
+    c->has_spectrum = true;
+    return NT_OK;
+}
+
+int32_t nt_set_targets(nt_ctx* c, const nt_target* t, int32_t n) {
+    if (!c || n < 0 || n > NT_MAX_TARGETS || (n > 0 && !t)) {
+        snprintf(g_err, sizeof g_err, "targets: 0..%d entries", NT_MAX_TARGETS);
+        return NT_ERR_ARG;
+    }
+    std::vector<float4> tg(n);
+    std::vector<uint32_t> ids(n);
+    for (int i = 0; i < n; ++i) {
+        if (!(t[i].radius > 0.0f)) { snprintf(g_err, sizeof g_err, "target %d: radius must be > 0", i); return NT_ERR_ARG; }
+        tg[i] = make_float4(t[i].position[0], t[i].position[1], t[i].position[2], t[i].radius * t[i].radius);
+        ids[i] = t[i].module_id;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    int64_t cap = c->targets_cap, cap2 = c->targets_cap;
+    int32_t st = grow(&c->d_targets, n, &cap);
+    if (st) return st;
+    st = grow(&c->d_module_ids, n, &cap2);
+    if (st) return st;
+    c->targets_cap = (int)std::min(cap, cap2);
+    if (n) {
+        HIP_TRY(hipMemcpy(c->d_targets, tg.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(c->d_module_ids, ids.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    }
+    c->n_targets = n;
+    c->has_targets = true;
+    return NT_OK;
+}
+
+int32_t nt_set_sources(nt_ctx* c, const nt_source* s, int32_t n) {
+    if (!c || n < 0 || (n > 0 && !s)) return NT_ERR_ARG;
+    std::vector<DevSource> ds(n);
+    std::vector<int64_t> offs(n + 1);
+    offs[0] = 0;
+    for (int i = 0; i < n; ++i) {
+        const nt_source& a = s[i];
+        if (a.photons < 0 || a.kind < NT_SRC_POINTLIKE_ISOTROPIC || a.kind > NT_SRC_POINTLIKE_CHERENKOV) {
+            snprintf(g_err, sizeof g_err, "source %d: photons must be >= 0 and kind known", i);
+            return NT_ERR_ARG;
+        }
+        DevSource& d = ds[i];
+        memset(&d, 0, sizeof d);
+        d.kind = a.kind;
+        d.index = (uint32_t)i;
+        d.offset = offs[i];
+        d.photons = a.photons;
+        for (int k = 0; k < 3; ++k) { d.pos[k] = a.position[k]; d.dir[k] = a.direction[k]; }
+        d.t0 = a.time;
+        d.t1 = a.time_end;
+        d.angle = a.angle;
+        if (a.kind == NT_SRC_EXTENDED_CHERENKOV) {
+            if (!(a.long_a > 1.0f && a.long_b > 0.0f && a.long_lrad > 0.0f)) {
+                snprintf(g_err, sizeof g_err, "source %d: longitudinal parameters need a > 1, b > 0, lrad > 0", i);
+                return NT_ERR_ARG;
+            }
+            // rand_gamma(T(a), T(1/b)) set-up, utils.jl:269-270 and photon_prop_cuda.jl:168
+            d.gamma_d = (float)((double)a.long_a - 1.0 / 3.0);
+            d.gamma_c = 1.0f / sqrtf(9.0f * d.gamma_d);
+            d.gamma_scale = 1.0f / a.long_b;
+            d.lrad = a.long_lrad;
+        }
+        offs[i + 1] = offs[i] + a.photons;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    int64_t cap = c->sources_cap, cap2 = c->sources_cap ? c->sources_cap + 1 : 0;
+    int32_t st = grow(&c->d_sources, std::max(n, 1), &cap);
+    if (st) return st;
+    st = grow(&c->d_offsets, (int64_t)n + 1, &cap2);
+    if (st) return st;
+    c->sources_cap = (int)std::min(cap, cap2 - 1);
+    if (n) HIP_TRY(hipMemcpy(c->d_sources, ds.data(), sizeof(DevSource) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_offsets, offs.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice));
+    c->n_sources = n;
+    c->total = offs[n];
+    c->has_sources = true;
+    return NT_OK;
+}
+
+int32_t nt_launch_shape(nt_ctx* c, int32_t* blocks, int32_t* threads) {
+    if (!c) return NT_ERR_ARG;
+    if (blocks) *blocks = c->blocks;
+    if (threads) *threads = kThreads;
+    return NT_OK;
+}
+
+int32_t nt_launch(nt_ctx* c, uint64_t seed, const nt_hits* out, int64_t* d_counters, void* stream) {
+    if (!c || !out || !d_counters || out->capacity < 0) return NT_ERR_ARG;
+    if (out->capacity > 0 && (!out->position || !out->direction || !out->initial_direction ||
+                              !out->wavelength || !out->time || !out->dist_travelled || !out->module_id))
+        return NT_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    HitCols h{out->position, out->direction, out->initial_direction, out->wavelength, out->time,
+              out->dist_travelled, out->module_id, out->source_index, out->photon_id, out->capacity};
+    return launch(c, seed, h, d_counters, (hipStream_t)stream);
+}
+
+int32_t nt_propagate(nt_ctx* c, uint64_t seed, const nt_hits* out, int64_t* n_hits, int64_t* n_sim,
+                     int64_t* n_dropped) {
+    if (!c || !out || out->capacity < 0 || !n_hits || !n_sim) return NT_ERR_ARG;
+    if (out->capacity > 0 && (!out->position || !out->direction || !out->initial_direction ||
+                              !out->wavelength || !out->time || !out->dist_travelled || !out->module_id))
+        return NT_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    const int64_t cap = out->capacity;
+    if (cap > c->hits_cap) {
+        free_hits(c->d_hits);
+        c->hits_cap = 0;
+        HitCols& h = c->d_hits;
+        const int64_t n = std::max<int64_t>(cap, 1);
+        HIP_TRY(hipMalloc((void**)&h.pos, 12 * n));
+        HIP_TRY(hipMalloc((void**)&h.dir, 12 * n));
+        HIP_TRY(hipMalloc((void**)&h.idir, 12 * n));
+        HIP_TRY(hipMalloc((void**)&h.wl, 4 * n));
+        HIP_TRY(hipMalloc((void**)&h.time, 4 * n));
+        HIP_TRY(hipMalloc((void**)&h.dist, 4 * n));
+        HIP_TRY(hipMalloc((void**)&h.module_id, 2 * n));
+        HIP_TRY(hipMalloc((void**)&h.source_index, 4 * n));
+        HIP_TRY(hipMalloc((void**)&h.photon_id, 8 * n));
+        c->hits_cap = n;
+    }
+    HitCols h = c->d_hits;
+    h.capacity = cap;
+    int32_t st = launch(c, seed, h, c->d_counters, c->stream);
+    if (st) return st;
+    int64_t cnt[4] = {0, 0, 0, 0};
+    HIP_TRY(hipMemcpyAsync(cnt, c->d_counters, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const int64_t stored = std::min(cnt[0], cap);
+    if (stored > 0) {
+        HIP_TRY(hipMemcpyAsync(out->position, h.pos, 12 * stored, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(out->direction, h.dir, 12 * stored, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(out->initial_direction, h.idir, 12 * stored, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(out->wavelength, h.wl, 4 * stored, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(out->time, h.time, 4 * stored, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(out->dist_travelled, h.dist, 4 * stored, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(out->module_id, h.module_id, 2 * stored, hipMemcpyDeviceToHost, c->stream));
+        if (out->source_index)
+            HIP_TRY(hipMemcpyAsync(out->source_index, h.source_index, 4 * stored, hipMemcpyDeviceToHost, c->stream));
+        if (out->photon_id)
+            HIP_TRY(hipMemcpyAsync(out->photon_id, h.photon_id, 8 * stored, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    *n_hits = stored;
+    *n_sim = cnt[1];
+    if (n_dropped) *n_dropped = cnt[0] - stored;
+    return cnt[0] > cap ? NT_ERR_HIT_OVERFLOW : NT_OK;
+}
+
+int32_t nt_eval(nt_ctx* c, int32_t fn, const float* in, int32_t iw, float* out, int32_t ow, int64_t n) {
+    if (!c || fn < 0 || fn >= NT_FN_COUNT || n < 0 || iw < 1 || ow < 1 || (n > 0 && (!in || !out)))
+        return NT_ERR_ARG;
+    if (n == 0) return NT_OK;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    float *d_in = nullptr, *d_out = nullptr;
+    HIP_TRY(hipMalloc((void**)&d_in, sizeof(float) * iw * n));
+    hipError_t e = hipMalloc((void**)&d_out, sizeof(float) * ow * n);
+    if (e != hipSuccess) { (void)hipFree(d_in); return fail_hip(e, "hipMalloc"); }
+    e = hipMemcpy(d_in, in, sizeof(float) * iw * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        const int tpb = 256;
+        const int64_t nb = (n + tpb - 1) / tpb;
+        const float* kn = c->d_knots;
+        Spectrum sp = c->spectrum;
+        if (!c->has_spectrum) { sp = Spectrum{}; sp.kind = NT_SPEC_MONOCHROMATIC; sp.wl = 400.0f; }
+        if (!kn) kn = d_in;  // never read unless a table spectrum is set
+        hipLaunchKernelGGL(eval_kernel, dim3((unsigned)nb), dim3(tpb), 0, c->stream, fn, d_in, iw, d_out, ow, n,
+                           c->medium, sp, kn);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipMemcpy(out, d_out, sizeof(float) * ow * n, hipMemcpyDeviceToHost);
+    }
+    (void)hipFree(d_in);
+    (void)hipFree(d_out);
+    if (e != hipSuccess) return fail_hip(e, "nt_eval");
+    return NT_OK;
+}
+
+}  // extern "C"

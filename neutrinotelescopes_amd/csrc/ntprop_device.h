@@ -1,0 +1,234 @@
+// ntprop_device.h — FP32 device functions of the photon-propagation kernel
+// (gfx950).  Each function states the reference function it implements
+// (paths relative to the reference repository root).  The library is built
+// with -ffp-contract=off: the only fused multiply-adds are the explicit
+// __builtin_fmaf calls, at the places where the reference calls fma.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/ntprop.h"
+
+namespace ntp {
+
+constexpr float kTwoPi = 6.2831855f;            // Float32(2π)
+constexpr float kCVacMNs = 0.299792458f;        // m/ns
+constexpr uint32_t kGammaBlock = 0x80000000u;   // first Philox block of Gamma trials
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11).  One call yields four 32-bit words.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint4 philox(uint4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c;
+}
+
+// [0,1) and (0,1] uniforms carrying 24 random bits
+__device__ __forceinline__ float u01(uint32_t x) { return (float)(x >> 8) * 0x1p-24f; }
+__device__ __forceinline__ float u01_oc(uint32_t x) { return (float)((x >> 8) + 1u) * 0x1p-24f; }
+
+struct Rng {
+    uint32_t k0, k1;        // key = seed
+    uint32_t plo, phi, src; // counter words 1..3
+    __device__ __forceinline__ uint4 block(uint32_t b) const {
+        return philox(make_uint4(b, plo, phi, src), k0, k1);
+    }
+};
+
+__device__ __forceinline__ float clampf(float x, float lo, float hi) {
+    return fminf(fmaxf(x, lo), hi);
+}
+
+// ---------------------------------------------------------------------------
+// Medium: medium.jl
+// ---------------------------------------------------------------------------
+struct Medium {
+    float a01, a2, a3, a4;    // Quan-Fry coefficients
+    float vcs, vcl;           // particle volume concentrations (ppm)
+    float g;                  // mean scattering cosine
+    float sl_fraction;        // Simplified-Liu mixture weight (0 = reference)
+    float sca_len_scale;      // scattering-length multiplier (1 = reference)
+    int dispersion_mode;
+    int max_steps;
+};
+
+// refractive_index_fry, medium.jl:234-244
+__device__ __forceinline__ float refractive_index(float wl, const Medium& m) {
+    const float x = 1.0f / wl;
+    const float x2 = x * x;
+    return __builtin_fmaf(x, m.a2, m.a01) + __builtin_fmaf(x2, m.a3, x2 * x * m.a4);
+}
+
+// group_velocity, medium.jl:313-318, with dispersion_fry, medium.jl:293-298.
+// Reference mode: the dispersion destructures (a01, a2, a3) as (a2, a3, a4)
+// and the group index uses λ0 = n·λ.
+__device__ __forceinline__ float group_velocity(float wl, const Medium& m) {
+    const float n = refractive_index(wl, m);
+    const bool ref = m.dispersion_mode == NT_DISPERSION_REF;
+    const float c0 = ref ? m.a01 : m.a2;
+    const float c1 = ref ? m.a2 : m.a3;
+    const float c2 = ref ? m.a3 : m.a4;
+    const float x = 1.0f / wl;
+    const float s = (c0 + 2.0f * x * c1) + 3.0f * (x * x) * c2;
+    const float disp = (s * -1.0f) / (wl * wl);
+    const float lambda0 = ref ? n * wl : wl;
+    return kCVacMNs / (n - lambda0 * disp);
+}
+
+// _sca_len_part_conc, medium.jl:331-347
+__device__ __forceinline__ float scattering_length(float wl, const Medium& m) {
+    const float x = 550.0f / wl;
+    const float c = (0.0017f * powf(x, 4.3f) + 1.34f * m.vcs * powf(x, 1.7f)) +
+                    0.312f * m.vcl * powf(x, 0.3f);
+    const float l = 1.0f / c;
+    return m.sca_len_scale == 1.0f ? l : l * m.sca_len_scale;
+}
+
+// ---------------------------------------------------------------------------
+// Directions: utils.jl, photon_prop_cuda.jl
+// ---------------------------------------------------------------------------
+// sph_to_cart, utils.jl:105-115
+__device__ __forceinline__ float3 sph_to_cart(float theta, float phi) {
+    float st, ct, sp, cp;
+    sincosf(theta, &st, &ct);
+    sincosf(phi, &sp, &cp);
+    return make_float3(cp * st, sp * st, ct);
+}
+
+// rot_from_ez_fast, utils.jl:228-243 (the |a_z| == 1 branch as written)
+__device__ __forceinline__ float3 rot_from_ez(float3 a, float3 o) {
+    if (fabsf(a.z) == 1.0f) return make_float3(o.x, copysignf(o.y, a.z), copysignf(o.z, a.z));
+    const float a1sq = a.x * a.x, a2sq = a.y * a.y;
+    const float fact = (1.0f - a.z) / (a2sq + a1sq);
+    const float a1a2 = a.x * a.y;
+    const float x = (__builtin_fmaf(-a1sq, fact, 1.0f) * o.x - a1a2 * fact * o.y) + a.x * o.z;
+    const float y = ((-a1a2) * fact * o.x + __builtin_fmaf(-a2sq, fact, 1.0f) * o.y) + a.y * o.z;
+    const float z = ((-a.x) * o.x - a.y * o.y) + a.z * o.z;
+    return make_float3(x, y, z);
+}
+
+// cuda_hg_scattering_func, photon_prop_cuda.jl:70-77
+__device__ __forceinline__ float hg_cos(float g, float eta) {
+    const float t = __builtin_fmaf(-g, g, 1.0f) / __builtin_fmaf(g, __builtin_fmaf(2.0f, eta, -1.0f), 1.0f);
+    const float c = (1.0f / (2.0f * g)) * (__builtin_fmaf(g, g, 1.0f) - t * t);
+    return clampf(c, -1.0f, 1.0f);
+}
+
+// Simplified-Liu: cosθ = 2 ξ^{1/(1+α)} − 1, α = 2g/(1−g)  (extension)
+__device__ __forceinline__ float sl_cos(float g, float xi) {
+    const float alpha = 2.0f * g / (1.0f - g);
+    return clampf(2.0f * powf(xi, 1.0f / (1.0f + alpha)) - 1.0f, -1.0f, 1.0f);
+}
+
+// update_direction, photon_prop_cuda.jl:204-222
+__device__ __forceinline__ float3 update_direction(float3 dir, float eta, float u_phi, float u_mix,
+                                                   const Medium& m) {
+    const float c = (u_mix < m.sl_fraction) ? sl_cos(m.g, eta) : hg_cos(m.g, eta);
+    const float s = sqrtf(__builtin_fmaf(-c, c, 1.0f));
+    const float phi = 0.0f + u_phi * kTwoPi;
+    float sp, cp;
+    sincosf(phi, &sp, &cp);
+    return rot_from_ez(dir, make_float3(cp * s, sp * s, c));
+}
+
+// initialize_direction_isotropic, photon_prop_cuda.jl:100-104
+__device__ __forceinline__ float3 iso_dir(float u_cos, float u_phi) {
+    const float theta = acosf(-1.0f + u_cos * 2.0f);
+    const float phi = 0.0f + u_phi * kTwoPi;
+    return sph_to_cart(theta, phi);
+}
+
+// sample_cherenkov_direction (photon_prop_cuda.jl:107-125) with
+// cherenkov_angle (medium.jl:309-311) and sample_cherenkov_track_direction
+// (utils.jl:249-260)
+__device__ __forceinline__ float3 cherenkov_dir(float wl, float u_phi, float u_tcos, float u_tphi,
+                                                float3 src_dir, const Medium& m) {
+    const float theta_c = acosf(1.0f / refractive_index(wl, m));
+    const float phi = 0.0f + u_phi * kTwoPi;
+    const float3 ph = sph_to_cart(theta_c, phi);
+    const float a = 0.39f, b = 2.61f;
+    const float I = 1.0f - expf(-b * powf(2.0f, a));
+    const float t = -logf(1.0f - u_tcos * I) / b;
+    const float ct = fmaxf(1.0f - powf(t, 1.0f / a), -1.0f);
+    const float3 track = sph_to_cart(acosf(ct), kTwoPi * u_tphi);
+    return rot_from_ez(src_dir, rot_from_ez(track, ph));
+}
+
+// check_intersection(::Spherical), photon_prop_cuda.jl:233-271, split in two:
+// the discriminant b for every target, the root only where b >= 0.
+__device__ __forceinline__ float isect_b(float3 p, float3 d, float4 t, float& a_out) {
+    const float dp0 = p.x - t.x, dp1 = p.y - t.y, dp2 = p.z - t.z;
+    const float a = (d.x * dp0 + d.y * dp1) + d.z * dp2;
+    const float pp = (dp0 * dp0 + dp1 * dp1) + dp2 * dp2;
+    a_out = a;
+    return __builtin_fmaf(a, a, -pp + t.w);   // t.w = radius^2
+}
+
+// ---------------------------------------------------------------------------
+// Spectrum: initialize_wavelength, photon_prop_cuda.jl:128-130
+// ---------------------------------------------------------------------------
+struct Spectrum {
+    int kind, addressing, n;
+    float wl, lo, hi;
+};
+
+// A 1-D texture with normalized coordinates, linear filtering and clamp
+// addressing (spectrum.jl:102-103) samples at x = u·N − 0.5 and blends the two
+// neighbouring texels, indices clamped to [0, N−1].
+__device__ __forceinline__ float sample_spectrum(const Spectrum& s, const float* knots, float u) {
+    if (s.kind == NT_SPEC_MONOCHROMATIC) return s.wl;
+    if (s.kind == NT_SPEC_INV_LAMBDA2) return 1.0f / ((1.0f - u) / s.lo + u / s.hi);
+    if (s.addressing == NT_TEX_ENDPOINT) {
+        const float x = u * (float)(s.n - 1);
+        const float fi = floorf(x);
+        const int i = (int)fi;
+        if (i >= s.n - 1) return knots[s.n - 1];
+        const float al = x - fi;
+        return (1.0f - al) * knots[i] + al * knots[i + 1];
+    }
+    const float x = u * (float)s.n - 0.5f;
+    const float fi = floorf(x);
+    float al = x - fi;
+    if (s.addressing == NT_TEX_TEXEL_CENTER_CLAMP_Q8) al = rintf(al * 256.0f) * (1.0f / 256.0f);
+    const int i0 = (int)fi;
+    const int j0 = min(max(i0, 0), s.n - 1);
+    const int j1 = min(max(i0 + 1, 0), s.n - 1);
+    return (1.0f - al) * knots[j0] + al * knots[j1];
+}
+
+// ---------------------------------------------------------------------------
+// rand_gamma, utils.jl:267-294 (Marsaglia-Tsang).  Trial t draws Philox block
+// kGammaBlock + t: Box-Muller normal from words 0,1, acceptance uniform from
+// word 2.  The squeeze of utils.jl:286 can never accept (1 − fma(0.0331, x⁴, 1)
+// ≤ 0 < u) and is omitted; the log test alone decides, as in the reference.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float rand_gamma(float d, float c, float scale, const Rng& rng,
+                                            int* trials) {
+    for (uint32_t t = 0;; ++t) {
+        const uint4 r = rng.block(kGammaBlock + t);
+        float sn, cs;
+        sincosf(kTwoPi * u01(r.y), &sn, &cs);
+        const float x = sqrtf(-2.0f * logf(u01_oc(r.x))) * cs;
+        float v = __builtin_fmaf(c, x, 1.0f);
+        if (v <= 0.0f) continue;
+        v = v * v * v;
+        const float u = u01_oc(r.z);
+        const float xsq = x * x;
+        if (logf(u) < 0.5f * xsq + d * ((1.0f - v) + logf(v))) {
+            if (trials) *trials = (int)t + 1;
+            return d * v * scale;
+        }
+    }
+}
+
+}  // namespace ntp

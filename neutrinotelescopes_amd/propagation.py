@@ -1,0 +1,267 @@
+"""Host driver of the propagation path: mirror of the public API of module
+PhotonPropagationCuda (src/photon_propagation/photon_prop_cuda.jl).
+
+    setup = PhotonPropSetup(sources, targets, medium, spectrum)
+    hits = propagate_photons(setup)                   # photon_prop_cuda.jl:804-817
+    hits, n_sim = run_photon_prop_no_local_cache(...)  # photon_prop_cuda.jl:726-800
+    calc_total_weight(hits, setup)                     # photon_prop_cuda.jl:850-882
+    calc_time_residual(hits, setup)                    # photon_prop_cuda.jl:906-924
+    make_hits_from_photons(hits, setup, orientation)   # photon_prop_cuda.jl:884-904
+
+All propagation runs in libntprop.so (HIP, gfx950) through the C ABI.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi
+from .detection import DetectionSphere, MultiPMTDetector, area_acceptance, check_pmt_hit, p_one_pmt_acc
+from .medium import WaterProperties, absorption_length, refractive_index
+from .spectrum import Spectrum
+
+C_VAC_M_NS = 0.299792458
+
+
+@dataclass
+class PhotonPropSetup:
+    """photon_prop_cuda.jl:34-45."""
+    sources: list
+    targets: list
+    medium: WaterProperties
+    spectrum: Spectrum
+    seed: int = 0
+
+    def __post_init__(self):
+        if not isinstance(self.sources, (list, tuple)):
+            self.sources = [self.sources]
+        if not isinstance(self.targets, (list, tuple)):
+            self.targets = [self.targets]
+
+
+HIT_COLUMNS = ("position", "direction", "initial_direction", "wavelength", "time",
+               "dist_travelled", "module_id", "source_index", "photon_id")
+
+
+@dataclass
+class PhotonHits:
+    """Columns of PhotonHit (photon_prop_cuda.jl:391-399) as numpy arrays, plus
+    the source index and in-source photon index of each hit."""
+    position: np.ndarray
+    direction: np.ndarray
+    initial_direction: np.ndarray
+    wavelength: np.ndarray
+    time: np.ndarray
+    dist_travelled: np.ndarray
+    module_id: np.ndarray
+    source_index: np.ndarray
+    photon_id: np.ndarray
+    n_simulated: int = 0
+    n_dropped: int = 0
+    extra: dict = field(default_factory=dict)
+
+    @classmethod
+    def empty(cls, capacity: int) -> "PhotonHits":
+        n = int(capacity)
+        return cls(np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32),
+                   np.zeros((n, 3), np.float32), np.zeros(n, np.float32), np.zeros(n, np.float32),
+                   np.zeros(n, np.float32), np.zeros(n, np.uint16), np.zeros(n, np.uint32),
+                   np.zeros(n, np.uint64))
+
+    def __len__(self):
+        return len(self.wavelength)
+
+    def truncated(self, n: int) -> "PhotonHits":
+        return PhotonHits(*(getattr(self, c)[:n] for c in HIT_COLUMNS), n_simulated=self.n_simulated,
+                          n_dropped=self.n_dropped)
+
+    def select(self, mask) -> "PhotonHits":
+        out = PhotonHits(*(getattr(self, c)[mask] for c in HIT_COLUMNS),
+                         n_simulated=self.n_simulated, n_dropped=self.n_dropped)
+        out.extra = {k: v[mask] for k, v in self.extra.items()}
+        return out
+
+    def sorted(self) -> "PhotonHits":
+        """Deterministic order: by (source index, photon index)."""
+        order = np.lexsort((self.photon_id, self.source_index))
+        return self.select(order)
+
+    def to_abi(self) -> _abi.nt_hits:
+        h = _abi.nt_hits()
+        for c in HIT_COLUMNS:
+            setattr(h, c, getattr(self, c).ctypes.data)
+        h.capacity = len(self)
+        return h
+
+    def to_dataframe(self):
+        import pandas as pd
+        d = {}
+        for c in ("position", "direction", "initial_direction"):
+            a = getattr(self, c)
+            for i, ax in enumerate("xyz"):
+                d[f"{c}_{ax}"] = a[:, i]
+        for c in ("wavelength", "time", "dist_travelled", "module_id", "source_index", "photon_id"):
+            d[c] = getattr(self, c)
+        d.update(self.extra)
+        return pd.DataFrame(d)
+
+
+class PropagationEngine:
+    """A libntprop context: device-resident medium, spectrum, targets and sources."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _abi.load_library()
+        h = C.c_void_p()
+        _abi.check(self.lib.nt_create(int(device), C.byref(h)), "nt_create")
+        self.handle = h
+        self._keep = []
+
+    def close(self):
+        if self.handle:
+            self.lib.nt_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_medium(self, medium: WaterProperties):
+        m = medium.to_abi()
+        _abi.check(self.lib.nt_set_medium(self.handle, C.byref(m)), "nt_set_medium")
+
+    def set_spectrum(self, spectrum: Spectrum):
+        s = spectrum.to_abi()
+        _abi.check(self.lib.nt_set_spectrum(self.handle, C.byref(s)), "nt_set_spectrum")
+
+    def set_targets(self, targets):
+        arr = _abi.as_c_array(_abi.nt_target, [t.to_abi() for t in targets])
+        _abi.check(self.lib.nt_set_targets(self.handle, arr, len(targets)), "nt_set_targets")
+
+    def set_sources(self, sources):
+        arr = _abi.as_c_array(_abi.nt_source, [s.to_abi() for s in sources])
+        _abi.check(self.lib.nt_set_sources(self.handle, arr, len(sources)), "nt_set_sources")
+
+    def setup(self, setup: PhotonPropSetup):
+        self.set_medium(setup.medium)
+        self.set_spectrum(setup.spectrum)
+        self.set_targets(setup.targets)
+        self.set_sources(setup.sources)
+
+    def launch_shape(self):
+        b, t = C.c_int32(), C.c_int32()
+        _abi.check(self.lib.nt_launch_shape(self.handle, C.byref(b), C.byref(t)), "nt_launch_shape")
+        return b.value, t.value
+
+    def propagate(self, seed: int = 0, capacity: int = 1 << 20, raise_on_overflow: bool = True):
+        """Synchronous propagation into host columns (nt_propagate)."""
+        hits = PhotonHits.empty(capacity)
+        h = hits.to_abi()
+        nh, ns, nd = C.c_int64(), C.c_int64(), C.c_int64()
+        st = self.lib.nt_propagate(self.handle, C.c_uint64(seed & (2**64 - 1)), C.byref(h), C.byref(nh),
+                                   C.byref(ns), C.byref(nd))
+        if st != _abi.NT_OK and not (st == _abi.NT_ERR_HIT_OVERFLOW and not raise_on_overflow):
+            raise _abi.NTError(st, "nt_propagate")
+        out = hits.truncated(nh.value)
+        out.n_simulated = ns.value
+        out.n_dropped = nd.value
+        return out
+
+    def eval(self, fn: int, inputs: np.ndarray, out_width: int) -> np.ndarray:
+        """Run one device function of the kernel on rows of `inputs`."""
+        x = np.ascontiguousarray(inputs, dtype=np.float32)
+        if x.ndim == 1:
+            x = x[:, None]
+        y = np.zeros((x.shape[0], out_width), dtype=np.float32)
+        _abi.check(self.lib.nt_eval(self.handle, int(fn), x.ctypes.data, x.shape[1], y.ctypes.data,
+                                    out_width, x.shape[0]), "nt_eval")
+        return y
+
+
+_ENGINES: dict = {}
+
+
+def get_engine(device: int = 0) -> PropagationEngine:
+    if device not in _ENGINES:
+        _ENGINES[device] = PropagationEngine(device)
+    return _ENGINES[device]
+
+
+def run_photon_prop_no_local_cache(sources, targets, medium, spectrum, *, seed: int = 0,
+                                   capacity: int = 1 << 22, device: int = 0):
+    """photon_prop_cuda.jl:726-800: (hits, n_photons_simulated).  One launch
+    covers all sources; the hit capacity is `capacity` (the reference reserves
+    0.7 of device memory on every call, photon_prop_cuda.jl:732-736)."""
+    eng = get_engine(device)
+    eng.setup(PhotonPropSetup(list(sources) if isinstance(sources, (list, tuple)) else [sources],
+                              list(targets) if isinstance(targets, (list, tuple)) else [targets],
+                              medium, spectrum))
+    hits = eng.propagate(seed=seed, capacity=capacity)
+    return hits, hits.n_simulated
+
+
+def propagate_photons(setup: PhotonPropSetup, *, capacity: int = 1 << 22, device: int = 0,
+                      single_target_swap: bool = True) -> PhotonHits:
+    """photon_prop_cuda.jl:804-817.
+
+    With one target the reference takes its single-target kernel, which stores
+    the initial direction in the `direction` column and the final direction in
+    `initial_direction` (photon_prop_cuda.jl:590-592); `single_target_swap`
+    (default True) reproduces that column order."""
+    hits, _ = run_photon_prop_no_local_cache(setup.sources, setup.targets, setup.medium,
+                                             setup.spectrum, seed=setup.seed, capacity=capacity,
+                                             device=device)
+    if single_target_swap and len(setup.targets) == 1:
+        hits.direction, hits.initial_direction = hits.initial_direction, hits.direction
+    return hits
+
+
+def _target_map(setup: PhotonPropSetup):
+    return {int(t.module_id): t for t in setup.targets}
+
+
+def calc_total_weight(hits: PhotonHits, setup: PhotonPropSetup) -> PhotonHits:
+    """photon_prop_cuda.jl:850-882: area_acc × wl_acc × exp(−dist/L_abs(λ)),
+    added as extra columns area_acc, abs_weight, wl_acc, ref_ix, total_weight."""
+    tmap = _target_map(setup)
+    area = np.array([area_acceptance(None, tmap[int(m)]) for m in hits.module_id], dtype=np.float64)
+    abs_len = absorption_length(hits.wavelength, setup.medium)
+    abs_weight = np.exp(-hits.dist_travelled / abs_len).astype(np.float64)
+    wl_acc = p_one_pmt_acc(hits.wavelength)
+    hits.extra["area_acc"] = area
+    hits.extra["abs_weight"] = abs_weight
+    hits.extra["wl_acc"] = wl_acc
+    hits.extra["ref_ix"] = refractive_index(hits.wavelength, setup.medium)
+    hits.extra["total_weight"] = area * wl_acc * abs_weight
+    return hits
+
+
+def calc_time_residual(hits: PhotonHits, setup: PhotonPropSetup) -> PhotonHits:
+    """photon_prop_cuda.jl:906-924: t − (|src₁ − target| − r)/(c/n(800 nm)),
+    with the first source's position for every hit, as in the reference."""
+    tmap = _target_map(setup)
+    src = np.asarray(setup.sources[0].position, dtype=np.float32)
+    n800 = float(refractive_index(np.float32(800.0), setup.medium))
+    tres = np.empty(len(hits), dtype=np.float32)
+    for mid in np.unique(hits.module_id):
+        t = tmap[int(mid)]
+        sel = hits.module_id == mid
+        dist = np.float32(np.linalg.norm(src - np.asarray(t.position, dtype=np.float32)))
+        tgeo = (dist - np.float32(t.radius)) / np.float32(C_VAC_M_NS / n800)
+        tres[sel] = hits.time[sel] - tgeo
+    hits.extra["tres"] = tres
+    return hits
+
+
+def make_hits_from_photons(hits: PhotonHits, setup: PhotonPropSetup, orientation=None) -> PhotonHits:
+    """photon_prop_cuda.jl:884-904: keep photons that land on a PMT and add pmt_id."""
+    tmap = _target_map(setup)
+    pmt = np.zeros(len(hits), dtype=np.int64)
+    for mid in np.unique(hits.module_id):
+        sel = hits.module_id == mid
+        pmt[sel] = check_pmt_hit(hits.position[sel], tmap[int(mid)], orientation)
+    hits.extra["pmt_id"] = pmt
+    return hits.select(pmt > 0)

@@ -1,0 +1,58 @@
+"""The C-ABI library loads and exports every function include/ntprop.h declares
+(no device calls: runs without a GPU)."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import pytest
+
+from neutrinotelescopes_amd import _abi
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_functions():
+    text = (ROOT / "include" / "ntprop.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:int32_t|void|const char\*)\s+\**(nt_\w+)\s*\(", text, re.M)))
+
+
+def test_header_declares_entry_points():
+    fns = declared_functions()
+    for name in ("nt_create", "nt_destroy", "nt_set_medium", "nt_set_spectrum", "nt_set_targets",
+                 "nt_set_sources", "nt_propagate", "nt_launch", "nt_eval"):
+        assert name in fns
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _abi.load_library()
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    # every declared function has a ctypes signature in the binding
+    assert set(declared_functions()) <= set(_abi.SIGNATURES)
+
+
+def test_abi_version_and_status_strings():
+    lib = _abi.load_library()
+    assert lib.nt_abi_version() == 1
+    for st in range(0, 8):
+        assert lib.nt_status_string(st)
+
+
+def test_struct_layouts():
+    assert C.sizeof(_abi.nt_source) == 64
+    assert C.sizeof(_abi.nt_medium) == 48
+    assert C.sizeof(_abi.nt_target) == 160
+    assert _abi.nt_hits.capacity.offset == 9 * 8
+
+
+def test_library_missing_fails_loudly(tmp_path):
+    with pytest.raises(RuntimeError):
+        _abi.load_library(tmp_path / "libntprop.so")
+
+
+def test_null_arguments_rejected_without_device():
+    lib = _abi.load_library()
+    assert lib.nt_set_medium(None, None) == _abi.NT_ERR_ARG
+    assert lib.nt_create(0, None) == _abi.NT_ERR_ARG
+    assert lib.nt_eval(None, 0, None, 1, None, 1, 1) == _abi.NT_ERR_ARG

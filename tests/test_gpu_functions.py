@@ -1,0 +1,254 @@
+"""Device functions of the propagation kernel (run through nt_eval) against
+the oracle's FP32 restatement and against FP64 evaluations.
+
+Error metric: |y_gpu - y_f64| in FP32 ulps of a per-function scale, the
+magnitude at which the reference formula itself rounds:
+  directions (unit vectors)  scale 1
+  HG / SL cosine             scale (1 + g^2 + t^2) / (2g), the largest term
+  n, c_grp, L_sca            scale |y|
+  intersection distance d    error bound ulp(|a|) + ulp(max(a^2, |p-c|^2)) / (2 sqrt(b))
+"""
+import ctypes as C
+import json
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import _oracle as O
+import neutrinotelescopes_amd as nt
+from neutrinotelescopes_amd import _abi
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).parent / "golden"
+ULP_MAX = 2.0
+
+
+def ulp32(x):
+    x = np.abs(np.asarray(x, dtype=np.float64)).astype(np.float32)
+    return np.spacing(x).astype(np.float64)
+
+
+def errs_in_ulps(y, ref, scale):
+    return np.abs(y.astype(np.float64) - ref) / ulp32(np.maximum(np.abs(ref), scale))
+
+
+@pytest.fixture(scope="module")
+def medium_engine(engine):
+    med = nt.make_cascadia_medium_properties(0.95)
+    engine.set_medium(med)
+    engine.set_spectrum(nt.Monochromatic(400.0))
+    return engine, med
+
+
+def test_philox_device_bit_exact(medium_engine):
+    eng, _ = medium_engine
+    kat = json.loads((GOLDEN / "philox4x32_10_kat.json").read_text())
+    rows, want = [], []
+    for v in kat["random123"] + kat["rocrand"]:
+        rows.append([int(x, 16) for x in v["ctr"] + v["key"]])
+        want.append([int(x, 16) for x in v["out"]])
+    rng = np.random.default_rng(0)
+    for _ in range(2000):
+        r = [int(x) for x in rng.integers(0, 2 ** 32, 6, dtype=np.uint64)]
+        rows.append(r)
+        want.append(O.philox(r[:4], r[4:]))
+    x = np.array(rows, dtype=np.uint32).view(np.float32)
+    y = eng.eval(_abi.NT_FN_PHILOX, x, 4).view(np.uint32)
+    assert np.array_equal(y, np.array(want, dtype=np.uint32))
+
+
+def test_intersection_device(medium_engine, oracle):
+    eng, _ = medium_engine
+    kats = json.loads((GOLDEN / "runtests_kats.json").read_text())["intersection"]
+    tp, r = kats["target"]["position"], kats["target"]["radius"]
+    rows = []
+    for c in kats["cases"]:
+        d = np.array(c["dir"], float)
+        d = d / np.linalg.norm(d)
+        rows.append(list(c["pos"]) + list(d) + [c["step"]] + list(tp) + [r])
+    y = eng.eval(_abi.NT_FN_INTERSECT, np.array(rows), 2)
+    assert [bool(v) for v in y[:, 0]] == [c["hit"] for c in kats["cases"]]
+    # random rays near a sphere: bit-exact with the oracle's FP32 restatement
+    rng = np.random.default_rng(1)
+    n = 20000
+    c = rng.uniform(-20, 20, (n, 3)).astype(np.float32)
+    rad = rng.uniform(0.1, 1.0, n).astype(np.float32)
+    u = rng.normal(size=(n, 3)); u /= np.linalg.norm(u, axis=1, keepdims=True)
+    perp = np.cross(u, rng.normal(size=(n, 3))); perp /= np.linalg.norm(perp, axis=1, keepdims=True)
+    D = rng.uniform(1.0, 30.0, n)
+    pos = (c - D[:, None] * u + (rng.uniform(0, 1.3, n) * rad)[:, None] * perp).astype(np.float32)
+    u = u.astype(np.float32)
+    step = rng.uniform(0.5, 60.0, n).astype(np.float32)
+    x = np.concatenate([pos, u, step[:, None], c, rad[:, None]], axis=1)
+    y = eng.eval(_abi.NT_FN_INTERSECT, x, 2)
+    hit_o = np.zeros(n, bool)
+    d_o = np.full(n, np.nan, np.float32)
+    for i in range(n):
+        dd = C.c_float()
+        hit_o[i] = oracle.ntor_intersect(O.fptr(pos[i]), O.fptr(u[i]), float(step[i]), O.fptr(c[i]),
+                                         float(rad[i]), C.byref(dd))
+        if hit_o[i]:
+            d_o[i] = dd.value
+    assert np.array_equal(y[:, 0] > 0, hit_o)
+    assert np.array_equal(y[hit_o, 1], d_o[hit_o])
+    # FP64 evaluation at the same FP32 inputs, with the FP32 error bound of the formula
+    p64, u64, c64, r64 = (a.astype(np.float64) for a in (pos, u, c, rad))
+    dp = p64 - c64
+    a = np.sum(u64 * dp, axis=1)
+    pp = np.sum(dp * dp, axis=1)
+    b = a * a - pp + r64 * r64
+    h = hit_o & (b > 1e-3)
+    d64 = -a[h] - np.sqrt(b[h])
+    bound = ulp32(a[h]) + 4 * ulp32(np.maximum(a[h] ** 2, pp[h])) / (2 * np.sqrt(b[h]))
+    assert np.all(np.abs(y[h, 1] - d64) <= ULP_MAX * bound)
+
+
+@pytest.mark.parametrize("g", [0.5, 0.9, 0.95, 0.99])
+def test_hg_and_sl_cos_device(medium_engine, oracle, g):
+    eng, _ = medium_engine
+    eta = np.random.default_rng(2).random(20000).astype(np.float32)
+    x = np.stack([np.full_like(eta, g), eta], axis=1)
+    y = eng.eval(_abi.NT_FN_HG_COS, x, 1)[:, 0]
+    yo = np.array([oracle.ntor_hg_cos(np.float32(g), e) for e in eta], np.float32)
+    assert np.array_equal(y, yo)                      # pure arithmetic: bit-exact
+    g32 = np.float64(np.float32(g))
+    e64 = eta.astype(np.float64)
+    t = (1 - g32 * g32) / (1 + g32 * (2 * e64 - 1))
+    f64 = np.clip((1 + g32 * g32 - t * t) / (2 * g32), -1, 1)
+    scale = (1 + g32 ** 2 + t ** 2) / (2 * g32)
+    assert errs_in_ulps(y, f64, scale).max() <= ULP_MAX
+    ys = eng.eval(_abi.NT_FN_SL_COS, x, 1)[:, 0]
+    s64 = np.array([oracle.ntor_sl_cos_f64(g32, e) for e in e64])
+    assert errs_in_ulps(ys, s64, 1.0).max() <= ULP_MAX
+
+
+def test_rotation_and_directions_device(medium_engine, oracle):
+    eng, _ = medium_engine
+    rng = np.random.default_rng(3)
+    n = 20000
+    a = rng.normal(size=(n, 3)); a /= np.linalg.norm(a, axis=1, keepdims=True)
+    op = rng.normal(size=(n, 3)); op /= np.linalg.norm(op, axis=1, keepdims=True)
+    a[:5] = [[0, 0, 1], [0, 0, -1], [1, 0, 0], [0, 1, 0], [0.6, 0, 0.8]]
+    a32, op32 = a.astype(np.float32), op.astype(np.float32)
+    y = eng.eval(_abi.NT_FN_ROT_FROM_EZ, np.concatenate([a32, op32], 1), 3)
+    yo = np.zeros_like(y)
+    for i in range(n):
+        oracle.ntor_rot_from_ez(O.fptr(a32[i]), O.fptr(op32[i]), yo[i].ctypes.data_as(C.POINTER(C.c_float)))
+    assert np.array_equal(y, yo)
+    f64 = np.zeros((n, 3))
+    for i in range(n):
+        oracle.ntor_rot_from_ez_f64(O.dptr(a32[i].astype(np.float64)), O.dptr(op32[i].astype(np.float64)),
+                                    f64[i].ctypes.data_as(C.POINTER(C.c_double)))
+    # the rotation's own rounding: terms up to |fact * a1^2| ~ 2 (fact = 1/(1+a_z))
+    fact = np.where(np.abs(a32[:, 2]) < 1, (1 - a32[:, 2].astype(np.float64)) /
+                    np.maximum(a32[:, 0].astype(np.float64) ** 2 + a32[:, 1].astype(np.float64) ** 2, 1e-30), 0)
+    scale = np.maximum(1.0, fact * np.maximum(a32[:, 0] ** 2, a32[:, 1] ** 2))[:, None]
+    assert errs_in_ulps(y, f64, scale).max() <= 4 * ULP_MAX
+    assert np.percentile(errs_in_ulps(y, f64, 1.0), 99) <= ULP_MAX
+
+    th = rng.uniform(0, np.pi, n).astype(np.float32)
+    ph = rng.uniform(0, 2 * np.pi, n).astype(np.float32)
+    y = eng.eval(_abi.NT_FN_SPH_TO_CART, np.stack([th, ph], 1), 3)
+    t64, p64 = th.astype(np.float64), ph.astype(np.float64)
+    f64 = np.stack([np.cos(p64) * np.sin(t64), np.sin(p64) * np.sin(t64), np.cos(t64)], 1)
+    assert errs_in_ulps(y, f64, 1.0).max() <= ULP_MAX
+
+    u = rng.random((n, 2)).astype(np.float32)
+    y = eng.eval(_abi.NT_FN_ISO_DIR, u, 3)
+    theta = np.arccos((np.float32(-1) + u[:, 0] * np.float32(2)).astype(np.float64))
+    phi = (u[:, 1] * np.float32(6.2831855)).astype(np.float64)
+    f64 = np.stack([np.cos(phi) * np.sin(theta), np.sin(phi) * np.sin(theta), np.cos(theta)], 1)
+    # acos is ill-conditioned near ±1: sinθ error is bounded by ulp(cosθ)/sinθ
+    sin_t = np.maximum(np.sin(theta), 1e-3)[:, None]
+    assert np.all(np.abs(y - f64) <= ULP_MAX * (ulp32(1.0) + ulp32(1.0) / sin_t))
+
+
+def test_medium_functions_device(medium_engine, oracle):
+    eng, med = medium_engine
+    wl = np.linspace(300, 800, 2001).astype(np.float32)
+    m = med.to_abi()
+    q64 = np.array(med.quan_fry_params, np.float64)
+    n = eng.eval(_abi.NT_FN_REFRACTIVE_INDEX, wl, 1)[:, 0]
+    assert np.array_equal(n, np.array([oracle.ntor_refractive_index(w, m.quan_fry) for w in wl], np.float32))
+    n64 = np.array([oracle.ntor_refractive_index_f64(float(w), O.dptr(q64)) for w in wl])
+    assert errs_in_ulps(n, n64, 1.0).max() <= ULP_MAX
+    cg = eng.eval(_abi.NT_FN_GROUP_VELOCITY, wl, 1)[:, 0]
+    assert np.array_equal(cg, np.array([oracle.ntor_group_velocity(w, C.byref(m)) for w in wl], np.float32))
+    cg64 = np.array([oracle.ntor_group_velocity_f64(float(w), O.dptr(q64), 0) for w in wl])
+    assert errs_in_ulps(cg, cg64, 0.0).max() <= 8      # n - λ0·dn/dλ cancels ≈ 2^3
+    sl = eng.eval(_abi.NT_FN_SCA_LEN, wl, 1)[:, 0]
+    sl64 = np.array([oracle.ntor_scattering_length_f64(float(w), float(np.float32(0.0075)),
+                                                       float(np.float32(0.0075))) for w in wl])
+    assert errs_in_ulps(sl, sl64, 0.0).max() <= 4     # three powf terms summed
+    slo = np.array([oracle.ntor_scattering_length(w, C.byref(m)) for w in wl], np.float32)
+    assert np.abs(sl - slo).max() <= 2 * ulp32(sl).max()
+
+
+def test_cherenkov_and_update_direction_device(medium_engine, oracle):
+    eng, med = medium_engine
+    rng = np.random.default_rng(4)
+    n = 5000
+    wl = rng.uniform(300, 800, n).astype(np.float32)
+    u = rng.random((n, 3)).astype(np.float32)
+    src = rng.normal(size=(n, 3)); src /= np.linalg.norm(src, axis=1, keepdims=True)
+    src = src.astype(np.float32)
+    y = eng.eval(_abi.NT_FN_CHERENKOV_DIR, np.concatenate([wl[:, None], u, src], 1), 3)
+    yo = np.zeros_like(y)
+    q = med.to_abi().quan_fry
+    for i in range(n):
+        oracle.ntor_cherenkov_dir(wl[i], u[i, 0], u[i, 1], u[i, 2], O.fptr(src[i]), q,
+                                  yo[i].ctypes.data_as(C.POINTER(C.c_float)))
+    # transcendental chains (acos, pow, exp, log, sincos): ocml vs glibc differ in the last bits
+    assert np.abs(y - yo).max() < 5e-5
+    assert np.allclose(np.linalg.norm(y, axis=1), 1, atol=1e-5)
+
+    d = rng.normal(size=(n, 3)); d /= np.linalg.norm(d, axis=1, keepdims=True)
+    x = np.concatenate([d.astype(np.float32), rng.random((n, 3)).astype(np.float32)], 1)
+    y = eng.eval(_abi.NT_FN_UPDATE_DIR, x, 3)
+    m = med.to_abi()
+    yo = np.zeros_like(y)
+    for i in range(n):
+        oracle.ntor_update_dir(O.fptr(x[i, :3]), x[i, 3], x[i, 4], x[i, 5], C.byref(m),
+                               yo[i].ctypes.data_as(C.POINTER(C.c_float)))
+    assert np.abs(y - yo).max() < 1e-5
+    assert np.allclose(np.linalg.norm(y, axis=1), 1, atol=1e-5)
+
+
+def test_spectrum_sampling_device(engine, oracle):
+    med = nt.make_cascadia_medium_properties(0.95)
+    engine.set_medium(med)
+    u = np.random.default_rng(5).random(50000).astype(np.float32)
+    u[:4] = [0.0, 1 / 60, 0.5, np.float32(1) - np.float32(2 ** -24)]
+    for spec in (nt.CherenkovSpectrum((300.0, 600.0), 30, med),
+                 nt.CherenkovSpectrum((300.0, 800.0), 30, med, addressing="texture_q8"),
+                 nt.CherenkovSpectrum((300.0, 800.0), 30, med, addressing="endpoint"),
+                 nt.InvLambda2Spectrum((300.0, 600.0)), nt.Monochromatic(420.0)):
+        engine.set_spectrum(spec)
+        y = engine.eval(_abi.NT_FN_SPECTRUM, u, 1)[:, 0]
+        sp = spec.to_abi()
+        yo = np.array([oracle.ntor_spectrum_sample(C.byref(sp), x) for x in u], np.float32)
+        assert np.array_equal(y, yo), spec
+
+
+def test_gamma_device(medium_engine, oracle):
+    eng, _ = medium_engine
+    n = 4000
+    rows = np.zeros((n, 8), np.float32)
+    rows[:, 0], rows[:, 1], rows[:, 2] = 7.837, 0.63207, 0.3561
+    bits = rows.view(np.uint32)
+    bits[:, 3], bits[:, 4] = 17, 0
+    bits[:, 5] = np.arange(n)
+    bits[:, 6], bits[:, 7] = 0, 2
+    y = eng.eval(_abi.NT_FN_GAMMA, rows, 2)
+    yo = np.zeros(n, np.float32)
+    tr = np.zeros(n)
+    t = C.c_int()
+    for i in range(n):
+        yo[i] = oracle.ntor_rand_gamma(np.float32(7.837), np.float32(1) / np.float32(0.63207), 17, 2, i,
+                                       C.byref(t)) * np.float32(0.3561)
+        tr[i] = t.value
+    assert np.mean(y[:, 1] == tr) > 0.999            # same Philox trials accepted
+    same = y[:, 1] == tr
+    assert np.allclose(y[same, 0], yo[same], rtol=2e-6)

@@ -1,0 +1,253 @@
+"""Propagation parity: the HIP kernel (through the C ABI) against the CPU
+oracle.
+
+Same seed: both sides draw the same Philox streams, so a photon's history is
+the same up to last-bit differences of the transcendental functions (ocml on
+the GPU, glibc in the oracle).  Hits are matched by (source, photon index):
+at least 99.5 % of each side's hits must match and matched hits agree within
+1e-4 relative.  Different seeds: independent samples, compared as the
+reference's Monte Carlo would be: per-module counts within 3σ Poisson and
+two-sample KS tests on arrival time and wavelength at p > 0.01.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import stats
+
+import _oracle as O
+import neutrinotelescopes_amd as nt
+from neutrinotelescopes_amd import _abi
+
+pytestmark = pytest.mark.gpu
+
+PMT_AREA = (75e-3 / 2) ** 2 * math.pi
+
+
+def key(h):
+    return (h.source_index.astype(np.uint64) << np.uint64(40)) | h.photon_id.astype(np.uint64)
+
+
+def assert_hits_match(g, o, min_frac=0.995):
+    kg, ko = key(g), key(o)
+    assert len(np.unique(kg)) == len(kg), "a photon produced two hits"
+    common, ig, io = np.intersect1d(kg, ko, return_indices=True)
+    if len(kg) + len(ko) == 0:
+        return
+    assert len(common) >= min_frac * len(ko) - 1, (len(common), len(ko), len(kg))
+    assert len(common) >= min_frac * len(kg) - 1, (len(common), len(ko), len(kg))
+    assert np.array_equal(g.module_id[ig], o.module_id[io])
+    assert np.array_equal(g.wavelength[ig], o.wavelength[io]) or \
+        np.allclose(g.wavelength[ig], o.wavelength[io], rtol=1e-6)
+    for col, tol in (("position", 1e-4), ("direction", 1e-4), ("initial_direction", 1e-5),
+                     ("time", 1e-4), ("dist_travelled", 1e-4)):
+        a, b = getattr(g, col)[ig], getattr(o, col)[io]
+        scale = np.maximum(np.abs(b), 1.0)
+        bad = np.abs(a - b) > tol * scale
+        assert bad.mean() <= 0.002, (col, int(bad.sum()), len(common))
+
+
+def run_gpu(engine, setup, seed=0, capacity=1 << 20):
+    engine.setup(setup)
+    h = engine.propagate(seed=seed, capacity=capacity)
+    return h
+
+
+def c1_ref_twin(n, g=0.95, r=0.21, **medium_ext):
+    """Config 1 in reference mode: isotropic point source, one DetectionSphere at
+    10 m, Cascadia water, Frank-Tamm 300-600 nm table of 30 knots."""
+    med = nt.make_cascadia_medium_properties(g, **medium_ext)
+    spec = nt.CherenkovSpectrum((300.0, 600.0), 30, med)
+    src = nt.PointlikeIsotropicEmitter(np.zeros(3, np.float32), 0.0, n)
+    tgt = nt.DetectionSphere(np.array([0, 0, 10.0], np.float32), r, 16, PMT_AREA, 1)
+    return nt.PhotonPropSetup([src], [tgt], med, spec)
+
+
+def string_targets(n=20, spacing=50.0, x=20.0):
+    coords = nt.make_pom_pmt_coordinates(np.float32)
+    return [nt.MultiPMTDetector(np.array([x, 0.0, -475.0 + spacing * k], np.float32), 0.21, PMT_AREA,
+                                coords, k + 1) for k in range(n)]
+
+
+def c2_cascade(n, energy=1e4):
+    med = nt.make_cascadia_medium_properties(0.95)
+    spec = nt.CherenkovSpectrum((300.0, 800.0), 30, med)
+    lp = nt.LongitudinalParameterisation.from_energy(energy, med, nt.PEMinus)
+    src = nt.ExtendedCherenkovEmitter(np.zeros(3, np.float32), np.array([1, 0, 0], np.float32), 0.0, n, lp)
+    return nt.PhotonPropSetup([src], string_targets(), med, spec)
+
+
+def test_c1_ref_twin_per_photon(engine):
+    s = c1_ref_twin(1_000_000, r=1.0)
+    g = run_gpu(engine, s, seed=5)
+    o = O.propagate(s, seed=5)
+    assert g.n_simulated == o.n_simulated == 1_000_000
+    assert len(o) > 1000
+    assert_hits_match(g, o)
+
+
+def test_c1_statistics_independent_seeds(engine):
+    s = c1_ref_twin(2_000_000, r=1.0)
+    g = run_gpu(engine, s, seed=11)
+    o = O.propagate(s, seed=12)
+    ng, no = len(g), len(o)
+    assert abs(ng - no) <= 3 * math.sqrt(ng + no)
+    assert stats.ks_2samp(g.time, o.time).pvalue > 0.01
+    assert stats.ks_2samp(g.wavelength, o.wavelength).pvalue > 0.01
+    assert stats.ks_2samp(g.dist_travelled, o.dist_travelled).pvalue > 0.01
+
+
+def test_c2_cascade_string_per_photon_and_counts(engine):
+    s = c2_cascade(300_000)
+    g = run_gpu(engine, s, seed=3)
+    o = O.propagate(s, seed=3)
+    assert len(o) > 100
+    assert_hits_match(g, o)
+    g2 = run_gpu(engine, s, seed=4)
+    for mid in range(1, 21):
+        a, b = int(np.sum(g2.module_id == mid)), int(np.sum(o.module_id == mid))
+        assert abs(a - b) <= 3 * math.sqrt(a + b) + 1, (mid, a, b)
+
+
+@pytest.mark.parametrize("kind", ["time_range", "axicone", "pencil", "pointlike_cherenkov",
+                                  "extended_cherenkov_vertical"])
+def test_source_kinds(engine, kind):
+    med = nt.make_cascadia_medium_properties(0.9)
+    spec = nt.CherenkovSpectrum((300.0, 800.0), 30, med)
+    p = np.array([0.5, -0.3, 0.2], np.float32)
+    d = np.array([0.0, 0.6, 0.8], np.float32)
+    n = 300_000
+    src = {
+        "time_range": nt.PointlikeTimeRangeEmitter(p, (5.0, 25.0), n),
+        "axicone": nt.AxiconeEmitter(p, d, 1.0, n, 0.7),
+        "pencil": nt.PencilEmitter(p, d, 2.0, n),
+        "pointlike_cherenkov": nt.PointlikeCherenkovEmitter(p, d, 0.0, n),
+        "extended_cherenkov_vertical": nt.ExtendedCherenkovEmitter(
+            p, np.array([0, 0, 1.0], np.float32), 0.0, n, nt.LongitudinalParameterisation(6.0, 0.63, 0.36)),
+    }[kind]
+    tgts = [nt.DetectionSphere(np.array([0, 6.0, 8.0], np.float32), 1.0, 16, PMT_AREA, 7),
+            nt.DetectionSphere(np.array([0, -5.0, 3.0], np.float32), 1.0, 16, PMT_AREA, 8)]
+    s = nt.PhotonPropSetup([src], tgts, med, spec)
+    g = run_gpu(engine, s, seed=9)
+    o = O.propagate(s, seed=9)
+    assert len(o) > 50
+    assert_hits_match(g, o)
+
+
+def test_multi_source_ragged(engine):
+    med = nt.make_cascadia_medium_properties(0.95)
+    spec = nt.Monochromatic(450.0)
+    srcs = [nt.PointlikeIsotropicEmitter(np.array([0, 0, float(i)], np.float32), float(i), nph)
+            for i, nph in enumerate([0, 1, 123457, 0, 77777, 5])]
+    tgt = [nt.DetectionSphere(np.array([0, 0, 4.0], np.float32), 0.8, 16, PMT_AREA, 3)]
+    s = nt.PhotonPropSetup(srcs, tgt, med, spec)
+    g = run_gpu(engine, s, seed=21)
+    o = O.propagate(s, seed=21)
+    assert g.n_simulated == sum(x.photons for x in srcs)
+    assert set(np.unique(g.source_index)) <= {1, 2, 4, 5}
+    assert_hits_match(g, o)
+
+
+def test_extensions_sl_invlambda2_quanfry(engine):
+    med = nt.make_cascadia_medium_properties(0.95, sl_fraction=0.2, dispersion_mode="quan_fry",
+                                             max_steps=25)
+    med.sca_len_scale = float(50.0 / nt.scattering_length(np.float32(400.0), med))
+    spec = nt.InvLambda2Spectrum((300.0, 600.0))
+    src = nt.PointlikeIsotropicEmitter(np.zeros(3, np.float32), 0.0, 500_000)
+    tgt = [nt.DetectionSphere(np.array([0, 0, 10.0], np.float32), 1.0, 16, PMT_AREA, 1)]
+    s = nt.PhotonPropSetup([src], tgt, med, spec)
+    g = run_gpu(engine, s, seed=2)
+    o = O.propagate(s, seed=2)
+    assert_hits_match(g, o)
+
+
+def test_no_targets_and_zero_photons(engine):
+    med = nt.make_cascadia_medium_properties(0.95)
+    s = nt.PhotonPropSetup([nt.PointlikeIsotropicEmitter(np.zeros(3), 0.0, 10000)], [], med,
+                           nt.Monochromatic(400.0))
+    g = run_gpu(engine, s)
+    assert len(g) == 0 and g.n_simulated == 10000
+    s = nt.PhotonPropSetup([nt.PointlikeIsotropicEmitter(np.zeros(3), 0.0, 0)],
+                           [nt.DetectionSphere(np.array([0, 0, 2.0]), 1.0, 1, 1.0)], med, nt.Monochromatic(400.0))
+    g = run_gpu(engine, s)
+    assert len(g) == 0 and g.n_simulated == 0
+
+
+def test_overflow_keeps_prefix_and_counts(engine):
+    s = c1_ref_twin(400_000, r=2.0)
+    full = run_gpu(engine, s, seed=8)
+    engine.setup(s)
+    part = engine.propagate(seed=8, capacity=100, raise_on_overflow=False)
+    assert len(part) == 100 and part.n_dropped == len(full) - 100
+    with pytest.raises(_abi.NTError) as e:
+        engine.propagate(seed=8, capacity=100)
+    assert e.value.status == _abi.NT_ERR_HIT_OVERFLOW
+    kf = dict(zip(key(full).tolist(), range(len(full))))
+    idx = [kf[k] for k in key(part).tolist()]          # every stored hit is a real hit
+    assert np.array_equal(full.time[idx], part.time)
+
+
+def test_invalid_arguments(engine):
+    lib = engine.lib
+    bad = nt.make_cascadia_medium_properties(0.95).to_abi()
+    bad.mean_scattering_angle = 1.5
+    assert lib.nt_set_medium(engine.handle, bad) == _abi.NT_ERR_ARG
+    src = nt.PointlikeIsotropicEmitter(np.zeros(3), 0.0, 10).to_abi()
+    src.photons = -1
+    arr = _abi.as_c_array(_abi.nt_source, [src])
+    assert lib.nt_set_sources(engine.handle, arr, 1) == _abi.NT_ERR_ARG
+    src.photons, src.kind = 10, 42
+    arr = _abi.as_c_array(_abi.nt_source, [src])
+    assert lib.nt_set_sources(engine.handle, arr, 1) == _abi.NT_ERR_ARG
+
+
+def test_full_size_c1_properties(engine):
+    """1e8 photons (the parity size of config 1): size-independent properties."""
+    n = 100_000_000
+    s = c1_ref_twin(n)
+    g = run_gpu(engine, s, seed=1, capacity=1 << 22)
+    assert g.n_simulated == n and g.n_dropped == 0
+    # every hit lies on the sphere surface, inside the table's wavelengths,
+    # not earlier than light from the origin could arrive
+    assert np.allclose(np.linalg.norm(g.position - np.array([0, 0, 10.0]), axis=1), 0.21, atol=5e-5)
+    assert g.wavelength.min() >= 300.0 and g.wavelength.max() <= 600.0
+    cmax = float(np.max(nt.group_velocity(np.linspace(300, 600, 301).astype(np.float32), s.medium)))
+    assert g.time.min() >= (10.0 - 0.21) / cmax * (1 - 1e-5)
+    assert np.all(g.dist_travelled >= 10.0 - 0.21 - 1e-4)
+    assert len(np.unique(g.photon_id)) == len(g)
+    # hit count agrees with the oracle's rate measured on 2e6 photons (3σ)
+    o = O.propagate(c1_ref_twin(2_000_000), seed=99)
+    rate = len(o) / 2e6
+    expect = rate * n
+    sigma = math.sqrt(expect + (n / 2e6) ** 2 * len(o))
+    assert abs(len(g) - expect) < 3 * sigma, (len(g), expect)
+    # unscattered photons alone give at least the solid-angle fraction × survival
+    # to 9.79 m without scattering
+    assert len(g) > n * (1 - math.sqrt(1 - 0.021 ** 2)) / 2 * math.exp(-9.79 / 18.0) * 0.9
+
+
+def test_device_launch_with_torch_buffers(engine):
+    torch = pytest.importorskip("torch")
+    s = c1_ref_twin(500_000, r=1.0)
+    host = run_gpu(engine, s, seed=4)
+    cap = 1 << 16
+    dev = torch.device("cuda:0")
+    cols = dict(position=torch.zeros(cap, 3, device=dev), direction=torch.zeros(cap, 3, device=dev),
+                initial_direction=torch.zeros(cap, 3, device=dev), wavelength=torch.zeros(cap, device=dev),
+                time=torch.zeros(cap, device=dev), dist_travelled=torch.zeros(cap, device=dev),
+                module_id=torch.zeros(cap, dtype=torch.int16, device=dev),
+                source_index=torch.zeros(cap, dtype=torch.int32, device=dev),
+                photon_id=torch.zeros(cap, dtype=torch.int64, device=dev))
+    counters = torch.zeros(4, dtype=torch.int64, device=dev)
+    h = _abi.nt_hits()
+    for k, v in cols.items():
+        setattr(h, k, v.data_ptr())
+    h.capacity = cap
+    stream = torch.cuda.current_stream()
+    _abi.check(engine.lib.nt_launch(engine.handle, 4, h, counters.data_ptr(), stream.cuda_stream), "nt_launch")
+    torch.cuda.synchronize()
+    c = counters.cpu().numpy()
+    assert c[0] == len(host) and c[1] == 500_000 and c[2] == 500_000 * 15 or c[2] > 0
+    n = int(c[0])
+    pid = cols["photon_id"][:n].cpu().numpy().astype(np.uint64)
+    assert np.array_equal(np.sort(pid), np.sort(host.photon_id))

@@ -1,0 +1,508 @@
+"""Pins the CPU oracle: reference known-answer tests (test/runtests.jl),
+published Philox vectors, closed forms and invariants.  CPU only."""
+import ctypes as C
+import json
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+from scipy import stats
+
+import _oracle as O
+import neutrinotelescopes_amd as nt
+from neutrinotelescopes_amd import _abi
+
+GOLDEN = Path(__file__).parent / "golden"
+KATS = json.loads((GOLDEN / "runtests_kats.json").read_text())
+
+
+def hexs(xs):
+    return [int(x, 16) for x in xs]
+
+
+def cascadia(g=0.95, **kw):
+    return nt.make_cascadia_medium_properties(g, **kw)
+
+
+# ---------------------------------------------------------------- Philox
+@pytest.mark.parametrize("group", ["random123", "rocrand"])
+def test_philox_kat(group):
+    for v in json.loads((GOLDEN / "philox4x32_10_kat.json").read_text())[group]:
+        assert O.philox(hexs(v["ctr"]), hexs(v["key"])) == hexs(v["out"])
+
+
+def test_philox_uniform_statistics():
+    # 4e4 words from consecutive counters: uniform on [0, 2^32) (KS) and
+    # bit balance; catches a swapped word or a missing round in a way the
+    # KATs alone could not isolate.
+    words = np.array([O.philox([i, 7, 0, 3], [0x1234, 0x5678]) for i in range(10000)],
+                     dtype=np.uint64).ravel()
+    u = words / 2.0 ** 32
+    assert stats.kstest(u, "uniform").pvalue > 1e-3
+    bits = ((words[:, None] >> np.arange(32, dtype=np.uint64)) & 1).mean(axis=0)
+    assert np.all(np.abs(bits - 0.5) < 0.02)
+
+
+# ---------------------------------------------------------------- intersection
+def test_intersection_kats(oracle):
+    k = KATS["intersection"]
+    tpos = k["target"]["position"]
+    r = k["target"]["radius"]
+    for case in k["cases"]:
+        d = np.array(case["dir"], dtype=np.float64)
+        if case.get("normalize_dir"):
+            d = d / np.linalg.norm(d)
+        dist = C.c_float()
+        hit = oracle.ntor_intersect(O.fptr(case["pos"]), O.fptr(d.astype(np.float32)),
+                                    case["step"], O.fptr(tpos), r, C.byref(dist))
+        assert bool(hit) == case["hit"], case
+        d64 = C.c_double()
+        hit64 = oracle.ntor_intersect_f64(O.dptr(case["pos"]), O.dptr(d), case["step"],
+                                          O.dptr(tpos), r, C.byref(d64))
+        assert bool(hit64) == case["hit"]
+
+
+def test_intersection_geometry_closed_form(oracle):
+    # ray from distance D toward the centre hits at D - r; rays with impact
+    # parameter b < r hit at sqrt(D^2-b^2) - sqrt(r^2-b^2), b > r miss.
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        c = rng.uniform(-50, 50, 3)
+        r = rng.uniform(0.1, 2.0)
+        D = rng.uniform(3.0, 40.0)
+        u = rng.normal(size=3); u /= np.linalg.norm(u)
+        perp = np.cross(u, rng.normal(size=3)); perp /= np.linalg.norm(perp)
+        b = rng.uniform(0, 1.5 * r)
+        pos = c - D * u + b * perp
+        dist = C.c_double()
+        hit = oracle.ntor_intersect_f64(O.dptr(pos), O.dptr(u), 1e3, O.dptr(c), r, C.byref(dist))
+        assert bool(hit) == (b < r)
+        if b < r:
+            assert dist.value == pytest.approx(math.sqrt(D * D) - math.sqrt(r * r - b * b), rel=1e-9,
+                                               abs=1e-9)
+            if b < 0.8 * r:
+                # FP32: a^2 and |p-c|^2 (~D^2) cancel, absolute error ~D^2 * 2^-23
+                d32 = C.c_float()
+                assert oracle.ntor_intersect(O.fptr(pos), O.fptr(u), 1e3, O.fptr(c), r, C.byref(d32))
+                assert d32.value == pytest.approx(dist.value, abs=3e-3)
+
+
+# ---------------------------------------------------------------- rotations
+def calc_rot_matrix(a, b):
+    """utils.jl:173-184 (Rodrigues form), the reference's own check of the fast rotations."""
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    if np.array_equal(a, b):
+        return np.eye(3)
+    v = np.cross(a, b)
+    s = np.array([[0, -v[2], v[1]], [v[2], 0, -v[0]], [-v[1], v[0], 0]])
+    return np.eye(3) + s + s @ s * (1 - a @ b) / (v @ v)
+
+
+def apply_rot(a, b, op):
+    r = calc_rot_matrix(a, b) @ np.asarray(op, float)
+    return r / np.linalg.norm(r)
+
+
+def sph(theta, phi):
+    return np.array([np.cos(phi) * np.sin(theta), np.sin(phi) * np.sin(theta), np.cos(theta)])
+
+
+def test_rotation_kats(oracle):
+    k = KATS["rotations"]
+    ez = np.array([0.0, 0.0, 1.0])
+    d = sph(k["to_ez"]["theta"], k["to_ez"]["phi"])
+    assert np.allclose(apply_rot(d, ez, d), ez, atol=1e-9)
+    r45 = k["rot45"]
+    out = apply_rot(sph(r45["theta1"], r45["phi1"]), sph(r45["theta2"], r45["phi2"]), r45["operand"])
+    assert np.allclose(out, r45["expected"], atol=1e-9)
+    fv = k["fast_vs_matrix"]
+    d1, d2 = sph(fv["theta1"], fv["phi1"]), sph(fv["theta2"], fv["phi2"])
+    o32 = np.zeros(3, np.float32)
+    oracle.ntor_rot_to_ez(O.fptr(d1), O.fptr(d2), o32.ctypes.data_as(C.POINTER(C.c_float)))
+    assert np.allclose(o32, apply_rot(d1, ez, d2), atol=1e-6)
+    oracle.ntor_rot_from_ez(O.fptr(d1), O.fptr(d2), o32.ctypes.data_as(C.POINTER(C.c_float)))
+    assert np.allclose(o32, apply_rot(ez, d1, d2), atol=1e-6)
+    o64 = np.zeros(3)
+    oracle.ntor_rot_from_ez_f64(O.dptr(d1), O.dptr(d2), o64.ctypes.data_as(C.POINTER(C.c_double)))
+    assert np.allclose(o64, apply_rot(ez, d1, d2), atol=1e-12)
+
+
+def test_rot_from_ez_random_vs_matrix(oracle):
+    rng = np.random.default_rng(2)
+    ez = np.array([0.0, 0.0, 1.0])
+    for _ in range(300):
+        a = rng.normal(size=3); a /= np.linalg.norm(a)
+        op = rng.normal(size=3); op /= np.linalg.norm(op)
+        o = np.zeros(3)
+        oracle.ntor_rot_from_ez_f64(O.dptr(a), O.dptr(op), o.ctypes.data_as(C.POINTER(C.c_double)))
+        assert np.allclose(o, apply_rot(ez, a, op), atol=1e-9)
+        assert abs(np.linalg.norm(o) - 1) < 1e-12
+
+
+def test_rot_vertical_axis_quirk(oracle):
+    # utils.jl:230-232: for a = -e_z the fast rotation flips y and z signs
+    # instead of rotating by π (not a proper rotation: det = +1 but it maps
+    # e_x to e_x, unlike any rotation taking e_z to -e_z about a horizontal axis
+    # which is what calc_rot_matrix is undefined for).
+    o = np.zeros(3, np.float32)
+    oracle.ntor_rot_from_ez(O.fptr([0, 0, -1]), O.fptr([0.6, 0.8, 0.0]),
+                            o.ctypes.data_as(C.POINTER(C.c_float)))
+    assert list(o) == [np.float32(0.6), np.float32(-0.8), np.float32(-0.0)]
+
+
+# ---------------------------------------------------------------- quadrature / sph
+def test_integrate_gauss_quad_kats(oracle):
+    c = KATS["integrate_gauss_quad"]["cases"]
+    assert oracle.ntor_integrate_gauss_quad_pow(c[0]["a"], c[0]["b"], 3) == pytest.approx(c[0]["expected"], rel=1e-12)
+    assert oracle.ntor_integrate_gauss_quad_cos(c[1]["a"], c[1]["b"]) == pytest.approx(c[1]["expected"], rel=1e-12)
+    assert nt.integrate_gauss_quad(lambda x: x ** 3, 0.0, 1.0) == pytest.approx(0.25, rel=1e-12)
+
+
+def test_sph_to_cart_kats(oracle):
+    o = np.zeros(3, np.float32)
+    for case in KATS["sph_to_cart"]["cases"]:
+        oracle.ntor_sph_to_cart(case["theta"], case["phi"], o.ctypes.data_as(C.POINTER(C.c_float)))
+        assert np.allclose(o, case["expected"], atol=1e-6)
+    from neutrinotelescopes_amd.detection import cart_to_sph, sph_to_cart
+    rt = KATS["cart_to_sph_roundtrip"]
+    assert np.allclose(cart_to_sph(*sph_to_cart(rt["theta"], rt["phi"])), (rt["theta"], rt["phi"]))
+
+
+def test_split_source_conservation():
+    k = KATS["split_source"]
+    src = nt.PointlikeTimeRangeEmitter(np.array([0, 0, 0.7], np.float32), (0.0, 1e6), k["photons"])
+    parts = nt.split_source(src, k["parts"])
+    assert sum(p.photons for p in parts) == k["photons"] and len(parts) == k["parts"]
+    with pytest.raises(ValueError):
+        nt.split_source(nt.PointlikeIsotropicEmitter(np.zeros(3), 0.0, 3), 5)
+
+
+# ---------------------------------------------------------------- medium
+def test_quan_fry_textbook_and_cascadia(oracle):
+    m = _abi.nt_medium()
+    dens = C.c_double()
+    oracle.ntor_make_cascadia_medium(0.95, C.byref(m), C.byref(dens), None)
+    # Quan & Fry (1995) polynomial: n = n0 + (n2 - n3 T + n4 T^2) S + n1 P - n5 T^2
+    #   + (n6 + n7 S - n8 T)/λ - n9/λ^2 + n10/λ^3
+    S, T, P = 34.82, 1.8, 269.44088 / 1.01325
+    n_tb = lambda wl: (1.31405 + (1.779e-4 - 1.05e-6 * T + 1.6e-8 * T * T) * S + 1.45e-5 * P
+                       - 2.02e-6 * T * T + (15.868 + 0.01155 * S - 0.00423 * T) / wl
+                       - 4382 / wl ** 2 + 1.1455e6 / wl ** 3)
+    for wl in (300.0, 400.0, 550.0, 800.0):
+        n32 = oracle.ntor_refractive_index(wl, m.quan_fry)
+        assert n32 == pytest.approx(n_tb(wl), rel=2e-7)
+    assert oracle.ntor_refractive_index(400.0, m.quan_fry) == pytest.approx(1.35520, abs=1e-5)
+    assert dens.value == pytest.approx(1013.26, abs=0.01)
+    py = cascadia()
+    assert [np.float32(v) for v in m.quan_fry] == list(py.quan_fry_params)
+    assert py.density == np.float32(dens.value)
+
+
+def test_group_velocity_quirk_and_correct_form(oracle):
+    med = cascadia()
+    m_ref = med.to_abi()
+    # reference reading (medium.jl:293-318): 0.22049 m/ns at 400 nm
+    assert oracle.ntor_group_velocity(400.0, C.byref(m_ref)) == pytest.approx(0.22049, abs=2e-5)
+    m_ok = cascadia(dispersion_mode="quan_fry").to_abi()
+    q = np.array(list(m_ok.quan_fry), dtype=np.float64)
+    for wl in (300.0, 400.0, 600.0):
+        # group velocity from a numerical derivative of the Quan-Fry n(λ)
+        h = 1e-3
+        dn = (oracle.ntor_refractive_index_f64(wl + h, O.dptr(q)) -
+              oracle.ntor_refractive_index_f64(wl - h, O.dptr(q))) / (2 * h)
+        n = oracle.ntor_refractive_index_f64(wl, O.dptr(q))
+        vg = 0.299792458 / (n - wl * dn)
+        assert oracle.ntor_group_velocity(wl, C.byref(m_ok)) == pytest.approx(vg, rel=2e-6)
+        assert oracle.ntor_group_velocity_f64(wl, O.dptr(q), 1) == pytest.approx(vg, rel=1e-9)
+    assert oracle.ntor_group_velocity(400.0, C.byref(m_ok)) == pytest.approx(0.21494, abs=2e-5)
+    ratios = {300: 1.057, 400: 1.026, 600: 1.012}
+    for wl, r in ratios.items():
+        got = oracle.ntor_group_velocity(wl, C.byref(m_ref)) / oracle.ntor_group_velocity(wl, C.byref(m_ok))
+        assert got == pytest.approx(r, abs=1.5e-3)
+    # host mirror agrees bit for bit
+    for wl in (300.0, 412.5, 600.0):
+        assert nt.group_velocity(np.float32(wl), med) == np.float32(oracle.ntor_group_velocity(wl, C.byref(m_ref)))
+
+
+def test_scattering_length_values(oracle):
+    m = cascadia().to_abi()
+    got = [oracle.ntor_scattering_length(w, C.byref(m)) for w in (300.0, 400.0, 600.0)]
+    assert got == pytest.approx([18.517, 37.693, 82.527], abs=2e-3)
+    for w in (300.0, 400.0, 600.0):
+        f64 = oracle.ntor_scattering_length_f64(w, 0.0075, 0.0075)
+        assert oracle.ntor_scattering_length(w, C.byref(m)) == pytest.approx(f64, rel=3e-6)
+    m.sca_len_scale = 50.0 / got[1]
+    assert oracle.ntor_scattering_length(400.0, C.byref(m)) == pytest.approx(50.0, rel=1e-6)
+
+
+# ---------------------------------------------------------------- phase functions
+def hg_cdf(c, g):
+    return (1 - g * g) / (2 * g) * (1 / np.sqrt(1 + g * g - 2 * g * c) - 1 / (1 + g))
+
+
+@pytest.mark.parametrize("g", [0.5, 0.9, 0.95, 0.99])
+def test_hg_inverse_cdf_closed_form(oracle, g):
+    eta = np.linspace(0.001, 0.999, 301)
+    c64 = np.array([oracle.ntor_hg_cos_f64(g, e) for e in eta])
+    assert np.allclose(hg_cdf(c64, g), eta, atol=1e-10)
+    c32 = np.array([oracle.ntor_hg_cos(np.float32(g), np.float32(e)) for e in eta])
+    assert np.allclose(c32, c64, atol=2e-5)
+    # E[cos θ] = g (midpoint rule over η)
+    n = 200000
+    mid = (np.arange(n) + 0.5) / n
+    t = (1 - g * g) / (1 + g * (2 * mid - 1))
+    assert np.mean((1 + g * g - t * t) / (2 * g)) == pytest.approx(g, abs=1e-6)
+
+
+@pytest.mark.parametrize("g", [0.8, 0.95])
+def test_simplified_liu_closed_form(oracle, g):
+    alpha = 2 * g / (1 - g)
+    xi = np.linspace(0.001, 0.999, 101)
+    c = np.array([oracle.ntor_sl_cos_f64(g, x) for x in xi])
+    assert np.allclose(((1 + c) / 2) ** (1 + alpha), xi, atol=1e-10)   # CDF round trip
+    n = 200000
+    mid = (np.arange(n) + 0.5) / n
+    assert np.mean(2 * mid ** (1 / (1 + alpha)) - 1) == pytest.approx(g, abs=1e-5)
+
+
+def test_update_direction_statistics(oracle):
+    # scattered directions keep unit length and the mean cosine to the old
+    # direction equals g
+    m = cascadia(0.9).to_abi()
+    rng = np.random.default_rng(3)
+    d = rng.normal(size=3); d = (d / np.linalg.norm(d)).astype(np.float32)
+    cos = []
+    o = np.zeros(3, np.float32)
+    for i in range(20000):
+        oracle.ntor_update_dir(O.fptr(d), rng.random(), rng.random(), 0.5, C.byref(m),
+                               o.ctypes.data_as(C.POINTER(C.c_float)))
+        assert abs(np.linalg.norm(o) - 1) < 1e-5
+        cos.append(float(o @ d))
+    assert np.mean(cos) == pytest.approx(0.9, abs=0.01)
+
+
+# ---------------------------------------------------------------- sources
+def test_isotropic_direction_distribution(oracle):
+    rng = np.random.default_rng(4)
+    o = np.zeros(3, np.float32)
+    zs, phis = [], []
+    for _ in range(20000):
+        oracle.ntor_iso_dir(rng.random(), rng.random(), o.ctypes.data_as(C.POINTER(C.c_float)))
+        zs.append(o[2]); phis.append(math.atan2(o[1], o[0]))
+    assert stats.kstest(zs, stats.uniform(-1, 2).cdf).pvalue > 1e-3
+    assert stats.kstest(phis, stats.uniform(-np.pi, 2 * np.pi).cdf).pvalue > 1e-3
+
+
+def test_cherenkov_track_direction_cdf(oracle):
+    # utils.jl:249-260: t = 1 - cosθ has CDF (1 - exp(-b t^a)) / (1 - exp(-b 2^a)), a=0.39, b=2.61
+    a, b = 0.39, 2.61
+    I = 1 - math.exp(-b * 2 ** a)
+    rng = np.random.default_rng(5)
+    o = np.zeros(3, np.float32)
+    t = []
+    for _ in range(20000):
+        oracle.ntor_cherenkov_track_dir(rng.random(), rng.random(), o.ctypes.data_as(C.POINTER(C.c_float)))
+        t.append(1 - o[2])
+    cdf = lambda x: (1 - np.exp(-b * np.clip(x, 0, 2) ** a)) / I
+    assert stats.kstest(t, cdf).pvalue > 1e-3
+
+
+def test_cherenkov_emission_angle(oracle):
+    med = cascadia()
+    q = med.to_abi().quan_fry
+    o = np.zeros(3, np.float32)
+    src = np.array([0.6, 0.0, 0.8], np.float32)
+    for wl in (320.0, 400.0, 550.0):
+        theta_c = oracle.ntor_cherenkov_angle(wl, q)
+        assert theta_c == pytest.approx(math.acos(1 / oracle.ntor_refractive_index(wl, q)), abs=1e-6)
+        for u in (0.1, 0.7):
+            # u_tcos = 0: the track is the source axis, so the photon sits on the Cherenkov cone
+            oracle.ntor_cherenkov_dir(wl, u, 0.0, 0.3, O.fptr(src), q, o.ctypes.data_as(C.POINTER(C.c_float)))
+            assert math.acos(float(np.clip(o @ src, -1, 1))) == pytest.approx(theta_c, abs=2e-4)
+    assert math.degrees(oracle.ntor_cherenkov_angle(400.0, q)) == pytest.approx(42.4, abs=0.1)
+
+
+def test_rand_gamma_distribution(oracle):
+    a, b = 7.837, 0.63207
+    x = np.array([oracle.ntor_rand_gamma(a, 1 / b, 11, 0, i, None) for i in range(20000)])
+    assert stats.kstest(x, stats.gamma(a, scale=1 / b).cdf).pvalue > 1e-3
+    assert x.mean() == pytest.approx(a / b, rel=0.01)
+    trials = C.c_int()
+    n = [0]
+    for i in range(2000):
+        oracle.ntor_rand_gamma(a, 1 / b, 11, 0, i, C.byref(trials))
+        n.append(trials.value)
+    assert 1.0 <= np.mean(n[1:]) < 1.1       # Marsaglia-Tsang efficiency at a ≈ 7.8
+
+
+# ---------------------------------------------------------------- spectrum
+def test_frank_tamm_norm_values(oracle):
+    q = cascadia().to_abi().quan_fry
+    assert oracle.ntor_frank_tamm_norm(300.0, 800.0, q) == pytest.approx(43360, rel=1e-3)
+    assert oracle.ntor_frank_tamm_norm(300.0, 600.0, q) == pytest.approx(34864, rel=1e-3)
+    med = cascadia()
+    assert nt.frank_tamm_norm((300.0, 800.0), lambda wl: np.asarray(nt.refractive_index(wl.astype(np.float32), med), float)) \
+        == pytest.approx(oracle.ntor_frank_tamm_norm(300.0, 800.0, q), rel=1e-6)
+
+
+def test_spectrum_knots_closed_form_constant_n(oracle):
+    # with constant n the Frank-Tamm CDF is (1/λ0 - 1/λ)/(1/λ0 - 1/λ1); the knots
+    # must follow λ(u) = 1/((1-u)/λ0 + u/λ1) up to the 1-nm interpolation error
+    q = (C.c_float * 4)(1.34, 0.0, 0.0, 0.0)
+    k = np.zeros(30, np.float32)
+    assert oracle.ntor_cherenkov_spectrum_knots(300.0, 600.0, 30, q, k.ctypes.data_as(C.POINTER(C.c_float))) == 0
+    u = np.linspace(0, 1, 30)
+    exact = 1 / ((1 - u) / 300 + u / 600)
+    assert np.allclose(k, exact, atol=0.02)
+
+
+def test_spectrum_knots_oracle_vs_mirror(oracle):
+    med = cascadia()
+    for lo, hi, n in ((300.0, 600.0, 30), (300.0, 800.0, 30), (300.0, 800.0, 64)):
+        k = np.zeros(n, np.float32)
+        oracle.ntor_cherenkov_spectrum_knots(lo, hi, n, med.to_abi().quan_fry, k.ctypes.data_as(C.POINTER(C.c_float)))
+        assert np.allclose(nt.CherenkovSpectrum((lo, hi), n, med).knots, k, atol=1e-3)
+
+
+def test_texture_sampling_end_knots(oracle):
+    # texel-centre clamp addressing: u < 1/(2N) returns knot 0 exactly and
+    # u >= 1 - 1/(2N) returns knot N-1, so each end knot carries 1/(2N)
+    knots = np.linspace(300, 600, 30).astype(np.float32)
+    s = nt.CherenkovSpectrum((300.0, 600.0), knots=knots)
+    sp = s.to_abi()
+    u = (np.arange(300000) + 0.5) / 300000
+    vals = np.array([oracle.ntor_spectrum_sample(C.byref(sp), x) for x in u[::10]])
+    assert np.mean(vals == knots[0]) == pytest.approx(1 / 60, abs=2e-3)
+    assert np.mean(vals == knots[-1]) == pytest.approx(1 / 60, abs=2e-3)
+    # texel centres reproduce the knots
+    for i in (0, 7, 29):
+        assert oracle.ntor_spectrum_sample(C.byref(sp), (i + 0.5) / 30) == knots[i]
+    sp.addressing = _abi.NT_TEX_ENDPOINT
+    assert oracle.ntor_spectrum_sample(C.byref(sp), 0.0) == knots[0]
+    assert oracle.ntor_spectrum_sample(C.byref(sp), 7 / 29) == pytest.approx(knots[7], abs=1e-3)
+
+
+def test_inv_lambda2_spectrum(oracle):
+    sp = nt.InvLambda2Spectrum((300.0, 600.0)).to_abi()
+    rng = np.random.default_rng(6)
+    wl = np.array([oracle.ntor_spectrum_sample(C.byref(sp), x) for x in rng.random(20000)])
+    cdf = lambda l: (1 / 300 - 1 / np.clip(l, 300, 600)) / (1 / 300 - 1 / 600)
+    assert stats.kstest(wl, cdf).pvalue > 1e-3
+
+
+# ---------------------------------------------------------------- post-processing
+def test_fast_linear_interp_straw_points(oracle):
+    xs = np.array([300.0, 365.0, 400.0, 450.0, 585.0, 800.0])      # medium.jl:386-387
+    ys = np.array([10.4, 10.4, 14.5, 27.7, 7.1, 7.1])
+    for x, y in ((400.0, 14.5), (450.0, 27.7), (250.0, 10.4), (900.0, 7.1), (425.0, (14.5 + 27.7) / 2)):
+        assert oracle.ntor_fast_linear_interp(x, O.dptr(xs), O.dptr(ys), 6) == pytest.approx(y)
+    from neutrinotelescopes_amd.medium import fast_linear_interp
+    q = np.array([250, 300, 333.3, 400, 612, 800, 900.0])
+    ref = [oracle.ntor_fast_linear_interp(x, O.dptr(xs), O.dptr(ys), 6) for x in q]
+    assert np.allclose(fast_linear_interp(q, xs, ys), ref)
+    assert float(nt.absorption_length(np.float32(400.0), cascadia())) == pytest.approx(14.5, abs=1e-5)
+
+
+def test_pom_coordinates_and_pmt_hit(oracle):
+    c = np.zeros(32)
+    oracle.ntor_pom_pmt_coordinates(c.ctypes.data_as(C.POINTER(C.c_double)))
+    oc = c.reshape(16, 2).T
+    pc = nt.make_pom_pmt_coordinates()
+    assert np.allclose(oc, pc, atol=1e-12)
+    axes = sph(pc[0], pc[1]).T
+    assert np.allclose(np.linalg.norm(axes, axis=1), 1)
+    # rings: the layout is rotated so that the original z axis points along x
+    assert np.allclose(sorted(np.round(np.abs(axes[:, 0]), 6)), sorted(np.round(np.abs(np.cos(np.deg2rad([32.5] * 8 + [65] * 8))), 6)))
+    pmt_area = (75e-3 / 2) ** 2 * np.pi
+    tgt = nt.MultiPMTDetector(np.zeros(3), 0.21, pmt_area, pc)
+    opening = math.asin(math.sqrt(pmt_area / math.pi) / 0.21)
+    assert math.degrees(opening) == pytest.approx(10.29, abs=0.01)
+    for j in range(16):
+        hit = axes[j] * 0.21
+        assert nt.check_pmt_hit(hit[None], tgt)[0] == j + 1
+        assert oracle.ntor_check_pmt_hit(O.dptr(axes[j]), O.dptr(axes), 16, opening) == j + 1
+    rng = np.random.default_rng(7)
+    rel = rng.normal(size=(500, 3)); rel /= np.linalg.norm(rel, axis=1, keepdims=True)
+    mine = nt.check_pmt_hit(rel * 0.21, tgt)
+    ref = [oracle.ntor_check_pmt_hit(O.dptr(r), O.dptr(axes), 16, opening) for r in rel]
+    assert list(mine) == ref
+    # covered solid-angle fraction: 16 caps of half-angle `opening`
+    cover = 16 * (1 - math.cos(opening)) / 2
+    big = rng.normal(size=(200000, 3))
+    assert np.mean(nt.check_pmt_hit(big, tgt) > 0) == pytest.approx(cover, rel=0.03)
+
+
+# ---------------------------------------------------------------- propagation invariants
+def _setup(sources, targets, medium=None, spectrum=None):
+    medium = medium or cascadia()
+    spectrum = spectrum or nt.Monochromatic(400.0)
+    return nt.PhotonPropSetup(sources, targets, medium, spectrum)
+
+
+def test_propagate_empty_and_no_targets():
+    s = _setup([nt.PointlikeIsotropicEmitter(np.zeros(3), 0.0, 0)],
+               [nt.DetectionSphere(np.array([0, 0, 10.0]), 0.21, 16, 0.0044)])
+    h = O.propagate(s)
+    assert len(h) == 0 and h.n_simulated == 0
+    s = _setup([nt.PointlikeIsotropicEmitter(np.zeros(3), 0.0, 5000)], [])
+    h = O.propagate(s)
+    assert len(h) == 0 and h.n_simulated == 5000
+
+
+def test_propagate_unscattered_solid_angle():
+    # no scattering within reach (L_sca ×1e6): the hit fraction is the solid
+    # angle of the sphere, (1 - sqrt(1 - (r/D)^2))/2, and every hit lies on
+    # the sphere at a time dist/c_grp
+    med = cascadia(sca_len_scale=1e6)
+    D, r = 10.0, 1.0
+    tgt = nt.DetectionSphere(np.array([0, 0, D]), r, 1, 1.0)
+    s = _setup([nt.PointlikeIsotropicEmitter(np.zeros(3), 0.0, 400000)], [tgt], med)
+    h = O.propagate(s)
+    p = (1 - math.sqrt(1 - (r / D) ** 2)) / 2
+    n = 400000
+    assert abs(len(h) - n * p) < 4 * math.sqrt(n * p)
+    assert np.allclose(np.linalg.norm(h.position - np.array([0, 0, D]), axis=1), r, atol=2e-5)
+    assert np.allclose(h.dist_travelled, np.linalg.norm(h.position, axis=1), rtol=1e-5)
+    cg = float(nt.group_velocity(np.float32(400.0), med))
+    assert np.allclose(h.time, h.dist_travelled / cg, rtol=1e-5)
+
+
+def test_propagate_pencil_beam():
+    med = cascadia(sca_len_scale=1e6)
+    src = nt.PencilEmitter(np.array([1.0, 2.0, -5.0]), np.array([0, 0, 1.0]), 3.0, 1000)
+    s = _setup([src], [nt.DetectionSphere(np.array([1.0, 2.0, 5.0]), 0.5, 1, 1.0, 9)], med)
+    h = O.propagate(s)
+    assert len(h) == 1000 and set(h.module_id) == {9}
+    assert np.allclose(h.dist_travelled, 9.5, atol=1e-4)
+    cg = float(nt.group_velocity(np.float32(400.0), med))
+    assert np.allclose(h.time, 3.0 + 9.5 / cg, atol=1e-4)
+    assert np.all(h.direction == h.initial_direction)
+
+
+def test_propagate_first_listed_target_wins():
+    med = cascadia(sca_len_scale=1e6)
+    src = nt.PencilEmitter(np.zeros(3), np.array([0, 0, 1.0]), 0.0, 100)
+    near = nt.DetectionSphere(np.array([0, 0, 5.0]), 0.5, 1, 1.0, 1)
+    far = nt.DetectionSphere(np.array([0, 0, 8.0]), 0.5, 1, 1.0, 2)
+    assert set(O.propagate(_setup([src], [far, near], med)).module_id) == {2}
+    assert set(O.propagate(_setup([src], [near, far], med)).module_id) == {1}
+
+
+def test_propagate_deterministic_and_seeded():
+    s = _setup([nt.PointlikeIsotropicEmitter(np.zeros(3), 0.0, 200000)],
+               [nt.DetectionSphere(np.array([0, 0, 5.0]), 0.5, 1, 1.0)])
+    a = O.propagate(s, seed=1, threads=1)
+    b = O.propagate(s, seed=1, threads=3)
+    c = O.propagate(s, seed=2)
+    assert np.array_equal(a.photon_id, b.photon_id) and np.array_equal(a.time, b.time)
+    assert not np.array_equal(a.photon_id[:10], c.photon_id[:10])
+
+
+def test_propagate_overflow():
+    s = _setup([nt.PointlikeIsotropicEmitter(np.zeros(3), 0.0, 100000)],
+               [nt.DetectionSphere(np.array([0, 0, 3.0]), 1.0, 1, 1.0)])
+    full = O.propagate(s)
+    small = O.propagate(s, capacity=10)
+    assert len(small) == 10 and len(full) > 10
+    assert np.array_equal(small.photon_id, full.photon_id[:10])
