@@ -37,11 +37,14 @@ PMT_AREA = (75e-3 / 2) ** 2 * math.pi
 
 # Algorithmic work (counting rule: add/mul/cmp/select 1, fma 2, div 1,
 # transcendental 1): per scattering step 61 FLOP + 4 transcendentals besides
-# the sphere tests, 18 FLOP per sphere test; per photon initialisation 49 FLOP
+# the sphere tests, 18 FLOP per sphere test, 27 FLOP per string-bound test
+# (the kernel tests a step against each vertical string's bounding cylinder
+# before its spheres); per photon initialisation 49 FLOP
 # + 9 transcendentals (isotropic) and 95 FLOP + 20 transcendentals more for an
 # extended Cherenkov emitter.
 FLOP_STEP = 61 + 4
 FLOP_TEST = 18
+FLOP_STRING_BOUND = 27   # z-range + segment/axis distance test of one string
 FLOP_INIT = {"isotropic": 49 + 9, "cascade": 49 + 9 + 95 + 20}
 
 
@@ -97,6 +100,11 @@ def main():
     ap.add_argument("--workload", default="cascade", choices=["cascade", "isotropic"])
     ap.add_argument("--photons", type=int, default=10_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 256 MiB device-to-device copy 5 times (reference for "
+                         "FETCH_SIZE/WRITE_SIZE counter scaling)")
+    ap.add_argument("--no-culling", action="store_true",
+                    help="test every sphere on every step instead of string bounds first")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU per process; multi-GPU runs are not "
@@ -104,6 +112,14 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_calibration:
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": a.numel() * 4, "dispatches": 5}))
+        return
     from neutrinotelescopes_amd import _abi
     from neutrinotelescopes_amd.propagation import PropagationEngine
 
@@ -111,6 +127,8 @@ def main():
     setup = build_workload(args.workload, args.photons)
     eng = PropagationEngine(0)
     eng.setup(setup)
+    _abi.check(eng.lib.nt_set_option(eng.handle, _abi.NT_OPT_TARGET_CULLING, 0 if args.no_culling else 1),
+               "nt_set_option")
     cap = 1 << 22
     cols = dict(position=torch.empty(cap, 3, device=dev), direction=torch.empty(cap, 3, device=dev),
                 initial_direction=torch.empty(cap, 3, device=dev), wavelength=torch.empty(cap, device=dev),
@@ -135,6 +153,7 @@ def main():
     c = counters.cpu().numpy()
     hits_per_step = int(c[0])
     tests_per_step = int(c[2])
+    bound_tests_per_step = int(c[3])
 
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
     torch.cuda.synchronize()
@@ -150,8 +169,12 @@ def main():
     assert int(c[1]) == args.photons, c
 
     n_targets = len(setup.targets)
-    steps_total = tests_per_step / max(n_targets, 1)
-    flop = args.photons * FLOP_INIT[args.workload] + steps_total * FLOP_STEP + tests_per_step * FLOP_TEST
+    n_strings = len({(float(t.position[0]), float(t.position[1])) for t in setup.targets})
+    # every step tests every string bound (culling on) or every sphere (off)
+    steps_total = (bound_tests_per_step / n_strings if bound_tests_per_step
+                   else tests_per_step / max(n_targets, 1))
+    flop = (args.photons * FLOP_INIT[args.workload] + steps_total * FLOP_STEP
+            + tests_per_step * FLOP_TEST + bound_tests_per_step * FLOP_STRING_BOUND)
     k_ms = float(np.mean(kernel_ms))
     achieved = flop / (k_ms * 1e-3) / 1e12
     value = args.photons * args.steps / wall
@@ -172,8 +195,10 @@ def main():
         "config": {"workload": "cherenkov_cascade_1e4GeV_1e7ph_20x16pmt_string" if args.workload == "cascade"
                    else "isotropic_point_source_1_sphere_10m", "photons_per_step": args.photons,
                    "targets": n_targets, "hits_per_step": hits_per_step,
+                   "string_culling": not args.no_culling,
                    "scattering_steps_per_photon": steps_total / args.photons,
-                   "sphere_tests_per_step": tests_per_step / max(steps_total, 1)},
+                   "sphere_tests_per_scattering_step": tests_per_step / max(steps_total, 1),
+                   "string_bound_tests_per_scattering_step": bound_tests_per_step / max(steps_total, 1)},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
                      "kernel": "propagate_kernel", "kernel_ms": k_ms,

@@ -189,10 +189,18 @@ int32_t nt_propagate(nt_ctx* ctx, uint64_t seed, const nt_hits* out_host,
 
 /* Asynchronous device form: hits go to caller-owned DEVICE columns, counters to
    `d_counters` (device int64[4]: total hits found, photons simulated, sphere
-   tests, reserved), which the call clears first.  `stream` is a hipStream_t
+   tests, string-bound tests), which the call clears first.  `stream` is a hipStream_t
    (NULL = default stream).  Hits beyond capacity are counted but not stored. */
 int32_t nt_launch(nt_ctx* ctx, uint64_t seed, const nt_hits* out_dev,
                   int64_t* d_counters, void* stream);
+
+/* Options (no reference counterpart).
+   NT_OPT_TARGET_CULLING (default 1): test a step against the bounding
+   cylinder of each vertical string of targets (targets sharing x, y) before
+   its spheres.  The bounds are inflated beyond the FP32 rounding of the sphere
+   test, so results are identical to testing every target in list order (0). */
+enum { NT_OPT_TARGET_CULLING = 0 };
+int32_t nt_set_option(nt_ctx* ctx, int32_t option, int64_t value);
 
 /* Grid used by nt_launch (blocks, threads per block). */
 int32_t nt_launch_shape(nt_ctx* ctx, int32_t* blocks, int32_t* threads);

@@ -41,6 +41,8 @@ NT_TEX_TEXEL_CENTER_CLAMP = 0
 NT_TEX_TEXEL_CENTER_CLAMP_Q8 = 1
 NT_TEX_ENDPOINT = 2
 
+NT_OPT_TARGET_CULLING = 0
+
 NT_TARGET_SPHERE = 0
 NT_TARGET_MULTI_PMT = 1
 NT_MAX_PMTS = 16
@@ -138,6 +140,7 @@ SIGNATURES = {
     "nt_propagate": (C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(nt_hits), C.POINTER(C.c_int64),
                                  C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "nt_launch": (C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(nt_hits), C.c_void_p, C.c_void_p]),
+    "nt_set_option": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int64]),
     "nt_launch_shape": (C.c_int32, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "nt_eval": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
                             C.c_int64]),
@@ -156,6 +159,14 @@ def load_library(path: Path | None = None) -> C.CDLL:
         raise RuntimeError(
             f"{p} is not built: run neutrinotelescopes_amd._build.build_library() "
             "(or __graft_entry__.build()) first")
+    # One HIP runtime per process: torch ships its own libamdhip64 (same
+    # SONAME).  Loading torch first makes the dynamic loader bind
+    # libntprop.so to that runtime, so torch tensors, streams and events and
+    # this library share one device context.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(str(p))
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

@@ -60,21 +60,148 @@ struct HitCols {
     int64_t capacity;
 };
 
+// Target clusters.  Targets sharing (x, y) form a vertical string; a photon
+// step is tested against a string's bounding cylinder first and against its
+// spheres only when the step's segment comes within reach.  Per cluster:
+//   geometry float4 (axis x, axis y, lowest centre z, highest centre z)
+//   largest radius (float), member range int2 (first, count)
+// Members (targets reordered by cluster): float4 (x, y, z, r^2) and their
+// list index, so the first listed target still wins.
 struct KParams {
     const DevSource* __restrict__ sources;
     const int64_t* __restrict__ offsets;     // n_sources + 1
-    const float4* __restrict__ targets;      // x, y, z, r^2
-    const uint32_t* __restrict__ module_ids;
+    const float4* __restrict__ targets;      // x, y, z, r^2 in list order
+    const uint32_t* __restrict__ module_ids; // per list index
+    const float4* __restrict__ clusters;     // geometry per cluster
+    const float* __restrict__ cluster_rmax;  // largest radius per cluster
+    const int2* __restrict__ cluster_range;  // (first member, count) per cluster
+    const float4* __restrict__ members;      // x, y, z, r^2 in cluster order
+    const uint32_t* __restrict__ member_idx; // list index of each member
     const float* __restrict__ knots;
-    int64_t* counters;                       // [0] hits found, [1] photons, [2] sphere tests
+    int64_t* counters;                       // [0] hits found, [1] photons, [2] sphere tests, [3] cluster tests
     HitCols hits;
     int64_t total;
     Medium medium;
     Spectrum spectrum;
     int n_sources;
     int n_targets;
+    int n_clusters;                          // 0: test every target in list order
     uint32_t seed_lo, seed_hi;
 };
+
+// Exact reference sphere test (photon_prop_cuda.jl:248-269) on one target.
+__device__ __forceinline__ bool sphere_hit(float3 pos, float3 dir, float step, float4 T, float& d) {
+    float a;
+    const float b = isect_b(pos, dir, T, a);
+    if (b >= 0.0f) {
+        const float dd = -a - sqrtf(b);
+        if ((dd > 0.0f) & (dd < step)) {
+            d = dd;
+            return true;
+        }
+    }
+    return false;
+}
+
+// First listed target hit by the step [pos, pos + step·dir]: returns its list
+// index or -1.  Brute force over the list (n_clusters == 0) or through the
+// cluster bounds.  The bounds are inflated by m with
+// m = 2^-10·(step + r_max + 1) + 1e-3 m: the FP32 discriminant of the sphere
+// test can accept rays passing up to ~sqrt(5·2^-23)·|p − c| beyond the radius,
+// and |p − c| ≤ step + r for any accepted hit, so a culled target can never be
+// one the exact test would accept.
+__device__ __forceinline__ int first_hit(const KParams& p, float3 pos, float3 dir, float step, float& dt,
+                                         int64_t& tests, int64_t& ctests) {
+    int best = 0x7fffffff;
+    if (p.n_clusters == 0) {
+        const const_f4_ptr tgt = (const_f4_ptr)(const void*)p.targets;
+        for (int t = 0; t < p.n_targets; ++t) {
+            const f4 tv = tgt[t];
+            float d;
+            if (sphere_hit(pos, dir, step, make_float4(tv.x, tv.y, tv.z, tv.w), d) && best == 0x7fffffff) {
+                best = t;
+                dt = d;
+            }
+        }
+        tests += p.n_targets;
+        return best == 0x7fffffff ? -1 : best;
+    }
+    typedef const __attribute__((address_space(4))) uint32_t* const_u32_ptr;
+    typedef const __attribute__((address_space(4))) float* const_f32_ptr;
+    typedef int i2 __attribute__((ext_vector_type(2)));
+    typedef const __attribute__((address_space(4))) i2* const_i2_ptr;
+    const const_f4_ptr cl = (const_f4_ptr)(const void*)p.clusters;
+    const const_f32_ptr cl_r = (const_f32_ptr)(const void*)p.cluster_rmax;
+    const const_i2_ptr cl_k = (const_i2_ptr)(const void*)p.cluster_range;
+    const const_f4_ptr mem = (const_f4_ptr)(const void*)p.members;
+    const const_u32_ptr midx = (const_u32_ptr)(const void*)p.member_idx;
+    const float z1 = pos.z + dir.z * step;
+    const float zmin = fminf(pos.z, z1), zmax = fmaxf(pos.z, z1);
+    const float dx = dir.x * step, dy = dir.y * step;
+    const float dd = dx * dx + dy * dy;
+    const float inv_dd = dd > 0.0f ? __builtin_amdgcn_rcpf(dd) : 0.0f;
+    for (int c = 0; c < p.n_clusters; ++c) {
+        const f4 c0 = cl[c];
+        const float rmax = cl_r[c];
+        const i2 range = cl_k[c];
+        const int first = range.x;
+        const int count = range.y;
+        if (first < 0 || count < 0 || first + count > p.n_targets) continue;  // corrupt table guard
+        ++ctests;
+        const float re = rmax + 0x1p-10f * (step + rmax + 1.0f) + 1e-3f;
+        if (zmax < c0.z - re || zmin > c0.w + re) continue;
+        const float wx = c0.x - pos.x, wy = c0.y - pos.y;
+        const float t = fminf(fmaxf((wx * dx + wy * dy) * inv_dd, 0.0f), 1.0f);
+        const float ex = wx - t * dx, ey = wy - t * dy;
+        if (ex * ex + ey * ey > re * re) continue;
+        for (int k = first; k < first + count; ++k) {
+            const f4 mv = mem[k];
+            if (zmax < mv.z - re || zmin > mv.z + re) continue;
+            ++tests;
+            float d;
+            if (sphere_hit(pos, dir, step, make_float4(mv.x, mv.y, mv.z, mv.w), d)) {
+                const int li = (int)midx[k];
+                if (li < best) {
+                    best = li;
+                    dt = d;
+                }
+            }
+        }
+    }
+    return best == 0x7fffffff ? -1 : best;
+}
+
+// Kernel arguments used only off the step loop (photon initialisation, hit
+// flush) are re-read from the kernarg segment where they are used, through a
+// pointer the compiler cannot match with the loads at kernel entry: their
+// SGPRs are then live only there, not across the whole persistent loop.
+typedef const __attribute__((address_space(4))) KParams* const_kparams_ptr;
+__device__ __forceinline__ const_kparams_ptr kargs() {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const_kparams_ptr k = (const_kparams_ptr)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(k));
+    return k;
+#else
+    return nullptr;   // host pass: never called
+#endif
+}
+
+__device__ __forceinline__ Medium load_medium(const_kparams_ptr k) {
+    Medium m;
+    m.a01 = k->medium.a01; m.a2 = k->medium.a2; m.a3 = k->medium.a3; m.a4 = k->medium.a4;
+    m.vcs = k->medium.vcs; m.vcl = k->medium.vcl; m.g = k->medium.g;
+    m.sl_fraction = k->medium.sl_fraction; m.sca_len_scale = k->medium.sca_len_scale;
+    m.dispersion_mode = k->medium.dispersion_mode; m.max_steps = k->medium.max_steps;
+    m.track_norm = k->medium.track_norm;
+    return m;
+}
+
+__device__ __forceinline__ Spectrum load_spectrum(const_kparams_ptr k) {
+    Spectrum s;
+    s.kind = k->spectrum.kind; s.addressing = k->spectrum.addressing; s.n = k->spectrum.n;
+    s.wl = k->spectrum.wl; s.lo = k->spectrum.lo; s.hi = k->spectrum.hi;
+    return s;
+}
 
 __device__ __forceinline__ int find_source(const int64_t* __restrict__ offs, int n, int64_t g) {
     int lo = 0, hi = n - 1;            // largest s with offs[s] <= g
@@ -86,14 +213,18 @@ __device__ __forceinline__ int find_source(const int64_t* __restrict__ offs, int
 }
 
 // Write the staged hits of this wavefront to the output columns.
-__device__ __forceinline__ void flush_stage(const KParams& p, const uint32_t* __restrict__ st, int n,
-                                            int lane) {
+__device__ __forceinline__ void flush_stage(const uint32_t* __restrict__ st, int n, int lane) {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
+    const const_kparams_ptr kp = kargs();
     unsigned long long base = 0;
-    if (lane == 0) base = atomicAdd((unsigned long long*)&p.counters[0], (unsigned long long)n);
+    if (lane == 0) base = atomicAdd((unsigned long long*)&kp->counters[0], (unsigned long long)n);
     base = __shfl(base, 0);
-    const HitCols& h = p.hits;
+    HitCols h;
+    h.pos = kp->hits.pos; h.dir = kp->hits.dir; h.idir = kp->hits.idir; h.wl = kp->hits.wl;
+    h.time = kp->hits.time; h.dist = kp->hits.dist; h.module_id = kp->hits.module_id;
+    h.source_index = kp->hits.source_index; h.photon_id = kp->hits.photon_id;
+    h.capacity = kp->hits.capacity;
     for (int i = lane; i < n; i += 64) {
         const int64_t idx = (int64_t)base + i;
         if (idx >= h.capacity) continue;
@@ -120,7 +251,14 @@ __device__ __forceinline__ void flush_stage(const KParams& p, const uint32_t* __
     __builtin_amdgcn_wave_barrier();
 }
 
-__global__ __launch_bounds__(kThreads) void propagate_kernel(KParams p) {
+#ifndef NTP_MIN_WAVES_PER_SIMD
+#define NTP_MIN_WAVES_PER_SIMD 6
+#endif
+
+// KIND >= 0: every source of the launch has that kind (smaller code, fewer
+// registers); KIND < 0: mixed kinds, dispatched per photon.
+template <int KIND>
+__global__ __launch_bounds__(kThreads, NTP_MIN_WAVES_PER_SIMD) void propagate_kernel(KParams p) {
     extern __shared__ float lds[];
     float* s_knots = lds;                                          // n_knots (padded to 4)
     uint32_t* s_stage = reinterpret_cast<uint32_t*>(lds + ((p.spectrum.n + 3) & ~3));
@@ -134,6 +272,7 @@ __global__ __launch_bounds__(kThreads) void propagate_kernel(KParams p) {
     int staged = 0;                                                // wave-uniform
     int64_t simulated = 0;
     int64_t tests = 0;
+    int64_t ctests = 0;
 
     const Medium& m = p.medium;
     const int64_t stride = (int64_t)gridDim.x * kThreads;
@@ -148,23 +287,26 @@ __global__ __launch_bounds__(kThreads) void propagate_kernel(KParams p) {
         uint32_t mid = 0, sidx = 0;
         uint64_t pid = 0;
         if (g < p.total) {
+            const const_kparams_ptr kp = kargs();
+            const Medium mi = load_medium(kp);      // initialisation-only fields
+            const int64_t* offsets = kp->offsets;
             if (g < src_lo || g >= src_hi) {
-                src = find_source(p.offsets, p.n_sources, g);
-                src_lo = p.offsets[src];
-                src_hi = p.offsets[src + 1];
+                src = find_source(offsets, kp->n_sources, g);
+                src_lo = offsets[src];
+                src_hi = offsets[src + 1];
             }
-            const DevSource& s = p.sources[src];
+            const DevSource& s = kp->sources[src];
             pid = (uint64_t)(g - src_lo);
             sidx = s.index;
             const Rng rng{p.seed_lo, p.seed_hi, (uint32_t)pid, (uint32_t)(pid >> 32), sidx};
 
             // initialize_photon_state, photon_prop_cuda.jl:133-188
             const uint4 r0 = rng.block(0u);
-            wl = sample_spectrum(p.spectrum, s_knots, u01(r0.x));
+            wl = sample_spectrum(load_spectrum(kp), s_knots, u01(r0.x));
             pos = make_float3(s.pos[0], s.pos[1], s.pos[2]);
             const float3 sdir = make_float3(s.dir[0], s.dir[1], s.dir[2]);
             time = s.t0;
-            switch (s.kind) {
+            switch (KIND < 0 ? s.kind : KIND) {
                 case NT_SRC_POINTLIKE_ISOTROPIC:
                     dir = iso_dir(u01(r0.y), u01(r0.z));
                     break;
@@ -182,48 +324,34 @@ __global__ __launch_bounds__(kThreads) void propagate_kernel(KParams p) {
                     const float lp = rand_gamma(s.gamma_d, s.gamma_c, s.gamma_scale, rng, nullptr) * s.lrad;
                     pos = make_float3(pos.x + lp * sdir.x, pos.y + lp * sdir.y, pos.z + lp * sdir.z);
                     time = s.t0 + lp / kCVacMNs;
-                    dir = cherenkov_dir(wl, u01(r0.y), u01(r0.z), u01(r0.w), sdir, m);
+                    dir = cherenkov_dir(wl, u01(r0.y), u01(r0.z), u01(r0.w), sdir, mi);
                     break;
                 }
                 default:  // NT_SRC_POINTLIKE_CHERENKOV
-                    dir = cherenkov_dir(wl, u01(r0.y), u01(r0.z), u01(r0.w), sdir, m);
+                    dir = cherenkov_dir(wl, u01(r0.y), u01(r0.z), u01(r0.w), sdir, mi);
                     break;
             }
             idir = dir;
-            const float sca_len = scattering_length(wl, m);
-            const float c_grp = group_velocity(wl, m);
+            const float sca_len = scattering_length(wl, mi);
+            const float c_grp = group_velocity(wl, mi);
 
             for (int k = 0; k < m.max_steps; ++k) {
                 const uint4 r = rng.block(1u + (uint32_t)k);
                 const float step = -logf(u01_oc(r.x)) * sca_len;
                 // first listed target hit within this step (photon_prop_cuda.jl:457-468)
-                int hit_t = -1;
                 float dt = 0.0f;
-                const const_f4_ptr tgt = (const_f4_ptr)(const void*)p.targets;
-                for (int t = 0; t < p.n_targets; ++t) {
-                    const f4 tv = tgt[t];
-                    const float4 T = make_float4(tv.x, tv.y, tv.z, tv.w);
-                    float a;
-                    const float b = isect_b(pos, dir, T, a);
-                    if (b >= 0.0f && hit_t < 0) {
-                        const float d = -a - sqrtf(b);
-                        if ((d > 0.0f) & (d < step)) {
-                            hit_t = t;
-                            dt = d;
-                        }
-                    }
-                }
-                tests += p.n_targets;
+                const int hit_t = first_hit(p, pos, dir, step, dt, tests, ctests);
                 if (hit_t < 0) {
                     pos = make_float3(pos.x + dir.x * step, pos.y + dir.y * step, pos.z + dir.z * step);
                     dist += step;
-                    time += step / c_grp;
                     dir = update_direction(dir, u01(r.y), u01(r.z), u01(r.w), m);
                     continue;
                 }
                 pos = make_float3(pos.x + dir.x * dt, pos.y + dir.y * dt, pos.z + dir.z * dt);
                 dist += dt;
-                time += dt / c_grp;
+                // Σ step/c_grp + dt/c_grp of photon_prop_cuda.jl:473,481 taken at
+                // once: the same sum up to FP32 rounding of the partial sums
+                time += dist / c_grp;
                 mid = p.module_ids[hit_t];
                 hit = true;
                 break;
@@ -236,7 +364,7 @@ __global__ __launch_bounds__(kThreads) void propagate_kernel(KParams p) {
         if (mask) {
             const int cnt = __popcll(mask);
             if (staged + cnt > kStageSlots) {
-                flush_stage(p, st, staged, lane);
+                flush_stage(st, staged, lane);
                 staged = 0;
             }
             if (hit) {
@@ -264,16 +392,33 @@ __global__ __launch_bounds__(kThreads) void propagate_kernel(KParams p) {
             staged += cnt;
         }
     }
-    if (staged) flush_stage(p, st, staged, lane);
+    if (staged) flush_stage(st, staged, lane);
 
     // one atomic per wavefront for the photon and test counters
     for (int off = 32; off > 0; off >>= 1) {
         simulated += __shfl_down(simulated, off);
         tests += __shfl_down(tests, off);
+        ctests += __shfl_down(ctests, off);
     }
     if (lane == 0) {
         atomicAdd((unsigned long long*)&p.counters[1], (unsigned long long)simulated);
         atomicAdd((unsigned long long*)&p.counters[2], (unsigned long long)tests);
+        atomicAdd((unsigned long long*)&p.counters[3], (unsigned long long)ctests);
+    }
+}
+
+typedef void (*PropagateFn)(KParams);
+constexpr int kKernelVariants = 7;     // one per source kind + mixed
+
+PropagateFn propagate_variant(int v) {
+    switch (v) {
+        case NT_SRC_POINTLIKE_ISOTROPIC: return propagate_kernel<NT_SRC_POINTLIKE_ISOTROPIC>;
+        case NT_SRC_POINTLIKE_TIME_RANGE: return propagate_kernel<NT_SRC_POINTLIKE_TIME_RANGE>;
+        case NT_SRC_AXICONE: return propagate_kernel<NT_SRC_AXICONE>;
+        case NT_SRC_PENCIL: return propagate_kernel<NT_SRC_PENCIL>;
+        case NT_SRC_EXTENDED_CHERENKOV: return propagate_kernel<NT_SRC_EXTENDED_CHERENKOV>;
+        case NT_SRC_POINTLIKE_CHERENKOV: return propagate_kernel<NT_SRC_POINTLIKE_CHERENKOV>;
+        default: return propagate_kernel<-1>;
     }
 }
 
@@ -371,7 +516,9 @@ struct nt_ctx {
     int device = 0;
     int num_cus = 0;
     hipStream_t stream = nullptr;
-    int blocks = 0;
+    int blocks = 0;                    // grid of the mixed-kind kernel
+    int blocks_per_variant[7] = {};
+    int variant = 6;                   // source kind shared by all sources, 6 = mixed
     std::mutex mu;
 
     bool has_medium = false, has_spectrum = false, has_targets = false, has_sources = false;
@@ -384,6 +531,15 @@ struct nt_ctx {
     float4* d_targets = nullptr;
     uint32_t* d_module_ids = nullptr;
     int targets_cap = 0;
+    // cluster index over the targets (vertical strings)
+    bool culling = true;
+    int n_clusters = 0;
+    float4* d_clusters = nullptr;
+    float* d_cluster_rmax = nullptr;
+    int2* d_cluster_range = nullptr;
+    float4* d_members = nullptr;
+    uint32_t* d_member_idx = nullptr;
+    int64_t clusters_cap = 0, rmax_cap = 0, range_cap = 0, members_cap = 0, member_idx_cap = 0;
 
     int n_sources = 0;
     DevSource* d_sources = nullptr;
@@ -446,6 +602,12 @@ int32_t launch(nt_ctx* c, uint64_t seed, const HitCols& cols, int64_t* d_counter
     kp.offsets = c->d_offsets;
     kp.targets = c->d_targets;
     kp.module_ids = c->d_module_ids;
+    kp.clusters = c->d_clusters;
+    kp.cluster_rmax = c->d_cluster_rmax;
+    kp.cluster_range = c->d_cluster_range;
+    kp.members = c->d_members;
+    kp.member_idx = c->d_member_idx;
+    kp.n_clusters = c->culling ? c->n_clusters : 0;
     kp.knots = c->d_knots;
     kp.counters = d_counters;
     kp.hits = cols;
@@ -457,8 +619,9 @@ int32_t launch(nt_ctx* c, uint64_t seed, const HitCols& cols, int64_t* d_counter
     kp.seed_lo = (uint32_t)seed;
     kp.seed_hi = (uint32_t)(seed >> 32);
     int64_t need_blocks = (c->total + kThreads - 1) / kThreads;
-    int blocks = (int)std::min<int64_t>(c->blocks, need_blocks);
-    hipLaunchKernelGGL(propagate_kernel, dim3(blocks), dim3(kThreads), lds_bytes(c->spectrum.n), s, kp);
+    int blocks = (int)std::min<int64_t>(c->blocks_per_variant[c->variant], need_blocks);
+    hipLaunchKernelGGL(propagate_variant(c->variant), dim3(blocks), dim3(kThreads), lds_bytes(c->spectrum.n),
+                       s, kp);
     HIP_TRY(hipGetLastError());
     return NT_OK;
 }
@@ -499,11 +662,14 @@ int32_t nt_create(int32_t device, nt_ctx** out) {
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete c; return NT_ERR_DEVICE; }
     c->num_cus = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return NT_ERR_DEVICE; }
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, propagate_kernel, kThreads,
-                                                     lds_bytes(NT_MAX_KNOTS)) != hipSuccess || per_cu < 1)
-        per_cu = 1;
-    c->blocks = c->num_cus * per_cu;
+    for (int v = 0; v < kKernelVariants; ++v) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, propagate_variant(v), kThreads,
+                                                         lds_bytes(NT_MAX_KNOTS)) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        c->blocks_per_variant[v] = c->num_cus * per_cu;
+    }
+    c->blocks = c->blocks_per_variant[kKernelVariants - 1];
     if (hipMalloc((void**)&c->d_counters, 4 * sizeof(int64_t)) != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
         delete c;
@@ -517,7 +683,9 @@ void nt_destroy(nt_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    void* ptrs[] = {c->d_knots, c->d_targets, c->d_module_ids, c->d_sources, c->d_offsets, c->d_counters};
+    void* ptrs[] = {c->d_knots, c->d_targets, c->d_module_ids, c->d_sources, c->d_offsets, c->d_counters,
+                    c->d_clusters, c->d_cluster_rmax, c->d_cluster_range, c->d_members,
+                    c->d_member_idx};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     free_hits(c->d_hits);
@@ -544,6 +712,9 @@ int32_t nt_set_medium(nt_ctx* c, const nt_medium* m) {
     d.sca_len_scale = m->sca_len_scale;
     d.dispersion_mode = m->dispersion_mode;
     d.max_steps = m->max_steps;
+    // normaliser of the Cherenkov track-direction law (utils.jl:251-253),
+    // a constant: evaluated once here in FP32 with the host's libm
+    d.track_norm = 1.0f - expf(-2.61f * powf(2.0f, 0.39f));
     c->has_medium = true;
     return NT_OK;
 }
@@ -617,6 +788,61 @@ int32_t nt_set_targets(nt_ctx* c, const nt_target* t, int32_t n) {
         HIP_TRY(hipMemcpy(c->d_targets, tg.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(c->d_module_ids, ids.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
     }
+    // Clusters: targets with identical (x, y) form a vertical string; a
+    // string of one is tested directly.  Members keep list order inside a
+    // cluster, clusters are ordered by their first member.
+    std::vector<int> cluster_of(n, -1);
+    std::vector<std::vector<int>> groups;
+    for (int i = 0; i < n; ++i) {
+        int g = -1;
+        for (size_t k = 0; k < groups.size() && g < 0; ++k) {
+            const int j = groups[k][0];
+            if (tg[j].x == tg[i].x && tg[j].y == tg[i].y) g = (int)k;
+        }
+        if (g < 0) { groups.push_back({}); g = (int)groups.size() - 1; }
+        groups[g].push_back(i);
+        cluster_of[i] = g;
+    }
+    std::vector<float4> cl, mem;
+    std::vector<float> cl_rmax;
+    std::vector<int2> cl_range;
+    std::vector<uint32_t> midx;
+    for (const auto& grp : groups) {
+        float zlo = INFINITY, zhi = -INFINITY, rmax = 0.0f;
+        for (int i : grp) {
+            zlo = std::min(zlo, tg[i].z);
+            zhi = std::max(zhi, tg[i].z);
+            rmax = std::max(rmax, t[i].radius);
+        }
+        const int first = (int)mem.size(), count = (int)grp.size();
+        cl.push_back(make_float4(tg[grp[0]].x, tg[grp[0]].y, zlo, zhi));
+        cl_rmax.push_back(rmax);
+        cl_range.push_back(make_int2(first, count));
+        for (int i : grp) {
+            mem.push_back(tg[i]);
+            midx.push_back((uint32_t)i);
+        }
+    }
+    const int nc = (int)groups.size();
+    st = grow(&c->d_clusters, std::max(nc, 1), &c->clusters_cap);
+    if (st) return st;
+    st = grow(&c->d_cluster_rmax, std::max(nc, 1), &c->rmax_cap);
+    if (st) return st;
+    st = grow(&c->d_cluster_range, std::max(nc, 1), &c->range_cap);
+    if (st) return st;
+    st = grow(&c->d_members, std::max(n, 1), &c->members_cap);
+    if (st) return st;
+    st = grow(&c->d_member_idx, std::max(n, 1), &c->member_idx_cap);
+    if (st) return st;
+    if (nc) {
+        HIP_TRY(hipMemcpy(c->d_clusters, cl.data(), sizeof(float4) * cl.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(c->d_cluster_rmax, cl_rmax.data(), sizeof(float) * nc, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(c->d_cluster_range, cl_range.data(), sizeof(int2) * nc, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(c->d_members, mem.data(), sizeof(float4) * mem.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(c->d_member_idx, midx.data(), sizeof(uint32_t) * midx.size(), hipMemcpyHostToDevice));
+    }
+    // culling pays only when some cluster holds several targets
+    c->n_clusters = (nc < n) ? nc : 0;
     c->n_targets = n;
     c->has_targets = true;
     return NT_OK;
@@ -668,13 +894,27 @@ int32_t nt_set_sources(nt_ctx* c, const nt_source* s, int32_t n) {
     HIP_TRY(hipMemcpy(c->d_offsets, offs.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice));
     c->n_sources = n;
     c->total = offs[n];
+    c->variant = kKernelVariants - 1;
+    if (n > 0 && std::all_of(s, s + n, [&](const nt_source& a) { return a.kind == s[0].kind; }))
+        c->variant = s[0].kind;
     c->has_sources = true;
     return NT_OK;
 }
 
+int32_t nt_set_option(nt_ctx* c, int32_t option, int64_t value) {
+    if (!c) return NT_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    switch (option) {
+        case NT_OPT_TARGET_CULLING: c->culling = value != 0; return NT_OK;
+        default:
+            snprintf(g_err, sizeof g_err, "unknown option %d", option);
+            return NT_ERR_ARG;
+    }
+}
+
 int32_t nt_launch_shape(nt_ctx* c, int32_t* blocks, int32_t* threads) {
     if (!c) return NT_ERR_ARG;
-    if (blocks) *blocks = c->blocks;
+    if (blocks) *blocks = c->blocks_per_variant[c->variant];
     if (threads) *threads = kThreads;
     return NT_OK;
 }

@@ -60,6 +60,7 @@ struct Medium {
     float sca_len_scale;      // scattering-length multiplier (1 = reference)
     int dispersion_mode;
     int max_steps;
+    float track_norm;         // 1 − exp(−2.61·2^0.39) of utils.jl:253, evaluated on the host
 };
 
 // refractive_index_fry, medium.jl:234-244
@@ -157,7 +158,7 @@ __device__ __forceinline__ float3 cherenkov_dir(float wl, float u_phi, float u_t
     const float phi = 0.0f + u_phi * kTwoPi;
     const float3 ph = sph_to_cart(theta_c, phi);
     const float a = 0.39f, b = 2.61f;
-    const float I = 1.0f - expf(-b * powf(2.0f, a));
+    const float I = m.track_norm;
     const float t = -logf(1.0f - u_tcos * I) / b;
     const float ct = fmaxf(1.0f - powf(t, 1.0f / a), -1.0f);
     const float3 track = sph_to_cart(acosf(ct), kTwoPi * u_tphi);
