@@ -150,6 +150,13 @@ typedef struct nt_hits {
     uint16_t* module_id;             /* [capacity] */
     uint32_t* source_index;          /* [capacity] optional (NULL: not written) */
     uint64_t* photon_id;             /* [capacity] optional (NULL: not written) */
+    /* Written only while the hit epilogue is enabled (nt_set_epilogue); each
+       optional.  Replace the reference's host post-processing columns. */
+    uint8_t* pmt_id;                 /* [capacity] check_pmt_hit: 1-based PMT, 0 = none
+                                        (detection.jl:72-104; DetectionSphere: 1) */
+    float* total_weight;             /* [capacity] area_acc·wl_acc·exp(−dist/L_abs(λ))
+                                        (photon_prop_cuda.jl:850-882) */
+    float* tres;                     /* [capacity] time residual (photon_prop_cuda.jl:906-924) */
     int64_t capacity;
 } nt_hits;
 
@@ -201,6 +208,37 @@ int32_t nt_launch(nt_ctx* ctx, uint64_t seed, const nt_hits* out_dev,
    test, so results are identical to testing every target in list order (0). */
 enum { NT_OPT_TARGET_CULLING = 0 };
 int32_t nt_set_option(nt_ctx* ctx, int32_t option, int64_t value);
+
+/* ---- hit epilogue ------------------------------------------------------
+   Replaces make_hits_from_photons / calc_total_weight! / calc_time_residual!
+   (photon_prop_cuda.jl:850-924), which the reference runs on the host over
+   the returned hits.  Computed by the propagation kernel when it writes each
+   hit, into the optional pmt_id / total_weight / tres columns:
+     pmt_id       first PMT whose axis (the module's (θ, φ) PMT coordinates
+                  rotated by `orientation`) lies within asin(sqrt(area/π)/r)
+                  of the hit direction seen from the module centre, in FP64
+                  (detection.jl:88-104); DetectionSphere: 1
+     total_weight area_acc × wl_acc × exp(−dist/L_abs(λ)): area_acc =
+                  n_pmts·pmt_area/(4πr²) for DetectionSphere, 1 for
+                  MultiPMTDetector; L_abs linear in the absorption table
+                  (× abs_len_scale); wl_acc linear in the acceptance table
+     tres         time − (|source_position − module| − r)/(c/n(800 nm)),
+                  as in the reference with its first source's position
+   The tables and orientation are copied; call again after nt_set_targets
+   or nt_set_medium changes. */
+typedef struct nt_epilogue {
+    int32_t enabled;                 /* 0 disables the epilogue */
+    int32_t n_abs;                   /* absorption table rows (2..1024) */
+    const float* abs_wavelength;     /* nm, ascending (medium.jl:392-404) */
+    const float* abs_length;         /* m */
+    float abs_len_scale;             /* 1 = reference */
+    int32_t n_acc;                   /* PMT acceptance rows (2..1024) */
+    const double* acc_wavelength;    /* nm, ascending (detection.jl:148-163) */
+    const double* acc_value;
+    double orientation[9];           /* row-major rotation of the PMT axes */
+    float source_position[3];        /* time-residual origin */
+} nt_epilogue;
+int32_t nt_set_epilogue(nt_ctx* ctx, const nt_epilogue* epilogue);
 
 /* Grid used by nt_launch (blocks, threads per block). */
 int32_t nt_launch_shape(nt_ctx* ctx, int32_t* blocks, int32_t* threads);

@@ -119,7 +119,25 @@ class nt_hits(C.Structure):
         ("module_id", C.c_void_p),
         ("source_index", C.c_void_p),
         ("photon_id", C.c_void_p),
+        ("pmt_id", C.c_void_p),
+        ("total_weight", C.c_void_p),
+        ("tres", C.c_void_p),
         ("capacity", C.c_int64),
+    ]
+
+
+class nt_epilogue(C.Structure):
+    _fields_ = [
+        ("enabled", C.c_int32),
+        ("n_abs", C.c_int32),
+        ("abs_wavelength", C.POINTER(C.c_float)),
+        ("abs_length", C.POINTER(C.c_float)),
+        ("abs_len_scale", C.c_float),
+        ("n_acc", C.c_int32),
+        ("acc_wavelength", C.POINTER(C.c_double)),
+        ("acc_value", C.POINTER(C.c_double)),
+        ("orientation", C.c_double * 9),
+        ("source_position", C.c_float * 3),
     ]
 
 
@@ -140,6 +158,7 @@ SIGNATURES = {
     "nt_propagate": (C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(nt_hits), C.POINTER(C.c_int64),
                                  C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "nt_launch": (C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(nt_hits), C.c_void_p, C.c_void_p]),
+    "nt_set_epilogue": (C.c_int32, [C.c_void_p, C.POINTER(nt_epilogue)]),
     "nt_set_option": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int64]),
     "nt_launch_shape": (C.c_int32, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "nt_eval": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,

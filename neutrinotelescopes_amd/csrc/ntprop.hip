@@ -57,7 +57,32 @@ struct HitCols {
     uint16_t* module_id;
     uint32_t* source_index;
     uint64_t* photon_id;
+    uint8_t* pmt_id;
+    float* total_weight;
+    float* tres;
     int64_t capacity;
+};
+
+// Hit epilogue data of one target (list order), prepared on the host.
+struct EpiTarget {
+    double pos[3];
+    double area_acc;                   // area acceptance (detection.jl:139-146)
+    double tgeo;                       // geometric arrival time for tres (ns)
+    double cos_open;                   // cos of the PMT opening half-angle
+    int32_t kind;
+    int32_t n_pmts;
+    double axes[NT_MAX_PMTS][3];       // PMT axes after the orientation
+};
+
+struct Epilogue {
+    const EpiTarget* targets;
+    const float* abs_x;                // absorption table (FP32, as the reference interpolates it)
+    const float* abs_y;
+    const double* acc_x;               // PMT acceptance table (FP64)
+    const double* acc_y;
+    int n_abs, n_acc;
+    float abs_len_scale;
+    int enabled;
 };
 
 // Target clusters.  Targets sharing (x, y) form a vertical string; a photon
@@ -210,6 +235,50 @@ __device__ __forceinline__ int find_source(const int64_t* __restrict__ offs, int
         if (offs[mid] <= g) lo = mid; else hi = mid - 1;
     }
     return lo;
+}
+
+// fast_linear_interp(x, xs, ys), utils.jl:24-53, in the table's precision.
+template <typename T>
+__device__ __forceinline__ T interp_table(T x, const T* xs, const T* ys, int n) {
+    const T lo = xs[0], hi = xs[n - 1];
+    x = x < lo ? lo : (x > hi ? hi : x);
+    if (x == lo) return ys[0];
+    if (x == hi) return ys[n - 1];
+    int a = 0, b = n - 1;                 // first index with xs[i] >= x lies in (a, b]
+    while (b - a > 1) {
+        const int mid = (a + b) >> 1;
+        if (xs[mid] < x) a = mid; else b = mid;
+    }
+    const T along = (x - xs[a]) / (xs[b] - xs[a]);
+    return ys[a] + (ys[b] - ys[a]) * along;
+}
+
+// PMT assignment, weight and time residual of one hit (see nt_epilogue).
+__device__ __forceinline__ void hit_epilogue(const Epilogue& ep, int ti, float3 pos, float wl, float dist,
+                                          float time, uint8_t& pmt, float& weight, float& tres) {
+    const EpiTarget* T = ep.targets + ti;
+    // check_pmt_hit, detection.jl:88-104 (FP64 as in the reference)
+    if (T->kind == NT_TARGET_SPHERE) {
+        pmt = 1;
+    } else {
+        double r0 = (double)pos.x - T->pos[0], r1 = (double)pos.y - T->pos[1], r2 = (double)pos.z - T->pos[2];
+        const double nr = sqrt((r0 * r0 + r1 * r1) + r2 * r2);
+        r0 /= nr; r1 /= nr; r2 /= nr;
+        pmt = 0;
+        for (int j = 0; j < T->n_pmts; ++j) {
+            double c = (r0 * T->axes[j][0] + r1 * T->axes[j][1]) + r2 * T->axes[j][2];
+            c = c < -1.0 ? -1.0 : (c > 1.0 ? 1.0 : c);
+            if (c > T->cos_open) { pmt = (uint8_t)(j + 1); break; }   // acos(c) < opening
+        }
+    }
+    // calc_total_weight!, photon_prop_cuda.jl:850-882
+    float abs_len = interp_table(wl, ep.abs_x, ep.abs_y, ep.n_abs);
+    if (ep.abs_len_scale != 1.0f) abs_len *= ep.abs_len_scale;
+    const double abs_w = (double)expf(-dist / abs_len);
+    const double wl_acc = interp_table((double)wl, ep.acc_x, ep.acc_y, ep.n_acc);
+    weight = (float)(T->area_acc * wl_acc * abs_w);
+    // calc_time_residual!, photon_prop_cuda.jl:906-917
+    tres = (float)((double)time - T->tgeo);
 }
 
 // Write the staged hits of this wavefront to the output columns.
@@ -426,6 +495,24 @@ __global__ void clear_counters(int64_t* c) {
     if (threadIdx.x < 4) c[threadIdx.x] = 0;
 }
 
+// Hit epilogue over the hits the propagation kernel stored (count read from
+// its counter, so no host round trip): one thread per hit.
+__global__ __launch_bounds__(256) void epilogue_kernel(Epilogue ep, HitCols h, const int64_t* counters,
+                                                       const int32_t* __restrict__ target_of_module) {
+    const int64_t n = min(counters[0], h.capacity);
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int ti = target_of_module[h.module_id[i]];
+        uint8_t pmt = 0;
+        float w = 0.0f, tres = 0.0f;
+        if (ti >= 0)
+            hit_epilogue(ep, ti, make_float3(h.pos[3 * i], h.pos[3 * i + 1], h.pos[3 * i + 2]), h.wl[i],
+                         h.dist[i], h.time[i], pmt, w, tres);
+        if (h.pmt_id) h.pmt_id[i] = pmt;
+        if (h.total_weight) h.total_weight[i] = w;
+        if (h.tres) h.tres[i] = tres;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Device-function evaluation kernel (test hook)
 // ---------------------------------------------------------------------------
@@ -548,6 +635,20 @@ struct nt_ctx {
     int64_t total = 0;
 
     int64_t* d_counters = nullptr;
+    // hit epilogue
+    std::vector<nt_target> host_targets;
+    bool epi_enabled = false;
+    std::vector<float> abs_x, abs_y;
+    std::vector<double> acc_x, acc_y;
+    float abs_len_scale = 1.0f;
+    double orientation[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    float epi_source[3] = {0, 0, 0};
+    EpiTarget* d_epi_targets = nullptr;
+    int32_t* d_target_of_module = nullptr;   // module id -> target list index (65536 entries)
+    float* d_abs = nullptr;             // abs_x then abs_y
+    double* d_acc = nullptr;            // acc_x then acc_y
+    int64_t epi_targets_cap = 0, abs_cap = 0, acc_cap = 0;
+    Epilogue epi{};
     // hit columns owned by the synchronous API
     HitCols d_hits{};
     int64_t hits_cap = 0;
@@ -580,7 +681,8 @@ int32_t grow(T** p, int64_t need, int64_t* cap_items) {
 }
 
 void free_hits(HitCols& h) {
-    void* ptrs[] = {h.pos, h.dir, h.idir, h.wl, h.time, h.dist, h.module_id, h.source_index, h.photon_id};
+    void* ptrs[] = {h.pos, h.dir, h.idir, h.wl, h.time, h.dist, h.module_id, h.source_index, h.photon_id,
+                    h.pmt_id, h.total_weight, h.tres};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     h = HitCols{};
@@ -623,6 +725,89 @@ int32_t launch(nt_ctx* c, uint64_t seed, const HitCols& cols, int64_t* d_counter
     hipLaunchKernelGGL(propagate_variant(c->variant), dim3(blocks), dim3(kThreads), lds_bytes(c->spectrum.n),
                        s, kp);
     HIP_TRY(hipGetLastError());
+    if (c->epi.enabled && (cols.pmt_id || cols.total_weight || cols.tres)) {
+        const int eb = (int)std::max<int64_t>(1, std::min<int64_t>((cols.capacity + 255) / 256, 1024));
+        hipLaunchKernelGGL(epilogue_kernel, dim3(eb), dim3(256), 0, s, c->epi, cols, d_counters,
+                           c->d_target_of_module);
+        HIP_TRY(hipGetLastError());
+    }
+    return NT_OK;
+}
+
+// Per-target epilogue data (see nt_epilogue), following the precision of the
+// reference's host code: PMT axes sph_to_cart in FP32 then rotated in FP64
+// (detection.jl:60-70), opening angle asin(sqrt(area/π)/r) in FP32
+// (detection.jl:94-95), area acceptance and time residual in FP64
+// (detection.jl:139-146, photon_prop_cuda.jl:906-917).
+int32_t rebuild_epilogue(nt_ctx* c) {
+    if (!c->epi_enabled) {
+        c->epi = Epilogue{};
+        return NT_OK;
+    }
+    if (!c->has_medium || !c->has_targets) {
+        snprintf(g_err, sizeof g_err, "set the medium and the targets before the epilogue");
+        return NT_ERR_STATE;
+    }
+    const Medium& m = c->medium;
+    const float x800 = 1.0f / 800.0f, x800sq = x800 * x800;
+    const float n800 = fmaf(x800, m.a2, m.a01) + fmaf(x800sq, m.a3, x800sq * x800 * m.a4);
+    const double c_med = 0.299792458 / (double)n800;
+    const int n = (int)c->host_targets.size();
+    std::vector<EpiTarget> et(std::max(n, 1));
+    for (int i = 0; i < n; ++i) {
+        const nt_target& t = c->host_targets[i];
+        EpiTarget& e = et[i];
+        memset(&e, 0, sizeof e);
+        for (int k = 0; k < 3; ++k) e.pos[k] = (double)t.position[k];
+        e.kind = t.kind;
+        e.n_pmts = t.kind == NT_TARGET_MULTI_PMT ? std::min(t.n_pmts, NT_MAX_PMTS) : 1;
+        if (t.kind == NT_TARGET_SPHERE) {
+            const float num = (float)t.n_pmts * t.pmt_area;
+            e.area_acc = (double)num / (4.0 * M_PI * (double)(t.radius * t.radius));
+        } else {
+            e.area_acc = 1.0;
+            const float pmt_radius = sqrtf(t.pmt_area / (float)M_PI);
+            e.cos_open = cos((double)asinf(pmt_radius / t.radius));
+            for (int j = 0; j < e.n_pmts; ++j) {
+                const float th = t.pmt_coordinates[2 * j], ph = t.pmt_coordinates[2 * j + 1];
+                const float v[3] = {cosf(ph) * sinf(th), sinf(ph) * sinf(th), cosf(th)};
+                for (int r = 0; r < 3; ++r)
+                    e.axes[j][r] = (c->orientation[3 * r + 0] * v[0] + c->orientation[3 * r + 1] * v[1]) +
+                                   c->orientation[3 * r + 2] * v[2];
+            }
+        }
+        const float dx = c->epi_source[0] - t.position[0], dy = c->epi_source[1] - t.position[1],
+                    dz = c->epi_source[2] - t.position[2];
+        const float dist = sqrtf((dx * dx + dy * dy) + dz * dz);
+        e.tgeo = (double)(dist - t.radius) / c_med;
+    }
+    int32_t st = grow(&c->d_epi_targets, (int64_t)et.size(), &c->epi_targets_cap);
+    if (st) return st;
+    // Dict(module_id => target) of photon_prop_cuda.jl:852: the last listed target wins
+    std::vector<int32_t> tom(65536, -1);
+    for (int i = 0; i < n; ++i) tom[c->host_targets[i].module_id] = i;
+    if (!c->d_target_of_module) HIP_TRY(hipMalloc((void**)&c->d_target_of_module, sizeof(int32_t) * 65536));
+    HIP_TRY(hipMemcpy(c->d_target_of_module, tom.data(), sizeof(int32_t) * 65536, hipMemcpyHostToDevice));
+    st = grow(&c->d_abs, (int64_t)(2 * c->abs_x.size()), &c->abs_cap);
+    if (st) return st;
+    st = grow(&c->d_acc, (int64_t)(2 * c->acc_x.size()), &c->acc_cap);
+    if (st) return st;
+    const size_t na = c->abs_x.size(), nc = c->acc_x.size();
+    HIP_TRY(hipMemcpy(c->d_epi_targets, et.data(), sizeof(EpiTarget) * et.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_abs, c->abs_x.data(), 4 * na, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_abs + na, c->abs_y.data(), 4 * na, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_acc, c->acc_x.data(), 8 * nc, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_acc + nc, c->acc_y.data(), 8 * nc, hipMemcpyHostToDevice));
+    Epilogue& d = c->epi;
+    d.targets = c->d_epi_targets;
+    d.abs_x = c->d_abs;
+    d.abs_y = c->d_abs + na;
+    d.acc_x = c->d_acc;
+    d.acc_y = c->d_acc + nc;
+    d.n_abs = (int)na;
+    d.n_acc = (int)nc;
+    d.abs_len_scale = c->abs_len_scale;
+    d.enabled = 1;
     return NT_OK;
 }
 
@@ -685,7 +870,7 @@ void nt_destroy(nt_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     void* ptrs[] = {c->d_knots, c->d_targets, c->d_module_ids, c->d_sources, c->d_offsets, c->d_counters,
                     c->d_clusters, c->d_cluster_rmax, c->d_cluster_range, c->d_members,
-                    c->d_member_idx};
+                    c->d_member_idx, c->d_epi_targets, c->d_abs, c->d_acc, c->d_target_of_module};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     free_hits(c->d_hits);
@@ -716,6 +901,10 @@ int32_t nt_set_medium(nt_ctx* c, const nt_medium* m) {
     // a constant: evaluated once here in FP32 with the host's libm
     d.track_norm = 1.0f - expf(-2.61f * powf(2.0f, 0.39f));
     c->has_medium = true;
+    if (c->epi_enabled) {
+        HIP_TRY(hipSetDevice(c->device));
+        return rebuild_epilogue(c);
+    }
     return NT_OK;
 }
 
@@ -844,7 +1033,9 @@ int32_t nt_set_targets(nt_ctx* c, const nt_target* t, int32_t n) {
     // culling pays only when some cluster holds several targets
     c->n_clusters = (nc < n) ? nc : 0;
     c->n_targets = n;
+    c->host_targets.assign(t, t + n);
     c->has_targets = true;
+    if (c->epi_enabled) return rebuild_epilogue(c);
     return NT_OK;
 }
 
@@ -901,6 +1092,36 @@ int32_t nt_set_sources(nt_ctx* c, const nt_source* s, int32_t n) {
     return NT_OK;
 }
 
+int32_t nt_set_epilogue(nt_ctx* c, const nt_epilogue* e) {
+    if (!c || !e) return NT_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    if (!e->enabled) {
+        c->epi_enabled = false;
+        return rebuild_epilogue(c);
+    }
+    auto ascending = [](const auto* x, int n) {
+        for (int i = 1; i < n; ++i)
+            if (!(x[i] > x[i - 1])) return false;
+        return true;
+    };
+    if (e->n_abs < 2 || e->n_abs > 1024 || !e->abs_wavelength || !e->abs_length || e->n_acc < 2 ||
+        e->n_acc > 1024 || !e->acc_wavelength || !e->acc_value || !(e->abs_len_scale > 0.0f) ||
+        !ascending(e->abs_wavelength, e->n_abs) || !ascending(e->acc_wavelength, e->n_acc)) {
+        snprintf(g_err, sizeof g_err, "epilogue tables need 2..1024 ascending rows and abs_len_scale > 0");
+        return NT_ERR_ARG;
+    }
+    c->abs_x.assign(e->abs_wavelength, e->abs_wavelength + e->n_abs);
+    c->abs_y.assign(e->abs_length, e->abs_length + e->n_abs);
+    c->acc_x.assign(e->acc_wavelength, e->acc_wavelength + e->n_acc);
+    c->acc_y.assign(e->acc_value, e->acc_value + e->n_acc);
+    c->abs_len_scale = e->abs_len_scale;
+    memcpy(c->orientation, e->orientation, sizeof c->orientation);
+    memcpy(c->epi_source, e->source_position, sizeof c->epi_source);
+    c->epi_enabled = true;
+    return rebuild_epilogue(c);
+}
+
 int32_t nt_set_option(nt_ctx* c, int32_t option, int64_t value) {
     if (!c) return NT_ERR_ARG;
     std::lock_guard<std::mutex> lk(c->mu);
@@ -927,7 +1148,8 @@ int32_t nt_launch(nt_ctx* c, uint64_t seed, const nt_hits* out, int64_t* d_count
     std::lock_guard<std::mutex> lk(c->mu);
     HIP_TRY(hipSetDevice(c->device));
     HitCols h{out->position, out->direction, out->initial_direction, out->wavelength, out->time,
-              out->dist_travelled, out->module_id, out->source_index, out->photon_id, out->capacity};
+              out->dist_travelled, out->module_id, out->source_index, out->photon_id,
+              out->pmt_id, out->total_weight, out->tres, out->capacity};
     return launch(c, seed, h, d_counters, (hipStream_t)stream);
 }
 
@@ -954,6 +1176,9 @@ int32_t nt_propagate(nt_ctx* c, uint64_t seed, const nt_hits* out, int64_t* n_hi
         HIP_TRY(hipMalloc((void**)&h.module_id, 2 * n));
         HIP_TRY(hipMalloc((void**)&h.source_index, 4 * n));
         HIP_TRY(hipMalloc((void**)&h.photon_id, 8 * n));
+        HIP_TRY(hipMalloc((void**)&h.pmt_id, n));
+        HIP_TRY(hipMalloc((void**)&h.total_weight, 4 * n));
+        HIP_TRY(hipMalloc((void**)&h.tres, 4 * n));
         c->hits_cap = n;
     }
     HitCols h = c->d_hits;
@@ -976,6 +1201,12 @@ int32_t nt_propagate(nt_ctx* c, uint64_t seed, const nt_hits* out, int64_t* n_hi
             HIP_TRY(hipMemcpyAsync(out->source_index, h.source_index, 4 * stored, hipMemcpyDeviceToHost, c->stream));
         if (out->photon_id)
             HIP_TRY(hipMemcpyAsync(out->photon_id, h.photon_id, 8 * stored, hipMemcpyDeviceToHost, c->stream));
+        if (out->pmt_id && c->epi.enabled)
+            HIP_TRY(hipMemcpyAsync(out->pmt_id, h.pmt_id, stored, hipMemcpyDeviceToHost, c->stream));
+        if (out->total_weight && c->epi.enabled)
+            HIP_TRY(hipMemcpyAsync(out->total_weight, h.total_weight, 4 * stored, hipMemcpyDeviceToHost, c->stream));
+        if (out->tres && c->epi.enabled)
+            HIP_TRY(hipMemcpyAsync(out->tres, h.tres, 4 * stored, hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
     }
     *n_hits = stored;

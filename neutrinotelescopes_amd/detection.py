@@ -123,9 +123,25 @@ def make_pom_pmt_coordinates(dtype=np.float64) -> np.ndarray:
 
 
 def get_pmt_positions(target: MultiPMTDetector, orientation: np.ndarray) -> np.ndarray:
-    """PMT axes (N×3) after `orientation`, detection.jl:60-70."""
-    pc = np.asarray(target.pmt_coordinates, dtype=np.float64)
-    return (np.asarray(orientation, dtype=np.float64) @ sph_to_cart(pc[0], pc[1]).T).T
+    """PMT axes (N×3) after `orientation`, detection.jl:60-70: sph_to_cart in
+    the coordinates' precision, then the FP64 rotation."""
+    pc = np.asarray(target.pmt_coordinates)
+    if pc.dtype == np.float32:
+        th, ph = pc[0].astype(np.float64), pc[1].astype(np.float64)
+        s = lambda x: x.astype(np.float32)            # correctly rounded float32 sin/cos
+        v = np.stack([s(np.cos(ph)) * s(np.sin(th)), s(np.sin(ph)) * s(np.sin(th)), s(np.cos(th))], -1)
+        v = v.astype(np.float64)
+    else:
+        v = sph_to_cart(pc[0], pc[1])
+    return (np.asarray(orientation, dtype=np.float64) @ v.T).T
+
+
+def opening_angle(target: MultiPMTDetector):
+    """asin(sqrt(pmt_area/π)/radius), detection.jl:94-95, in the target's precision."""
+    if np.asarray(target.pmt_coordinates).dtype == np.float32:
+        r = np.sqrt(np.float32(target.pmt_area) / np.float32(np.pi))
+        return float(np.float32(np.arcsin(np.float64(r / np.float32(target.radius)))))
+    return float(np.arcsin(np.sqrt(float(target.pmt_area) / np.pi) / float(target.radius)))
 
 
 def check_pmt_hit(hit_positions, target, orientation=None) -> np.ndarray:
@@ -134,8 +150,7 @@ def check_pmt_hit(hit_positions, target, orientation=None) -> np.ndarray:
     if isinstance(target, DetectionSphere):
         return np.ones(len(hit_positions), dtype=np.int64)
     pmt_pos = get_pmt_positions(target, np.eye(3) if orientation is None else orientation)
-    pmt_radius = np.sqrt(float(target.pmt_area) / np.pi)
-    opening = np.arcsin(pmt_radius / float(target.radius))
+    opening = opening_angle(target)
     rel = hit_positions - np.asarray(target.position, dtype=np.float64)
     rel = rel / np.linalg.norm(rel, axis=1, keepdims=True)
     ang = np.arccos(np.clip(rel @ pmt_pos.T, -1.0, 1.0))    # hits × PMTs
@@ -147,20 +162,27 @@ def check_pmt_hit(hit_positions, target, orientation=None) -> np.ndarray:
 def area_acceptance(position, target) -> float:
     """detection.jl:139-146."""
     if isinstance(target, DetectionSphere):
-        return target.n_pmts * np.float32(target.pmt_area) / (4 * np.pi * np.float32(target.radius) ** 2)
+        num = np.float32(np.float32(target.n_pmts) * np.float32(target.pmt_area))   # Int·Float32
+        r = np.float32(target.radius)
+        return float(num) / (4 * np.pi * float(r * r))                              # Float64
     return 1.0
 
 
 _PMT_ACC = None
 
 
-def p_one_pmt_acc(wavelength):
-    """Single-PMT wavelength acceptance, detection.jl:148-163 (linear interpolation
-    of assets/PMTAcc.csv)."""
+def pmt_acceptance_table():
+    """assets/PMTAcc.csv (wavelength nm, single-PMT acceptance), detection.jl:161-163."""
     global _PMT_ACC
     if _PMT_ACC is None:
         _PMT_ACC = _read_table("pmt_acceptance.csv")
-    xs, ys = _PMT_ACC
+    return _PMT_ACC
+
+
+def p_one_pmt_acc(wavelength):
+    """Single-PMT wavelength acceptance, detection.jl:148-163 (linear interpolation
+    of assets/PMTAcc.csv)."""
+    xs, ys = pmt_acceptance_table()
     wl = np.asarray(wavelength, dtype=np.float64)
     if np.any((wl < xs[0]) | (wl > xs[-1])):
         raise ValueError("wavelength outside the PMT acceptance table")

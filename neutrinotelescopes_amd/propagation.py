@@ -42,7 +42,8 @@ class PhotonPropSetup:
 
 
 HIT_COLUMNS = ("position", "direction", "initial_direction", "wavelength", "time",
-               "dist_travelled", "module_id", "source_index", "photon_id")
+               "dist_travelled", "module_id", "source_index", "photon_id", "pmt_id", "total_weight",
+               "tres")
 
 
 @dataclass
@@ -58,6 +59,9 @@ class PhotonHits:
     module_id: np.ndarray
     source_index: np.ndarray
     photon_id: np.ndarray
+    pmt_id: np.ndarray = None          # filled by the device epilogue
+    total_weight: np.ndarray = None
+    tres: np.ndarray = None
     n_simulated: int = 0
     n_dropped: int = 0
     extra: dict = field(default_factory=dict)
@@ -68,18 +72,20 @@ class PhotonHits:
         return cls(np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32),
                    np.zeros((n, 3), np.float32), np.zeros(n, np.float32), np.zeros(n, np.float32),
                    np.zeros(n, np.float32), np.zeros(n, np.uint16), np.zeros(n, np.uint32),
-                   np.zeros(n, np.uint64))
+                   np.zeros(n, np.uint64), np.zeros(n, np.uint8), np.zeros(n, np.float32),
+                   np.zeros(n, np.float32))
 
     def __len__(self):
         return len(self.wavelength)
 
+    def _cols(self, idx):
+        return (None if getattr(self, c) is None else getattr(self, c)[idx] for c in HIT_COLUMNS)
+
     def truncated(self, n: int) -> "PhotonHits":
-        return PhotonHits(*(getattr(self, c)[:n] for c in HIT_COLUMNS), n_simulated=self.n_simulated,
-                          n_dropped=self.n_dropped)
+        return PhotonHits(*self._cols(slice(0, n)), n_simulated=self.n_simulated, n_dropped=self.n_dropped)
 
     def select(self, mask) -> "PhotonHits":
-        out = PhotonHits(*(getattr(self, c)[mask] for c in HIT_COLUMNS),
-                         n_simulated=self.n_simulated, n_dropped=self.n_dropped)
+        out = PhotonHits(*self._cols(mask), n_simulated=self.n_simulated, n_dropped=self.n_dropped)
         out.extra = {k: v[mask] for k, v in self.extra.items()}
         return out
 
@@ -91,7 +97,7 @@ class PhotonHits:
     def to_abi(self) -> _abi.nt_hits:
         h = _abi.nt_hits()
         for c in HIT_COLUMNS:
-            setattr(h, c, getattr(self, c).ctypes.data)
+            setattr(h, c, None if getattr(self, c) is None else getattr(self, c).ctypes.data)
         h.capacity = len(self)
         return h
 
@@ -102,8 +108,10 @@ class PhotonHits:
             a = getattr(self, c)
             for i, ax in enumerate("xyz"):
                 d[f"{c}_{ax}"] = a[:, i]
-        for c in ("wavelength", "time", "dist_travelled", "module_id", "source_index", "photon_id"):
-            d[c] = getattr(self, c)
+        for c in ("wavelength", "time", "dist_travelled", "module_id", "source_index", "photon_id",
+                  "pmt_id", "total_weight", "tres"):
+            if getattr(self, c) is not None:
+                d[c] = getattr(self, c)
         d.update(self.extra)
         return pd.DataFrame(d)
 
@@ -151,6 +159,33 @@ class PropagationEngine:
         self.set_targets(setup.targets)
         self.set_sources(setup.sources)
 
+    def set_epilogue(self, medium: WaterProperties | None = None, orientation=None,
+                     source_position=(0.0, 0.0, 0.0), enabled: bool = True):
+        """Enable the device hit epilogue (PMT id, total weight, time residual)
+        with the absorption and PMT-acceptance tables of this package."""
+        e = _abi.nt_epilogue()
+        e.enabled = int(enabled)
+        if enabled:
+            from .detection import pmt_acceptance_table
+            from .medium import absorption_table
+            ax, ay = absorption_table()
+            self._abs = (np.ascontiguousarray(ax, np.float32), np.ascontiguousarray(ay, np.float32))
+            cx, cy = pmt_acceptance_table()
+            self._acc = (np.ascontiguousarray(cx, np.float64), np.ascontiguousarray(cy, np.float64))
+            e.n_abs = len(self._abs[0])
+            e.abs_wavelength = self._abs[0].ctypes.data_as(C.POINTER(C.c_float))
+            e.abs_length = self._abs[1].ctypes.data_as(C.POINTER(C.c_float))
+            e.abs_len_scale = float(medium.abs_len_scale) if medium is not None else 1.0
+            e.n_acc = len(self._acc[0])
+            e.acc_wavelength = self._acc[0].ctypes.data_as(C.POINTER(C.c_double))
+            e.acc_value = self._acc[1].ctypes.data_as(C.POINTER(C.c_double))
+            o = np.eye(3) if orientation is None else np.asarray(orientation, np.float64)
+            for i, v in enumerate(o.reshape(-1)):
+                e.orientation[i] = float(v)
+            for i, v in enumerate(np.asarray(source_position, np.float32).reshape(3)):
+                e.source_position[i] = float(v)
+        _abi.check(self.lib.nt_set_epilogue(self.handle, C.byref(e)), "nt_set_epilogue")
+
     def launch_shape(self):
         b, t = C.c_int32(), C.c_int32()
         _abi.check(self.lib.nt_launch_shape(self.handle, C.byref(b), C.byref(t)), "nt_launch_shape")
@@ -191,20 +226,27 @@ def get_engine(device: int = 0) -> PropagationEngine:
 
 
 def run_photon_prop_no_local_cache(sources, targets, medium, spectrum, *, seed: int = 0,
-                                   capacity: int = 1 << 22, device: int = 0):
+                                   capacity: int = 1 << 22, device: int = 0, epilogue: bool = False,
+                                   orientation=None):
     """photon_prop_cuda.jl:726-800: (hits, n_photons_simulated).  One launch
     covers all sources; the hit capacity is `capacity` (the reference reserves
     0.7 of device memory on every call, photon_prop_cuda.jl:732-736)."""
     eng = get_engine(device)
-    eng.setup(PhotonPropSetup(list(sources) if isinstance(sources, (list, tuple)) else [sources],
-                              list(targets) if isinstance(targets, (list, tuple)) else [targets],
-                              medium, spectrum))
+    setup = PhotonPropSetup(list(sources) if isinstance(sources, (list, tuple)) else [sources],
+                            list(targets) if isinstance(targets, (list, tuple)) else [targets],
+                            medium, spectrum)
+    eng.setup(setup)
+    eng.set_epilogue(medium, orientation, setup.sources[0].position if setup.sources else (0, 0, 0),
+                     enabled=epilogue)
     hits = eng.propagate(seed=seed, capacity=capacity)
+    if not epilogue:
+        hits.pmt_id = hits.total_weight = hits.tres = None
     return hits, hits.n_simulated
 
 
 def propagate_photons(setup: PhotonPropSetup, *, capacity: int = 1 << 22, device: int = 0,
-                      single_target_swap: bool = True) -> PhotonHits:
+                      single_target_swap: bool = True, epilogue: bool = False,
+                      orientation=None) -> PhotonHits:
     """photon_prop_cuda.jl:804-817.
 
     With one target the reference takes its single-target kernel, which stores
@@ -213,7 +255,8 @@ def propagate_photons(setup: PhotonPropSetup, *, capacity: int = 1 << 22, device
     (default True) reproduces that column order."""
     hits, _ = run_photon_prop_no_local_cache(setup.sources, setup.targets, setup.medium,
                                              setup.spectrum, seed=setup.seed, capacity=capacity,
-                                             device=device)
+                                             device=device, epilogue=epilogue,
+                                             orientation=orientation)
     if single_target_swap and len(setup.targets) == 1:
         hits.direction, hits.initial_direction = hits.initial_direction, hits.direction
     return hits
@@ -245,13 +288,14 @@ def calc_time_residual(hits: PhotonHits, setup: PhotonPropSetup) -> PhotonHits:
     tmap = _target_map(setup)
     src = np.asarray(setup.sources[0].position, dtype=np.float32)
     n800 = float(refractive_index(np.float32(800.0), setup.medium))
-    tres = np.empty(len(hits), dtype=np.float32)
+    tres = np.empty(len(hits), dtype=np.float64)
     for mid in np.unique(hits.module_id):
         t = tmap[int(mid)]
         sel = hits.module_id == mid
-        dist = np.float32(np.linalg.norm(src - np.asarray(t.position, dtype=np.float32)))
-        tgeo = (dist - np.float32(t.radius)) / np.float32(C_VAC_M_NS / n800)
-        tres[sel] = hits.time[sel] - tgeo
+        d = src - np.asarray(t.position, dtype=np.float32)
+        dist = np.sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2])          # float32 norm
+        tgeo = float(dist - np.float32(t.radius)) / (C_VAC_M_NS / n800)    # float64
+        tres[sel] = hits.time[sel].astype(np.float64) - tgeo
     hits.extra["tres"] = tres
     return hits
 

@@ -24,8 +24,9 @@ def load():
     global _lib
     if _lib is not None:
         return _lib
-    if not ORACLE_SO.exists():
-        subprocess.run(["make", "-C", str(ROOT / "oracle")], check=True, capture_output=True)
+    if not ORACLE_SO.exists() or os.access(ROOT / "oracle", os.W_OK):
+        # rebuild when stale (header or source newer); a no-op otherwise
+        subprocess.run(["make", "-s", "-C", str(ROOT / "oracle")], check=True, capture_output=True)
     lib = C.CDLL(str(ORACLE_SO))
     f, d, i, u32, u64, p = C.c_float, C.c_double, C.c_int, C.c_uint32, C.c_uint64, C.c_void_p
     fp = C.POINTER(C.c_float)

@@ -43,7 +43,7 @@ def test_struct_layouts():
     assert C.sizeof(_abi.nt_source) == 64
     assert C.sizeof(_abi.nt_medium) == 48
     assert C.sizeof(_abi.nt_target) == 160
-    assert _abi.nt_hits.capacity.offset == 9 * 8
+    assert _abi.nt_hits.capacity.offset == 12 * 8
 
 
 def test_library_missing_fails_loudly(tmp_path):

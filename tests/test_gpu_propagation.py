@@ -310,3 +310,39 @@ def test_culling_identical_to_brute_force(engine, layout):
     for col in ("position", "direction", "initial_direction", "wavelength", "time", "dist_travelled",
                 "module_id", "photon_id"):
         assert np.array_equal(getattr(a, col), getattr(b, col)), col
+
+
+def test_epilogue_matches_host_postprocessing(engine):
+    """Device PMT assignment, weights and time residuals against the host
+    mirror of make_hits_from_photons / calc_total_weight! / calc_time_residual!
+    evaluated on the same hits."""
+    from scipy.spatial.transform import Rotation
+    med = nt.make_cascadia_medium_properties(0.95)
+    spec = nt.CherenkovSpectrum((300.0, 800.0), 30, med)
+    lp = nt.LongitudinalParameterisation.from_energy(1e4, med, nt.PEMinus)
+    src = nt.ExtendedCherenkovEmitter(np.array([1.0, 2.0, 3.0], np.float32), np.array([1, 0, 0], np.float32),
+                                      5.0, 1_000_000, lp)
+    tg = string_targets_r(1.0) + [nt.DetectionSphere(np.array([-15.0, 0.0, 3.0], np.float32), 1.0, 16,
+                                                     PMT_AREA, 99)]
+    s = nt.PhotonPropSetup([src], tg, med, spec)
+    orient = Rotation.random(random_state=3).as_matrix()
+    engine.setup(s)
+    engine.set_epilogue(med, orient, src.position)
+    h = engine.propagate(seed=31)
+    engine.set_epilogue(enabled=False)
+    assert len(h) > 200 and set(np.unique(h.module_id)) >= {99}
+    ref = h.select(np.arange(len(h)))
+    nt.calc_total_weight(ref, s)
+    nt.calc_time_residual(ref, s)
+    tmap = {t.module_id: t for t in tg}
+    pmt = np.zeros(len(h), np.int64)
+    for mid in np.unique(h.module_id):
+        sel = h.module_id == mid
+        pmt[sel] = nt.check_pmt_hit(h.position[sel], tmap[int(mid)], orient)
+    assert np.sum(h.pmt_id != pmt) <= 1                  # FP64 boundary ties only
+    assert 0 < np.mean(pmt[h.module_id != 99] > 0) < 0.5
+    assert np.allclose(h.total_weight, ref.extra["total_weight"], rtol=2e-6, atol=1e-12)
+    assert np.allclose(h.tres, ref.extra["tres"], rtol=0, atol=2e-5 * np.maximum(1, np.abs(ref.extra["tres"])).max())
+    # the reference's PMT filter keeps only hits that land on a PMT
+    kept = nt.make_hits_from_photons(ref, s, orient)
+    assert len(kept) == int(np.sum(pmt > 0))
