@@ -240,6 +240,50 @@ typedef struct nt_epilogue {
 } nt_epilogue;
 int32_t nt_set_epilogue(nt_ctx* ctx, const nt_epilogue* epilogue);
 
+/* ---- serving: continuous batching of propagation requests --------------
+   No reference counterpart (the reference propagates one setup per call).
+   A server runs a scheduler thread over the context's medium, spectrum and
+   targets (set them before nt_server_create and leave them unchanged while
+   it runs).  Requests (sources + seed + hit capacity) are admitted in
+   submission order as soon as their hit buffers fit the device and pinned
+   host budgets; every round packs photon chunks of all admitted requests
+   into one launch of the propagation kernel (at most round_photons photons,
+   shared between requests), so requests that arrive while others run join
+   the next round.  A request's photons use Philox key = its seed, counter
+   source word = its source index and photon index = index inside the
+   source, exactly as nt_propagate with that seed: a result is the same
+   alone and in any mix.  Results are copied to pinned memory on a second
+   stream; latency is measured from submission to the end of that copy. */
+typedef struct nt_server nt_server;
+
+typedef struct nt_server_config {
+    uint64_t device_budget_bytes;    /* HBM for request hit buffers */
+    uint64_t host_budget_bytes;      /* pinned host memory for results */
+    int64_t round_photons;           /* photons per batched launch */
+    int32_t max_inflight_rounds;     /* launches queued ahead of the device (>= 1) */
+    int32_t reserved;
+} nt_server_config;
+
+typedef struct nt_server_stats {
+    int64_t submitted, completed, rounds, photons;
+    int64_t queued, active;          /* requests waiting for memory / admitted */
+    double busy_ms;                  /* sum of round durations (device time) */
+} nt_server_stats;
+
+int32_t nt_server_create(nt_ctx* ctx, const nt_server_config* cfg, nt_server** out);
+/* Thread-safe.  NT_ERR_BUDGET if the request can never fit the budgets. */
+int32_t nt_server_submit(nt_server* srv, const nt_source* sources, int32_t n_sources, uint64_t seed,
+                         int64_t hit_capacity, uint64_t* request_id);
+/* NT_OK when the request is complete, NT_ERR_NOT_READY otherwise. */
+int32_t nt_server_poll(nt_server* srv, uint64_t request_id);
+/* Blocks until the request is complete, copies its hits to host columns
+   (capacity >= the request's hit capacity), releases it.  Returns
+   NT_ERR_HIT_OVERFLOW when hits were lost (the first `hit_capacity` kept). */
+int32_t nt_server_wait(nt_server* srv, uint64_t request_id, const nt_hits* out_host, int64_t* n_hits,
+                       int64_t* n_simulated, int64_t* n_dropped, double* latency_ms);
+int32_t nt_server_stats_get(nt_server* srv, nt_server_stats* out);
+void nt_server_destroy(nt_server* srv);
+
 /* Grid used by nt_launch (blocks, threads per block). */
 int32_t nt_launch_shape(nt_ctx* ctx, int32_t* blocks, int32_t* threads);
 

@@ -145,6 +145,24 @@ assert C.sizeof(nt_source) == 64
 assert C.sizeof(nt_medium) == 48
 assert C.sizeof(nt_target) == 160
 
+class nt_server_config(C.Structure):
+    _fields_ = [
+        ("device_budget_bytes", C.c_uint64),
+        ("host_budget_bytes", C.c_uint64),
+        ("round_photons", C.c_int64),
+        ("max_inflight_rounds", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class nt_server_stats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_int64), ("completed", C.c_int64), ("rounds", C.c_int64),
+        ("photons", C.c_int64), ("queued", C.c_int64), ("active", C.c_int64),
+        ("busy_ms", C.c_double),
+    ]
+
+
 # exported symbol -> (restype, argtypes)
 SIGNATURES = {
     "nt_abi_version": (C.c_int32, []),
@@ -161,6 +179,14 @@ SIGNATURES = {
     "nt_set_epilogue": (C.c_int32, [C.c_void_p, C.POINTER(nt_epilogue)]),
     "nt_set_option": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int64]),
     "nt_launch_shape": (C.c_int32, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "nt_server_create": (C.c_int32, [C.c_void_p, C.POINTER(nt_server_config), C.POINTER(C.c_void_p)]),
+    "nt_server_submit": (C.c_int32, [C.c_void_p, C.POINTER(nt_source), C.c_int32, C.c_uint64, C.c_int64,
+                                     C.POINTER(C.c_uint64)]),
+    "nt_server_poll": (C.c_int32, [C.c_void_p, C.c_uint64]),
+    "nt_server_wait": (C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(nt_hits), C.POINTER(C.c_int64),
+                                   C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
+    "nt_server_stats_get": (C.c_int32, [C.c_void_p, C.POINTER(nt_server_stats)]),
+    "nt_server_destroy": (None, [C.c_void_p]),
     "nt_eval": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
                             C.c_int64]),
 }

@@ -35,17 +35,24 @@ constexpr int kStageSlots = 64;        // staged hits per wavefront
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef const __attribute__((address_space(4))) f4* const_f4_ptr;
 
-struct DevSource {                     // 80 bytes, one per source
+// One source of a launch.  Its photons occupy global indices
+// [offset, offset + photons); offsets are multiples of 64, so a wavefront's
+// 64 consecutive indices always belong to one source, which makes the
+// source, its Philox key and its output slot wave-uniform (scalar registers).
+struct DevSource {                     // 96 bytes
     int32_t kind;
-    uint32_t index;                    // Philox counter word 3
-    int64_t offset;                    // first global photon index
+    uint32_t rng_index;                // Philox counter word 3
+    int64_t offset;                    // first global index (multiple of 64)
     int64_t photons;
+    uint64_t photon_base;              // RNG photon index of the first photon
+    uint32_t key_lo, key_hi;           // XORed into the launch seed
+    int32_t slot;                      // output slot
     float pos[3];
     float dir[3];
     float t0, t1, angle;
     float gamma_d, gamma_c, gamma_scale, lrad;
-    float pad;
 };
+static_assert(sizeof(DevSource) == 96, "DevSource layout");
 
 struct HitCols {
     float* pos;
@@ -61,6 +68,13 @@ struct HitCols {
     float* total_weight;
     float* tres;
     int64_t capacity;
+};
+
+// Output of one request: hit columns and counters
+// ([0] hits found, [1] photons simulated, [2] sphere tests, [3] string-bound tests).
+struct SlotDesc {
+    HitCols cols;
+    int64_t* counters;
 };
 
 // Hit epilogue data of one target (list order), prepared on the host.
@@ -94,7 +108,8 @@ struct Epilogue {
 // list index, so the first listed target still wins.
 struct KParams {
     const DevSource* __restrict__ sources;
-    const int64_t* __restrict__ offsets;     // n_sources + 1
+    const int64_t* __restrict__ offsets;     // n_sources + 1 (multiples of 64)
+    const SlotDesc* __restrict__ slots;      // null: single output (hits, counters)
     const float4* __restrict__ targets;      // x, y, z, r^2 in list order
     const uint32_t* __restrict__ module_ids; // per list index
     const float4* __restrict__ clusters;     // geometry per cluster
@@ -136,7 +151,7 @@ __device__ __forceinline__ bool sphere_hit(float3 pos, float3 dir, float step, f
 // and |p − c| ≤ step + r for any accepted hit, so a culled target can never be
 // one the exact test would accept.
 __device__ __forceinline__ int first_hit(const KParams& p, float3 pos, float3 dir, float step, float& dt,
-                                         int64_t& tests, int64_t& ctests) {
+                                         uint32_t& tests, uint32_t& ctests) {
     int best = 0x7fffffff;
     if (p.n_clusters == 0) {
         const const_f4_ptr tgt = (const_f4_ptr)(const void*)p.targets;
@@ -228,8 +243,12 @@ __device__ __forceinline__ Spectrum load_spectrum(const_kparams_ptr k) {
     return s;
 }
 
-__device__ __forceinline__ int find_source(const int64_t* __restrict__ offs, int n, int64_t g) {
-    int lo = 0, hi = n - 1;            // largest s with offs[s] <= g
+typedef const __attribute__((address_space(4))) int64_t* const_i64_ptr;
+typedef const __attribute__((address_space(4))) DevSource* const_src_ptr;
+
+// Largest s with offs[s] <= g, for a wave-uniform g (scalar loads).
+__device__ __forceinline__ int find_source(const_i64_ptr offs, int n, int64_t g) {
+    int lo = 0, hi = n - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
         if (offs[mid] <= g) lo = mid; else hi = mid - 1;
@@ -281,19 +300,52 @@ __device__ __forceinline__ void hit_epilogue(const Epilogue& ep, int ti, float3 
     tres = (float)((double)time - T->tgeo);
 }
 
-// Write the staged hits of this wavefront to the output columns.
-__device__ __forceinline__ void flush_stage(const uint32_t* __restrict__ st, int n, int lane) {
+// Output columns and counters of a slot.
+__device__ __forceinline__ HitCols slot_cols(const_kparams_ptr kp, int slot, int64_t*& counters) {
+    HitCols h;
+    if (kp->slots == nullptr) {
+        h.pos = kp->hits.pos; h.dir = kp->hits.dir; h.idir = kp->hits.idir; h.wl = kp->hits.wl;
+        h.time = kp->hits.time; h.dist = kp->hits.dist; h.module_id = kp->hits.module_id;
+        h.source_index = kp->hits.source_index; h.photon_id = kp->hits.photon_id;
+        h.capacity = kp->hits.capacity;
+        counters = kp->counters;
+    } else {
+        const SlotDesc* sd = kp->slots + slot;
+        h.pos = sd->cols.pos; h.dir = sd->cols.dir; h.idir = sd->cols.idir; h.wl = sd->cols.wl;
+        h.time = sd->cols.time; h.dist = sd->cols.dist; h.module_id = sd->cols.module_id;
+        h.source_index = sd->cols.source_index; h.photon_id = sd->cols.photon_id;
+        h.capacity = sd->cols.capacity;
+        counters = sd->counters;
+    }
+    return h;
+}
+
+// Add a wavefront's photon and test counts to a slot's counters.
+__device__ __forceinline__ void commit_counts(int slot, int64_t simulated, uint32_t tests32, uint32_t ctests32,
+                                              int lane) {
+    uint64_t tests = tests32, ctests = ctests32;      // per-lane counts since the last commit
+    for (int off = 32; off > 0; off >>= 1) {
+        tests += __shfl_down(tests, off);
+        ctests += __shfl_down(ctests, off);
+    }
+    if (lane == 0) {
+        int64_t* counters;
+        (void)slot_cols(kargs(), slot, counters);
+        if (simulated) atomicAdd((unsigned long long*)&counters[1], (unsigned long long)simulated);
+        if (tests) atomicAdd((unsigned long long*)&counters[2], (unsigned long long)tests);
+        if (ctests) atomicAdd((unsigned long long*)&counters[3], (unsigned long long)ctests);
+    }
+}
+
+// Write the staged hits of this wavefront to the output columns of `slot`.
+__device__ __forceinline__ void flush_stage(const uint32_t* __restrict__ st, int n, int lane, int slot) {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    const const_kparams_ptr kp = kargs();
+    int64_t* counters;
+    const HitCols h = slot_cols(kargs(), slot, counters);
     unsigned long long base = 0;
-    if (lane == 0) base = atomicAdd((unsigned long long*)&kp->counters[0], (unsigned long long)n);
+    if (lane == 0) base = atomicAdd((unsigned long long*)&counters[0], (unsigned long long)n);
     base = __shfl(base, 0);
-    HitCols h;
-    h.pos = kp->hits.pos; h.dir = kp->hits.dir; h.idir = kp->hits.idir; h.wl = kp->hits.wl;
-    h.time = kp->hits.time; h.dist = kp->hits.dist; h.module_id = kp->hits.module_id;
-    h.source_index = kp->hits.source_index; h.photon_id = kp->hits.photon_id;
-    h.capacity = kp->hits.capacity;
     for (int i = lane; i < n; i += 64) {
         const int64_t idx = (int64_t)base + i;
         if (idx >= h.capacity) continue;
@@ -338,61 +390,70 @@ __global__ __launch_bounds__(kThreads, NTP_MIN_WAVES_PER_SIMD) void propagate_ke
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     uint32_t* st = s_stage + wave * (kStageFields * kStageSlots);
-    int staged = 0;                                                // wave-uniform
-    int64_t simulated = 0;
-    int64_t tests = 0;
-    int64_t ctests = 0;
+    int staged = 0;                     // hits in the stage (wave-uniform)
+    int stage_slot = 0;                 // their output slot
+    int count_slot = -1;                // slot the counts below belong to
+    int64_t simulated = 0;              // wave-uniform
+    uint32_t tests = 0, ctests = 0;     // per lane, since the last commit
 
     const Medium& m = p.medium;
     const int64_t stride = (int64_t)gridDim.x * kThreads;
-    int src = -1;
-    int64_t src_lo = 0, src_hi = 0;
+    int src = 0;
+    int64_t src_lo = 0, src_hi = 0;     // global index range of `src`
 
     for (int64_t base = (int64_t)blockIdx.x * kThreads + wave * 64; base < p.total; base += stride) {
-        const int64_t g = base + lane;
+        // the wavefront's source (offsets are multiples of 64)
+        const const_kparams_ptr kp = kargs();
+        if (base < src_lo || base >= src_hi) {
+            const const_i64_ptr offs = (const_i64_ptr)(const void*)kp->offsets;
+            src = find_source(offs, kp->n_sources, base);
+            src_lo = offs[src];
+            src_hi = offs[src + 1];
+        }
+        const const_src_ptr S = (const_src_ptr)(const void*)kp->sources + src;
+        const int slot = S->slot;
+        if (slot != count_slot) {
+            if (count_slot >= 0) commit_counts(count_slot, simulated, tests, ctests, lane);
+            simulated = tests = ctests = 0;
+            count_slot = slot;
+        }
+        const int64_t pid_local = base - src_lo + lane;
+        const bool active = pid_local < S->photons;
         bool hit = false;
         float3 pos, dir, idir;
         float wl = 0.f, time = 0.f, dist = 0.f;
-        uint32_t mid = 0, sidx = 0;
-        uint64_t pid = 0;
-        if (g < p.total) {
-            const const_kparams_ptr kp = kargs();
+        uint32_t mid = 0;
+        const uint32_t sidx = S->rng_index;
+        const uint64_t pid = S->photon_base + (uint64_t)pid_local;
+        if (active) {
             const Medium mi = load_medium(kp);      // initialisation-only fields
-            const int64_t* offsets = kp->offsets;
-            if (g < src_lo || g >= src_hi) {
-                src = find_source(offsets, kp->n_sources, g);
-                src_lo = offsets[src];
-                src_hi = offsets[src + 1];
-            }
-            const DevSource& s = kp->sources[src];
-            pid = (uint64_t)(g - src_lo);
-            sidx = s.index;
-            const Rng rng{p.seed_lo, p.seed_hi, (uint32_t)pid, (uint32_t)(pid >> 32), sidx};
+            const Rng rng{p.seed_lo ^ S->key_lo, p.seed_hi ^ S->key_hi, (uint32_t)pid, (uint32_t)(pid >> 32),
+                          sidx};
 
             // initialize_photon_state, photon_prop_cuda.jl:133-188
             const uint4 r0 = rng.block(0u);
             wl = sample_spectrum(load_spectrum(kp), s_knots, u01(r0.x));
-            pos = make_float3(s.pos[0], s.pos[1], s.pos[2]);
-            const float3 sdir = make_float3(s.dir[0], s.dir[1], s.dir[2]);
-            time = s.t0;
-            switch (KIND < 0 ? s.kind : KIND) {
+            pos = make_float3(S->pos[0], S->pos[1], S->pos[2]);
+            const float3 sdir = make_float3(S->dir[0], S->dir[1], S->dir[2]);
+            time = S->t0;
+            switch (KIND < 0 ? S->kind : KIND) {
                 case NT_SRC_POINTLIKE_ISOTROPIC:
                     dir = iso_dir(u01(r0.y), u01(r0.z));
                     break;
                 case NT_SRC_POINTLIKE_TIME_RANGE:
                     dir = iso_dir(u01(r0.y), u01(r0.z));
-                    time = s.t0 + u01(r0.w) * (s.t1 - s.t0);
+                    time = S->t0 + u01(r0.w) * (S->t1 - S->t0);
                     break;
                 case NT_SRC_AXICONE:
-                    dir = rot_from_ez(sdir, sph_to_cart(s.angle, 0.0f + u01(r0.z) * kTwoPi));
+                    dir = rot_from_ez(sdir, sph_to_cart(S->angle, 0.0f + u01(r0.z) * kTwoPi));
                     break;
                 case NT_SRC_PENCIL:
                     dir = sdir;
                     break;
                 case NT_SRC_EXTENDED_CHERENKOV: {
-                    const float lp = rand_gamma(s.gamma_d, s.gamma_c, s.gamma_scale, rng, nullptr) * s.lrad;
+                    const float lp = rand_gamma(S->gamma_d, S->gamma_c, S->gamma_scale, rng, nullptr) * S->lrad;
                     pos = make_float3(pos.x + lp * sdir.x, pos.y + lp * sdir.y, pos.z + lp * sdir.z);
-                    time = s.t0 + lp / kCVacMNs;
+                    time = S->t0 + lp / kCVacMNs;
                     dir = cherenkov_dir(wl, u01(r0.y), u01(r0.z), u01(r0.w), sdir, mi);
                     break;
                 }
@@ -425,55 +486,45 @@ __global__ __launch_bounds__(kThreads, NTP_MIN_WAVES_PER_SIMD) void propagate_ke
                 hit = true;
                 break;
             }
-            ++simulated;
         }
+        simulated += __popcll(__ballot(active));
 
         // wavefront compaction of this round's hits into the LDS stage
         const uint64_t mask = __ballot(hit);
         if (mask) {
             const int cnt = __popcll(mask);
-            if (staged + cnt > kStageSlots) {
-                flush_stage(st, staged, lane);
+            if (staged > 0 && (staged + cnt > kStageSlots || slot != stage_slot)) {
+                flush_stage(st, staged, lane, stage_slot);
                 staged = 0;
             }
+            stage_slot = slot;
             if (hit) {
-                const int slot = staged + (int)__builtin_amdgcn_mbcnt_hi(
-                                              (uint32_t)(mask >> 32),
-                                              __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+                const int at = staged + (int)__builtin_amdgcn_mbcnt_hi(
+                                            (uint32_t)(mask >> 32),
+                                            __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
                 float* f = reinterpret_cast<float*>(st);
-                f[0 * kStageSlots + slot] = pos.x;
-                f[1 * kStageSlots + slot] = pos.y;
-                f[2 * kStageSlots + slot] = pos.z;
-                f[3 * kStageSlots + slot] = dir.x;
-                f[4 * kStageSlots + slot] = dir.y;
-                f[5 * kStageSlots + slot] = dir.z;
-                f[6 * kStageSlots + slot] = idir.x;
-                f[7 * kStageSlots + slot] = idir.y;
-                f[8 * kStageSlots + slot] = idir.z;
-                f[9 * kStageSlots + slot] = wl;
-                f[10 * kStageSlots + slot] = time;
-                f[11 * kStageSlots + slot] = dist;
-                st[12 * kStageSlots + slot] = mid;
-                st[13 * kStageSlots + slot] = sidx;
-                st[14 * kStageSlots + slot] = (uint32_t)pid;
-                st[15 * kStageSlots + slot] = (uint32_t)(pid >> 32);
+                f[0 * kStageSlots + at] = pos.x;
+                f[1 * kStageSlots + at] = pos.y;
+                f[2 * kStageSlots + at] = pos.z;
+                f[3 * kStageSlots + at] = dir.x;
+                f[4 * kStageSlots + at] = dir.y;
+                f[5 * kStageSlots + at] = dir.z;
+                f[6 * kStageSlots + at] = idir.x;
+                f[7 * kStageSlots + at] = idir.y;
+                f[8 * kStageSlots + at] = idir.z;
+                f[9 * kStageSlots + at] = wl;
+                f[10 * kStageSlots + at] = time;
+                f[11 * kStageSlots + at] = dist;
+                st[12 * kStageSlots + at] = mid;
+                st[13 * kStageSlots + at] = sidx;
+                st[14 * kStageSlots + at] = (uint32_t)pid;
+                st[15 * kStageSlots + at] = (uint32_t)(pid >> 32);
             }
             staged += cnt;
         }
     }
-    if (staged) flush_stage(st, staged, lane);
-
-    // one atomic per wavefront for the photon and test counters
-    for (int off = 32; off > 0; off >>= 1) {
-        simulated += __shfl_down(simulated, off);
-        tests += __shfl_down(tests, off);
-        ctests += __shfl_down(ctests, off);
-    }
-    if (lane == 0) {
-        atomicAdd((unsigned long long*)&p.counters[1], (unsigned long long)simulated);
-        atomicAdd((unsigned long long*)&p.counters[2], (unsigned long long)tests);
-        atomicAdd((unsigned long long*)&p.counters[3], (unsigned long long)ctests);
-    }
+    if (staged) flush_stage(st, staged, lane, stage_slot);
+    if (count_slot >= 0) commit_counts(count_slot, simulated, tests, ctests, lane);
 }
 
 typedef void (*PropagateFn)(KParams);
@@ -692,16 +743,24 @@ size_t lds_bytes(int n_knots) {
     return sizeof(float) * ((n_knots + 3) & ~3) + sizeof(uint32_t) * kWaves * kStageFields * kStageSlots;
 }
 
-int32_t launch(nt_ctx* c, uint64_t seed, const HitCols& cols, int64_t* d_counters, hipStream_t s) {
-    if (!c->has_medium || !c->has_spectrum || !c->has_targets || !c->has_sources) {
-        snprintf(g_err, sizeof g_err, "medium, spectrum, targets and sources must be set first");
-        return NT_ERR_STATE;
-    }
-    hipLaunchKernelGGL(clear_counters, dim3(1), dim3(64), 0, s, d_counters);
-    if (c->total == 0) return NT_OK;
+// One launch of the propagation kernel over `n_sources` device sources.
+struct LaunchPlan {
+    const DevSource* sources;
+    const int64_t* offsets;
+    int n_sources;
+    int64_t total;                     // padded photon index range
+    int variant;                       // kernel variant (source kind or mixed)
+    const SlotDesc* slots;             // null: single output `cols`, `counters`
+    HitCols cols;
+    int64_t* counters;
+};
+
+int32_t launch_plan(nt_ctx* c, uint64_t seed, const LaunchPlan& lp, hipStream_t s) {
+    if (lp.total == 0) return NT_OK;
     KParams kp{};
-    kp.sources = c->d_sources;
-    kp.offsets = c->d_offsets;
+    kp.sources = lp.sources;
+    kp.offsets = lp.offsets;
+    kp.slots = lp.slots;
     kp.targets = c->d_targets;
     kp.module_ids = c->d_module_ids;
     kp.clusters = c->d_clusters;
@@ -711,20 +770,33 @@ int32_t launch(nt_ctx* c, uint64_t seed, const HitCols& cols, int64_t* d_counter
     kp.member_idx = c->d_member_idx;
     kp.n_clusters = c->culling ? c->n_clusters : 0;
     kp.knots = c->d_knots;
-    kp.counters = d_counters;
-    kp.hits = cols;
-    kp.total = c->total;
+    kp.counters = lp.counters;
+    kp.hits = lp.cols;
+    kp.total = lp.total;
     kp.medium = c->medium;
     kp.spectrum = c->spectrum;
-    kp.n_sources = c->n_sources;
+    kp.n_sources = lp.n_sources;
     kp.n_targets = c->n_targets;
     kp.seed_lo = (uint32_t)seed;
     kp.seed_hi = (uint32_t)(seed >> 32);
-    int64_t need_blocks = (c->total + kThreads - 1) / kThreads;
-    int blocks = (int)std::min<int64_t>(c->blocks_per_variant[c->variant], need_blocks);
-    hipLaunchKernelGGL(propagate_variant(c->variant), dim3(blocks), dim3(kThreads), lds_bytes(c->spectrum.n),
+    const int64_t need_blocks = (lp.total + kThreads - 1) / kThreads;
+    const int blocks = (int)std::min<int64_t>(c->blocks_per_variant[lp.variant], need_blocks);
+    hipLaunchKernelGGL(propagate_variant(lp.variant), dim3(blocks), dim3(kThreads), lds_bytes(c->spectrum.n),
                        s, kp);
     HIP_TRY(hipGetLastError());
+    return NT_OK;
+}
+
+int32_t launch(nt_ctx* c, uint64_t seed, const HitCols& cols, int64_t* d_counters, hipStream_t s) {
+    if (!c->has_medium || !c->has_spectrum || !c->has_targets || !c->has_sources) {
+        snprintf(g_err, sizeof g_err, "medium, spectrum, targets and sources must be set first");
+        return NT_ERR_STATE;
+    }
+    hipLaunchKernelGGL(clear_counters, dim3(1), dim3(64), 0, s, d_counters);
+    if (c->total == 0) return NT_OK;
+    LaunchPlan lp{c->d_sources, c->d_offsets, c->n_sources, c->total, c->variant, nullptr, cols, d_counters};
+    const int32_t st = launch_plan(c, seed, lp, s);
+    if (st) return st;
     if (c->epi.enabled && (cols.pmt_id || cols.total_weight || cols.tres)) {
         const int eb = (int)std::max<int64_t>(1, std::min<int64_t>((cols.capacity + 255) / 256, 1024));
         hipLaunchKernelGGL(epilogue_kernel, dim3(eb), dim3(256), 0, s, c->epi, cols, d_counters,
@@ -808,6 +880,37 @@ int32_t rebuild_epilogue(nt_ctx* c) {
     d.n_acc = (int)nc;
     d.abs_len_scale = c->abs_len_scale;
     d.enabled = 1;
+    return NT_OK;
+}
+
+int64_t round_up64(int64_t n) { return (n + 63) & ~int64_t(63); }
+
+// Device record of an nt_source (offset, key, base and slot left at 0).
+int32_t make_dev_source(const nt_source& a, int i, DevSource& d) {
+    if (a.photons < 0 || a.photons > (int64_t(1) << 56) || a.kind < NT_SRC_POINTLIKE_ISOTROPIC ||
+        a.kind > NT_SRC_POINTLIKE_CHERENKOV) {
+        snprintf(g_err, sizeof g_err, "source %d: photons must be in [0, 2^56] and kind known", i);
+        return NT_ERR_ARG;
+    }
+    memset(&d, 0, sizeof d);
+    d.kind = a.kind;
+    d.rng_index = (uint32_t)i;
+    d.photons = a.photons;
+    for (int k = 0; k < 3; ++k) { d.pos[k] = a.position[k]; d.dir[k] = a.direction[k]; }
+    d.t0 = a.time;
+    d.t1 = a.time_end;
+    d.angle = a.angle;
+    if (a.kind == NT_SRC_EXTENDED_CHERENKOV) {
+        if (!(a.long_a > 1.0f && a.long_b > 0.0f && a.long_lrad > 0.0f)) {
+            snprintf(g_err, sizeof g_err, "source %d: longitudinal parameters need a > 1, b > 0, lrad > 0", i);
+            return NT_ERR_ARG;
+        }
+        // rand_gamma(T(a), T(1/b)) set-up, utils.jl:269-270 and photon_prop_cuda.jl:168
+        d.gamma_d = (float)((double)a.long_a - 1.0 / 3.0);
+        d.gamma_c = 1.0f / sqrtf(9.0f * d.gamma_d);
+        d.gamma_scale = 1.0f / a.long_b;
+        d.lrad = a.long_lrad;
+    }
     return NT_OK;
 }
 
@@ -1045,33 +1148,10 @@ int32_t nt_set_sources(nt_ctx* c, const nt_source* s, int32_t n) {
     std::vector<int64_t> offs(n + 1);
     offs[0] = 0;
     for (int i = 0; i < n; ++i) {
-        const nt_source& a = s[i];
-        if (a.photons < 0 || a.kind < NT_SRC_POINTLIKE_ISOTROPIC || a.kind > NT_SRC_POINTLIKE_CHERENKOV) {
-            snprintf(g_err, sizeof g_err, "source %d: photons must be >= 0 and kind known", i);
-            return NT_ERR_ARG;
-        }
-        DevSource& d = ds[i];
-        memset(&d, 0, sizeof d);
-        d.kind = a.kind;
-        d.index = (uint32_t)i;
-        d.offset = offs[i];
-        d.photons = a.photons;
-        for (int k = 0; k < 3; ++k) { d.pos[k] = a.position[k]; d.dir[k] = a.direction[k]; }
-        d.t0 = a.time;
-        d.t1 = a.time_end;
-        d.angle = a.angle;
-        if (a.kind == NT_SRC_EXTENDED_CHERENKOV) {
-            if (!(a.long_a > 1.0f && a.long_b > 0.0f && a.long_lrad > 0.0f)) {
-                snprintf(g_err, sizeof g_err, "source %d: longitudinal parameters need a > 1, b > 0, lrad > 0", i);
-                return NT_ERR_ARG;
-            }
-            // rand_gamma(T(a), T(1/b)) set-up, utils.jl:269-270 and photon_prop_cuda.jl:168
-            d.gamma_d = (float)((double)a.long_a - 1.0 / 3.0);
-            d.gamma_c = 1.0f / sqrtf(9.0f * d.gamma_d);
-            d.gamma_scale = 1.0f / a.long_b;
-            d.lrad = a.long_lrad;
-        }
-        offs[i + 1] = offs[i] + a.photons;
+        const int32_t st = make_dev_source(s[i], i, ds[i]);
+        if (st) return st;
+        ds[i].offset = offs[i];
+        offs[i + 1] = offs[i] + round_up64(s[i].photons);
     }
     std::lock_guard<std::mutex> lk(c->mu);
     HIP_TRY(hipSetDevice(c->device));
@@ -1243,6 +1323,509 @@ int32_t nt_eval(nt_ctx* c, int32_t fn, const float* in, int32_t iw, float* out, 
     (void)hipFree(d_out);
     if (e != hipSuccess) return fail_hip(e, "nt_eval");
     return NT_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Serving: continuous batching of propagation requests (nt_server_*)
+// ===========================================================================
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <thread>
+#include <unordered_map>
+
+namespace {
+
+constexpr size_t kAlign = 256;
+size_t align_up(size_t n) { return (n + kAlign - 1) & ~(kAlign - 1); }
+
+// First-fit allocator over a byte range [0, size).
+class Arena {
+   public:
+    void init(size_t size) {
+        free_.clear();
+        if (size) free_[0] = size;
+    }
+    bool alloc(size_t n, size_t& off) {
+        n = align_up(n);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < n) continue;
+            off = it->first;
+            const size_t rest = it->second - n;
+            free_.erase(it);
+            if (rest) free_[off + n] = rest;
+            return true;
+        }
+        return false;
+    }
+    void release(size_t off, size_t n) {
+        n = align_up(n);
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+
+   private:
+    std::map<size_t, size_t> free_;
+};
+
+// Column layout of one request's hits inside an arena block (device and pinned host alike).
+struct ReqLayout {
+    size_t counters, pos, dir, idir, wl, time, dist, mid, sidx, pid, bytes;
+    explicit ReqLayout(int64_t cap) {
+        size_t o = 0;
+        auto take = [&](size_t n) { size_t at = o; o += align_up(n); return at; };
+        counters = take(4 * sizeof(int64_t));
+        pos = take(12 * cap);
+        dir = take(12 * cap);
+        idir = take(12 * cap);
+        wl = take(4 * cap);
+        time = take(4 * cap);
+        dist = take(4 * cap);
+        mid = take(2 * cap);
+        sidx = take(4 * cap);
+        pid = take(8 * cap);
+        bytes = o;
+    }
+    HitCols cols(char* base, int64_t cap) const {
+        HitCols h{};
+        h.pos = (float*)(base + pos); h.dir = (float*)(base + dir); h.idir = (float*)(base + idir);
+        h.wl = (float*)(base + wl); h.time = (float*)(base + time); h.dist = (float*)(base + dist);
+        h.module_id = (uint16_t*)(base + mid); h.source_index = (uint32_t*)(base + sidx);
+        h.photon_id = (uint64_t*)(base + pid); h.capacity = cap;
+        return h;
+    }
+};
+
+enum ReqState { RQ_QUEUED, RQ_ACTIVE, RQ_SCHEDULED, RQ_COPY_COUNTS, RQ_COPY_HITS, RQ_DONE, RQ_FAILED };
+
+struct Request {
+    uint64_t id = 0;
+    std::vector<DevSource> src;        // offset, slot, key, base set per round
+    std::vector<int64_t> next;         // next photon to schedule, per source
+    int64_t photons = 0, scheduled = 0;
+    uint64_t seed = 0;
+    int64_t capacity = 0;
+    size_t bytes = 0, dev_off = 0, host_off = 0;
+    int state = RQ_QUEUED;
+    uint64_t last_round = 0;
+    int64_t n_found = 0, n_sim = 0;
+    hipEvent_t ev = nullptr;           // completion of the pending copy
+    int32_t status = NT_OK;
+    std::chrono::steady_clock::time_point t_submit, t_done;
+};
+
+struct Staging {                       // per-round launch tables (pinned host + device)
+    DevSource* h_src = nullptr;
+    int64_t* h_off = nullptr;
+    SlotDesc* h_slot = nullptr;
+    DevSource* d_src = nullptr;
+    int64_t* d_off = nullptr;
+    SlotDesc* d_slot = nullptr;
+    int cap = 0;                       // sources (and slots) it can hold
+    hipEvent_t start = nullptr, done = nullptr;
+    bool busy = false;
+    uint64_t round = 0;
+    int64_t photons = 0;
+};
+
+}  // namespace
+
+struct nt_server {
+    nt_ctx* ctx = nullptr;
+    nt_server_config cfg{};
+    hipStream_t stream = nullptr, copy_stream = nullptr;
+    char* d_arena = nullptr;
+    char* h_arena = nullptr;
+    Arena dev_alloc, host_alloc;
+    std::vector<Staging> staging;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<uint64_t> queue;                        // waiting for memory, FIFO
+    std::vector<uint64_t> active;                      // admitted, in admission order
+    std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
+    uint64_t next_id = 1, next_round = 1;
+    bool stop = false;
+    int32_t fatal = NT_OK;
+    nt_server_stats stats{};
+    std::thread worker;
+};
+
+namespace {
+
+int32_t ensure_staging(Staging& s, int need) {
+    if (need <= s.cap) return NT_OK;
+    int cap = std::max(need, 2 * s.cap);
+    if (s.h_src) { (void)hipHostFree(s.h_src); (void)hipHostFree(s.h_off); (void)hipHostFree(s.h_slot); }
+    if (s.d_src) { (void)hipFree(s.d_src); (void)hipFree(s.d_off); (void)hipFree(s.d_slot); }
+    s = Staging{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, s.start, s.done, false, 0, 0};
+    HIP_TRY(hipHostMalloc((void**)&s.h_src, sizeof(DevSource) * cap));
+    HIP_TRY(hipHostMalloc((void**)&s.h_off, sizeof(int64_t) * (cap + 1)));
+    HIP_TRY(hipHostMalloc((void**)&s.h_slot, sizeof(SlotDesc) * cap));
+    HIP_TRY(hipMalloc((void**)&s.d_src, sizeof(DevSource) * cap));
+    HIP_TRY(hipMalloc((void**)&s.d_off, sizeof(int64_t) * (cap + 1)));
+    HIP_TRY(hipMalloc((void**)&s.d_slot, sizeof(SlotDesc) * cap));
+    s.cap = cap;
+    return NT_OK;
+}
+
+// Admit queued requests in order while their buffers fit.
+void admit(nt_server* sv) {
+    while (!sv->queue.empty()) {
+        Request& r = *sv->reqs[sv->queue.front()];
+        size_t doff, hoff;
+        if (!sv->dev_alloc.alloc(r.bytes, doff)) return;
+        if (!sv->host_alloc.alloc(r.bytes, hoff)) { sv->dev_alloc.release(doff, r.bytes); return; }
+        r.dev_off = doff;
+        r.host_off = hoff;
+        r.state = RQ_ACTIVE;
+        if (hipMemsetAsync(sv->d_arena + doff + ReqLayout(r.capacity).counters, 0, 4 * sizeof(int64_t),
+                           sv->stream) != hipSuccess)
+            sv->fatal = NT_ERR_DEVICE;
+        sv->active.push_back(r.id);
+        sv->queue.pop_front();
+    }
+}
+
+// Pack photon chunks of the active requests into one round and launch it.
+int32_t launch_round(nt_server* sv, Staging& stg) {
+    std::vector<Request*> want;
+    for (uint64_t id : sv->active) {
+        Request& r = *sv->reqs[id];
+        if (r.state == RQ_ACTIVE && r.scheduled < r.photons) want.push_back(&r);
+    }
+    if (want.empty()) return NT_ERR_NOT_READY;
+    int64_t budget = sv->cfg.round_photons;
+    const int64_t share = std::max<int64_t>(budget / (int64_t)want.size(), 1 << 16);
+    int n_src_max = 0;
+    for (Request* r : want) n_src_max += (int)r->src.size();
+    int32_t st = ensure_staging(stg, n_src_max);
+    if (st) return st;
+    const uint64_t round = sv->next_round++;
+    int ns = 0, nslot = 0, kind = -2;
+    int64_t off = 0, photons = 0;
+    for (Request* r : want) {
+        if (budget <= 0) break;
+        int64_t take = std::min({share, budget, r->photons - r->scheduled});
+        const ReqLayout L(r->capacity);
+        SlotDesc& sd = stg.h_slot[nslot];
+        sd.cols = L.cols(sv->d_arena + r->dev_off, r->capacity);
+        sd.counters = (int64_t*)(sv->d_arena + r->dev_off + L.counters);
+        for (size_t j = 0; j < r->src.size() && take > 0; ++j) {
+            const int64_t left = r->src[j].photons - r->next[j];
+            if (left <= 0) continue;
+            const int64_t c = std::min(take, left);
+            DevSource d = r->src[j];
+            d.photons = c;
+            d.photon_base = (uint64_t)r->next[j];
+            d.key_lo = (uint32_t)r->seed;
+            d.key_hi = (uint32_t)(r->seed >> 32);
+            d.rng_index = (uint32_t)j;
+            d.slot = nslot;
+            d.offset = off;
+            stg.h_off[ns] = off;
+            stg.h_src[ns++] = d;
+            off += round_up64(c);
+            r->next[j] += c;
+            r->scheduled += c;
+            take -= c;
+            budget -= c;
+            photons += c;
+            kind = (kind == -2 || kind == d.kind) ? d.kind : -1;
+        }
+        ++nslot;
+        if (r->scheduled == r->photons) {
+            r->state = RQ_SCHEDULED;
+            r->last_round = round;
+        }
+    }
+    stg.h_off[ns] = off;
+    HIP_TRY(hipMemcpyAsync(stg.d_src, stg.h_src, sizeof(DevSource) * ns, hipMemcpyHostToDevice, sv->stream));
+    HIP_TRY(hipMemcpyAsync(stg.d_off, stg.h_off, sizeof(int64_t) * (ns + 1), hipMemcpyHostToDevice, sv->stream));
+    HIP_TRY(hipMemcpyAsync(stg.d_slot, stg.h_slot, sizeof(SlotDesc) * nslot, hipMemcpyHostToDevice, sv->stream));
+    HIP_TRY(hipEventRecord(stg.start, sv->stream));
+    {
+        std::lock_guard<std::mutex> lk(sv->ctx->mu);
+        LaunchPlan lp{stg.d_src, stg.d_off, ns, off, kind < 0 ? kKernelVariants - 1 : kind, stg.d_slot,
+                      HitCols{}, nullptr};
+        st = launch_plan(sv->ctx, 0, lp, sv->stream);
+        if (st) return st;
+    }
+    HIP_TRY(hipEventRecord(stg.done, sv->stream));
+    stg.busy = true;
+    stg.round = round;
+    stg.photons = photons;
+    // counters of the requests this round completes go to pinned memory as soon as it ends
+    for (Request* r : want) {
+        if (r->state != RQ_SCHEDULED || r->last_round != round) continue;
+        const ReqLayout L(r->capacity);
+        HIP_TRY(hipStreamWaitEvent(sv->copy_stream, stg.done, 0));
+        HIP_TRY(hipMemcpyAsync(sv->h_arena + r->host_off + L.counters, sv->d_arena + r->dev_off + L.counters,
+                               4 * sizeof(int64_t), hipMemcpyDeviceToHost, sv->copy_stream));
+        HIP_TRY(hipEventRecord(r->ev, sv->copy_stream));
+        r->state = RQ_COPY_COUNTS;
+    }
+    sv->stats.rounds += 1;
+    sv->stats.photons += photons;
+    return NT_OK;
+}
+
+// Advance copies of finished requests; returns true if anything changed.
+bool progress(nt_server* sv) {
+    bool changed = false;
+    for (Staging& s : sv->staging) {
+        if (!s.busy || hipEventQuery(s.done) != hipSuccess) continue;
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, s.start, s.done) == hipSuccess) sv->stats.busy_ms += ms;
+        s.busy = false;
+        changed = true;
+    }
+    for (size_t i = 0; i < sv->active.size();) {
+        Request& r = *sv->reqs[sv->active[i]];
+        const hipError_t q = (r.state == RQ_COPY_COUNTS || r.state == RQ_COPY_HITS) ? hipEventQuery(r.ev)
+                                                                                  : hipErrorNotReady;
+        if (q != hipSuccess && q != hipErrorNotReady) { sv->fatal = NT_ERR_DEVICE; return true; }
+        if (q == hipSuccess && r.state == RQ_COPY_COUNTS) {
+            const ReqLayout L(r.capacity);
+            const int64_t* cnt = (const int64_t*)(sv->h_arena + r.host_off + L.counters);
+            r.n_found = cnt[0];
+            r.n_sim = cnt[1];
+            const int64_t n = std::min(r.n_found, r.capacity);
+            char* d = sv->d_arena + r.dev_off;
+            char* h = sv->h_arena + r.host_off;
+            const size_t off[9] = {L.pos, L.dir, L.idir, L.wl, L.time, L.dist, L.mid, L.sidx, L.pid};
+            const size_t w[9] = {12, 12, 12, 4, 4, 4, 2, 4, 8};
+            bool ok = true;
+            for (int k = 0; k < 9 && n > 0; ++k)
+                ok &= hipMemcpyAsync(h + off[k], d + off[k], w[k] * n, hipMemcpyDeviceToHost, sv->copy_stream) ==
+                      hipSuccess;
+            ok &= hipEventRecord(r.ev, sv->copy_stream) == hipSuccess;
+            if (!ok) { sv->fatal = NT_ERR_DEVICE; return true; }
+            r.state = RQ_COPY_HITS;
+            changed = true;
+        } else if (q == hipSuccess && r.state == RQ_COPY_HITS) {
+            r.t_done = std::chrono::steady_clock::now();
+            r.state = RQ_DONE;
+            r.status = r.n_found > r.capacity ? NT_ERR_HIT_OVERFLOW : NT_OK;
+            sv->dev_alloc.release(r.dev_off, r.bytes);
+            sv->active.erase(sv->active.begin() + i);
+            sv->stats.completed += 1;
+            changed = true;
+            continue;
+        }
+        ++i;
+    }
+    if (changed) sv->cv_done.notify_all();
+    return changed;
+}
+
+void server_loop(nt_server* sv) {
+    (void)hipSetDevice(sv->ctx->device);
+    std::unique_lock<std::mutex> lk(sv->mu);
+    while (!sv->stop && sv->fatal == NT_OK) {
+        admit(sv);
+        bool launched = false;
+        for (Staging& s : sv->staging) {
+            if (s.busy) continue;
+            const int32_t st = launch_round(sv, s);
+            if (st == NT_OK) launched = true;
+            else if (st != NT_ERR_NOT_READY) sv->fatal = st;
+            break;
+        }
+        const bool changed = progress(sv);
+        if (!launched && !changed) {
+            const bool idle = sv->queue.empty() && sv->active.empty();
+            if (idle) sv->cv_work.wait(lk);
+            else sv->cv_work.wait_for(lk, std::chrono::microseconds(20));
+        }
+    }
+    if (sv->fatal != NT_OK) {
+        for (auto& kv : sv->reqs)
+            if (kv.second->state != RQ_DONE) { kv.second->state = RQ_FAILED; kv.second->status = sv->fatal; }
+        sv->cv_done.notify_all();
+    }
+}
+
+void free_server(nt_server* sv) {
+    for (Staging& s : sv->staging) {
+        if (s.h_src) { (void)hipHostFree(s.h_src); (void)hipHostFree(s.h_off); (void)hipHostFree(s.h_slot); }
+        if (s.d_src) { (void)hipFree(s.d_src); (void)hipFree(s.d_off); (void)hipFree(s.d_slot); }
+        if (s.start) (void)hipEventDestroy(s.start);
+        if (s.done) (void)hipEventDestroy(s.done);
+    }
+    for (auto& kv : sv->reqs)
+        if (kv.second->ev) (void)hipEventDestroy(kv.second->ev);
+    if (sv->d_arena) (void)hipFree(sv->d_arena);
+    if (sv->h_arena) (void)hipHostFree(sv->h_arena);
+    if (sv->stream) (void)hipStreamDestroy(sv->stream);
+    if (sv->copy_stream) (void)hipStreamDestroy(sv->copy_stream);
+    delete sv;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t nt_server_create(nt_ctx* c, const nt_server_config* cfg, nt_server** out) {
+    if (!c || !cfg || !out || cfg->device_budget_bytes < kAlign || cfg->host_budget_bytes < kAlign ||
+        cfg->round_photons < 64 || cfg->max_inflight_rounds < 1 || cfg->max_inflight_rounds > 16) {
+        snprintf(g_err, sizeof g_err, "server config: budgets >= 256 B, round_photons >= 64, 1..16 rounds in flight");
+        return NT_ERR_ARG;
+    }
+    if (!c->has_medium || !c->has_spectrum || !c->has_targets) {
+        snprintf(g_err, sizeof g_err, "set medium, spectrum and targets before creating a server");
+        return NT_ERR_STATE;
+    }
+    *out = nullptr;
+    HIP_TRY(hipSetDevice(c->device));
+    nt_server* sv = new nt_server();
+    sv->ctx = c;
+    sv->cfg = *cfg;
+    auto fail = [&](hipError_t e, const char* what) { free_server(sv); return fail_hip(e, what); };
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking)) != hipSuccess) return fail(e, "stream");
+    if ((e = hipStreamCreateWithFlags(&sv->copy_stream, hipStreamNonBlocking)) != hipSuccess) return fail(e, "stream");
+    if ((e = hipMalloc((void**)&sv->d_arena, cfg->device_budget_bytes)) != hipSuccess) return fail(e, "device budget");
+    if ((e = hipHostMalloc((void**)&sv->h_arena, cfg->host_budget_bytes)) != hipSuccess) return fail(e, "host budget");
+    sv->dev_alloc.init(cfg->device_budget_bytes);
+    sv->host_alloc.init(cfg->host_budget_bytes);
+    sv->staging.resize(cfg->max_inflight_rounds);
+    for (Staging& s : sv->staging) {
+        if ((e = hipEventCreate(&s.start)) != hipSuccess) return fail(e, "event");
+        if ((e = hipEventCreate(&s.done)) != hipSuccess) return fail(e, "event");
+    }
+    sv->worker = std::thread(server_loop, sv);
+    *out = sv;
+    return NT_OK;
+}
+
+int32_t nt_server_submit(nt_server* sv, const nt_source* s, int32_t n, uint64_t seed, int64_t cap, uint64_t* id) {
+    if (!sv || n < 1 || !s || cap < 0 || !id) return NT_ERR_ARG;
+    auto r = std::make_unique<Request>();
+    r->src.resize(n);
+    r->next.assign(n, 0);
+    for (int i = 0; i < n; ++i) {
+        const int32_t st = make_dev_source(s[i], i, r->src[i]);
+        if (st) return st;
+        r->photons += s[i].photons;
+    }
+    r->seed = seed;
+    r->capacity = cap;
+    r->bytes = ReqLayout(cap).bytes;
+    if (r->bytes > sv->cfg.device_budget_bytes || r->bytes > sv->cfg.host_budget_bytes) {
+        snprintf(g_err, sizeof g_err, "request needs %zu bytes of hit buffers, above the budget", r->bytes);
+        return NT_ERR_BUDGET;
+    }
+    if (hipSetDevice(sv->ctx->device) != hipSuccess || hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess)
+        return NT_ERR_DEVICE;
+    r->t_submit = std::chrono::steady_clock::now();
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (sv->fatal != NT_OK) { (void)hipEventDestroy(r->ev); return sv->fatal; }
+    r->id = sv->next_id++;
+    *id = r->id;
+    if (r->photons == 0) {                       // nothing to propagate: complete at once
+        r->state = RQ_DONE;
+        r->t_done = r->t_submit;
+        sv->stats.completed += 1;
+    } else {
+        sv->queue.push_back(r->id);
+    }
+    sv->reqs[r->id] = std::move(r);
+    sv->stats.submitted += 1;
+    sv->cv_work.notify_one();
+    return NT_OK;
+}
+
+int32_t nt_server_poll(nt_server* sv, uint64_t id) {
+    if (!sv) return NT_ERR_ARG;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    auto it = sv->reqs.find(id);
+    if (it == sv->reqs.end()) return NT_ERR_ARG;
+    if (it->second->state == RQ_FAILED) return it->second->status;
+    return it->second->state == RQ_DONE ? NT_OK : NT_ERR_NOT_READY;
+}
+
+int32_t nt_server_wait(nt_server* sv, uint64_t id, const nt_hits* out, int64_t* n_hits, int64_t* n_sim,
+                       int64_t* n_dropped, double* latency_ms) {
+    if (!sv || !out) return NT_ERR_ARG;
+    std::unique_ptr<Request> r;
+    {
+        std::unique_lock<std::mutex> lk(sv->mu);
+        auto it = sv->reqs.find(id);
+        if (it == sv->reqs.end()) return NT_ERR_ARG;
+        Request* rp = it->second.get();
+        if (out->capacity < rp->capacity) {
+            snprintf(g_err, sizeof g_err, "output capacity below the request's hit capacity");
+            return NT_ERR_ARG;
+        }
+        sv->cv_done.wait(lk, [&] { return rp->state == RQ_DONE || rp->state == RQ_FAILED; });
+        r = std::move(it->second);
+        sv->reqs.erase(it);
+    }
+    int32_t status = r->state == RQ_FAILED ? r->status : NT_OK;
+    const int64_t n = std::min(r->n_found, r->capacity);
+    if (status == NT_OK && r->photons > 0 && n > 0) {
+        const ReqLayout L(r->capacity);
+        const char* h = sv->h_arena + r->host_off;
+        memcpy(out->position, h + L.pos, 12 * n);
+        memcpy(out->direction, h + L.dir, 12 * n);
+        memcpy(out->initial_direction, h + L.idir, 12 * n);
+        memcpy(out->wavelength, h + L.wl, 4 * n);
+        memcpy(out->time, h + L.time, 4 * n);
+        memcpy(out->dist_travelled, h + L.dist, 4 * n);
+        memcpy(out->module_id, h + L.mid, 2 * n);
+        if (out->source_index) memcpy(out->source_index, h + L.sidx, 4 * n);
+        if (out->photon_id) memcpy(out->photon_id, h + L.pid, 8 * n);
+    }
+    if (n_hits) *n_hits = status == NT_OK ? n : 0;
+    if (n_sim) *n_sim = r->n_sim;
+    if (n_dropped) *n_dropped = std::max<int64_t>(0, r->n_found - r->capacity);
+    if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+    if (status == NT_OK && r->n_found > r->capacity) status = NT_ERR_HIT_OVERFLOW;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        if (r->photons > 0 && r->state == RQ_DONE) sv->host_alloc.release(r->host_off, r->bytes);
+        sv->cv_work.notify_one();                  // freed host memory may admit the next request
+    }
+    if (r->ev) (void)hipEventDestroy(r->ev);
+    return status;
+}
+
+int32_t nt_server_stats_get(nt_server* sv, nt_server_stats* out) {
+    if (!sv || !out) return NT_ERR_ARG;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    *out = sv->stats;
+    out->queued = (int64_t)sv->queue.size();
+    out->active = (int64_t)sv->active.size();
+    return sv->fatal;
+}
+
+void nt_server_destroy(nt_server* sv) {
+    if (!sv) return;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->stop = true;
+        sv->cv_work.notify_all();
+    }
+    if (sv->worker.joinable()) sv->worker.join();
+    (void)hipSetDevice(sv->ctx->device);
+    (void)hipStreamSynchronize(sv->stream);
+    (void)hipStreamSynchronize(sv->copy_stream);
+    free_server(sv);
 }
 
 }  // extern "C"

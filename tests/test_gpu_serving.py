@@ -1,0 +1,107 @@
+"""Serving (continuous batching): each request's hits are identical to the
+same sources propagated alone with the same seed, whatever else shares its
+rounds; admission waits for memory; requests that cannot fit are refused."""
+import math
+
+import numpy as np
+import pytest
+
+import neutrinotelescopes_amd as nt
+from neutrinotelescopes_amd import _abi
+from neutrinotelescopes_amd.serving import PhotonServer, cascade_requests, cluster_targets, run_poisson_stream
+
+pytestmark = pytest.mark.gpu
+
+COLS = ("position", "direction", "initial_direction", "wavelength", "time", "dist_travelled", "module_id",
+        "source_index", "photon_id")
+
+
+@pytest.fixture(scope="module")
+def serving_engine():
+    import os
+    if not os.path.exists("/dev/kfd"):
+        pytest.skip("no GPU node on this machine")
+    eng = nt.PropagationEngine(0)
+    med = nt.make_cascadia_medium_properties(0.95)
+    eng.set_medium(med)
+    eng.set_spectrum(nt.CherenkovSpectrum((300.0, 800.0), 30, med))
+    eng.set_targets(cluster_targets(radius=2.0))
+    yield eng, med
+    eng.close()
+
+
+def alone(eng, sources, seed, cap):
+    eng.set_sources(sources)
+    return eng.propagate(seed=seed, capacity=cap).sorted()
+
+
+def assert_same(a, b):
+    assert len(a) == len(b)
+    for c in COLS:
+        assert np.array_equal(getattr(a, c), getattr(b, c)), c
+
+
+def test_results_identical_alone_and_in_mix(serving_engine):
+    eng, med = serving_engine
+    rng = np.random.default_rng(5)
+    reqs = cascade_requests(10, med, rng, radius=60.0, height=200.0, e_range=(1e2, 2e4))
+    # ragged extras: an empty request's source, a multi-source request of mixed kinds
+    lp = nt.LongitudinalParameterisation.from_energy(3e3, med, nt.PEMinus)
+    multi = [nt.PointlikeIsotropicEmitter(np.array([5.0, 5.0, 0.0], np.float32), 2.0, 123_457),
+             nt.ExtendedCherenkovEmitter(np.array([-10.0, 0.0, 30.0], np.float32),
+                                         np.array([0.0, 0.6, 0.8], np.float32), 0.0, 300_001, lp),
+             nt.PointlikeIsotropicEmitter(np.zeros(3, np.float32), 0.0, 0)]
+    jobs = [(r.sources, r.seed, r.capacity) for r in reqs] + [(multi, 77, 200_000)]
+    with PhotonServer(eng, device_budget=2 << 30, host_budget=1 << 30, round_photons=400_000) as srv:
+        ids = [srv.submit(s, seed, cap) for s, seed, cap in jobs]
+        got = [srv.wait(i) for i in ids]
+        stats = srv.stats()
+    assert stats["completed"] == len(jobs) and stats["rounds"] > 1
+    total_hits = 0
+    for (sources, seed, cap), (h, lat) in zip(jobs, got):
+        assert lat > 0
+        ref = alone(eng, sources, seed, cap)
+        assert h.n_simulated == sum(s.photons for s in sources) == ref.n_simulated
+        assert_same(h.sorted(), ref)
+        total_hits += len(ref)
+    assert total_hits > 100
+
+
+def test_small_budget_admits_in_turn(serving_engine):
+    eng, med = serving_engine
+    rng = np.random.default_rng(6)
+    reqs = cascade_requests(6, med, rng, radius=60.0, height=200.0, e_range=(1e3, 1e4))
+    cap = 50_000
+    one = 4096 + cap * 62 + 10 * 256                      # one request's hit buffers, rounded up
+    with PhotonServer(eng, device_budget=2 * one, host_budget=2 * one, round_photons=200_000) as srv:
+        ids = [srv.submit(r.sources, r.seed, cap) for r in reqs]
+        out = [srv.wait(i)[0] for i in ids]
+    for r, h in zip(reqs, out):
+        assert_same(h.sorted(), alone(eng, r.sources, r.seed, cap))
+
+
+def test_refusal_and_overflow(serving_engine):
+    eng, med = serving_engine
+    src = [nt.PointlikeIsotropicEmitter(np.array([0.0, 0.0, 0.0], np.float32), 0.0, 400_000)]
+    with PhotonServer(eng, device_budget=1 << 20, host_budget=1 << 20, round_photons=100_000) as srv:
+        with pytest.raises(_abi.NTError) as e:
+            srv.submit(src, 1, 10_000_000)                  # 620 MB of hit buffers, budget 1 MiB
+        assert e.value.status == _abi.NT_ERR_BUDGET
+        rid = srv.submit(src, 1, 3)
+        h, _ = srv.wait(rid, raise_on_overflow=False)
+        full = alone(eng, src, 1, 100_000)
+        assert len(h) == 3 and h.n_dropped == len(full) - 3
+        rid = srv.submit(src, 1, 3)
+        with pytest.raises(_abi.NTError) as e:
+            srv.wait(rid)
+        assert e.value.status == _abi.NT_ERR_HIT_OVERFLOW
+
+
+def test_poisson_stream_latency(serving_engine):
+    eng, med = serving_engine
+    rng = np.random.default_rng(7)
+    reqs = cascade_requests(40, med, rng, e_range=(1e2, 1e4))
+    with PhotonServer(eng, device_budget=4 << 30, host_budget=2 << 30, round_photons=5_000_000) as srv:
+        res = run_poisson_stream(srv, reqs, rate_hz=400.0, rng=rng)
+    assert res["requests"] == 40 and np.all(res["latency_ms"] > 0)
+    assert res["p99_ms"] >= res["p50_ms"] > 0
