@@ -92,6 +92,41 @@ def cpu_baseline(workload: str, target_s: float = 12.0):
                       f"(CPU oracle, g++ -O3, OpenMP)"}
 
 
+def serving_benchmark(duration_s: float = 8.0, seed: int = 11):
+    """Serving configuration: Poisson stream of cascade requests (position
+    uniform in a 300 m cylinder, isotropic direction, E log-uniform 1e2-1e6 GeV,
+    1e4·E/100 GeV photons) against a 7-string × 20-module cluster, arrival
+    rate 0.7 × the server's measured capacity."""
+    import neutrinotelescopes_amd as nt
+    from neutrinotelescopes_amd.propagation import PropagationEngine
+    from neutrinotelescopes_amd.serving import PhotonServer, cascade_requests, cluster_targets, run_poisson_stream
+    med = nt.make_cascadia_medium_properties(0.95)
+    eng = PropagationEngine(0)
+    eng.set_medium(med)
+    eng.set_spectrum(nt.CherenkovSpectrum((300.0, 800.0), 30, med))
+    eng.set_targets(cluster_targets())
+    rng = np.random.default_rng(seed)
+    mean_photons = 1e4 * (1e6 - 1e2) / math.log(1e4) / 100.0
+    with PhotonServer(eng, device_budget=32 << 30, host_budget=8 << 30, round_photons=20_000_000) as srv:
+        # capacity: a burst submitted at once
+        burst = cascade_requests(64, med, rng)
+        t0 = time.perf_counter()
+        ids = [srv.submit(r.sources, r.seed, r.capacity) for r in burst]
+        for i in ids:
+            srv.wait(i, raise_on_overflow=False)
+        cap_rate = sum(s.photons for r in burst for s in r.sources) / (time.perf_counter() - t0)
+        rate = 0.7 * cap_rate / mean_photons
+        n = max(50, int(rate * duration_s))
+        reqs = cascade_requests(n, med, rng)
+        res = run_poisson_stream(srv, reqs, rate_hz=rate, rng=rng)
+        st = srv.stats()
+    eng.close()
+    return {"config": "poisson_cascades_1e2-1e6GeV_7x20_cluster", "requests": res["requests"],
+            "arrival_rate_hz": rate, "p50_ms": res["p50_ms"], "p99_ms": res["p99_ms"],
+            "photons_per_s": res["photons_per_s"], "capacity_photons_per_s": cap_rate,
+            "rounds": st["rounds"], "device_busy_fraction": st["busy_ms"] / 1e3 / res["wall_s"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +135,8 @@ def main():
     ap.add_argument("--workload", default="cascade", choices=["cascade", "isotropic"])
     ap.add_argument("--photons", type=int, default=10_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving (p50/p99) measurement")
+    ap.add_argument("--serving-only", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device-to-device copy 5 times (reference for "
                          "FETCH_SIZE/WRITE_SIZE counter scaling)")
@@ -112,6 +149,9 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.serving_only:
+        print(json.dumps({"serving": serving_benchmark()}), flush=True)
+        return
     if args.copy_calibration:
         a = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
         b = torch.empty_like(a)
@@ -204,6 +244,9 @@ def main():
                      "kernel": "propagate_kernel", "kernel_ms": k_ms,
                      "flop_per_launch": flop},
     }
+    eng.close()
+    if not args.no_serving and args.workload == "cascade":
+        out["serving"] = serving_benchmark()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload)
         out["gpu_vs_cpu"] = value / out["cpu_baseline"]["value"]
