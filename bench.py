@@ -12,7 +12,9 @@ Workload (one step = one launch of the propagation kernel over the batch):
 
 Prints one JSON line.  `roofline` is for the propagation kernel (FP32 VALU
 bound): algorithmic FLOP per launch from the kernel's own step/test counters
-÷ the kernel time measured with HIP events on its stream.  `cpu_baseline`
+÷ the kernel time measured with HIP events on its stream; `traffic` is the
+HBM bytes per launch from the committed rocprofv3 summary for this workload.
+`cpu_baseline`
 times the CPU oracle (the reference algorithm restated in C, OpenMP) on a
 bounded sample of the same workload.
 """
@@ -46,6 +48,22 @@ FLOP_STEP = 61 + 4
 FLOP_TEST = 18
 FLOP_STRING_BOUND = 27   # z-range + segment/axis distance test of one string
 FLOP_INIT = {"isotropic": 49 + 9, "cascade": 49 + 9 + 95 + 20}
+
+
+PROFILE = ROOT / "profiles" / "propagate_kernel_cascade_1e7ph.md"
+
+
+def profiled_traffic(args):
+    """HBM bytes per launch of the propagation kernel from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (scaled by the copy calibration),
+    for the configuration that summary was taken on; None otherwise."""
+    if (args.workload != "cascade" or args.photons != 10_000_000 or args.no_culling
+            or not PROFILE.exists()):
+        return None
+    for line in PROFILE.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":", 1)[1])
+    return None
 
 
 def build_workload(name: str, photons: int):
@@ -240,7 +258,7 @@ def main():
                    "sphere_tests_per_scattering_step": tests_per_step / max(steps_total, 1),
                    "string_bound_tests_per_scattering_step": bound_tests_per_step / max(steps_total, 1)},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": profiled_traffic(args),
                      "kernel": "propagate_kernel", "kernel_ms": k_ms,
                      "flop_per_launch": flop},
     }

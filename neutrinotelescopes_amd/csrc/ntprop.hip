@@ -388,7 +388,9 @@ __global__ __launch_bounds__(kThreads, NTP_MIN_WAVES_PER_SIMD) void propagate_ke
     __syncthreads();
 
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    // wave-uniform by construction; readfirstlane lets the compiler keep the
+    // photon base, source range and counts derived from it in SGPRs
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     uint32_t* st = s_stage + wave * (kStageFields * kStageSlots);
     int staged = 0;                     // hits in the stage (wave-uniform)
     int stage_slot = 0;                 // their output slot
