@@ -212,8 +212,9 @@ int32_t nt_set_option(nt_ctx* ctx, int32_t option, int64_t value);
 /* ---- hit epilogue ------------------------------------------------------
    Replaces make_hits_from_photons / calc_total_weight! / calc_time_residual!
    (photon_prop_cuda.jl:850-924), which the reference runs on the host over
-   the returned hits.  Computed by the propagation kernel when it writes each
-   hit, into the optional pmt_id / total_weight / tres columns:
+   the returned hits.  Computed on the device by an epilogue kernel that runs
+   on the launch stream right after the propagation kernel, over the hits it
+   stored, into the optional pmt_id / total_weight / tres columns:
      pmt_id       first PMT whose axis (the module's (θ, φ) PMT coordinates
                   rotated by `orientation`) lies within asin(sqrt(area/π)/r)
                   of the hit direction seen from the module centre, in FP64
