@@ -189,7 +189,30 @@ SIGNATURES = {
     "nt_server_destroy": (None, [C.c_void_p]),
     "nt_eval": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
                             C.c_int64]),
+    # include/ntflow.h (flow surrogate)
+    "ntf_create": (C.c_int32, [C.c_int32, C.POINTER(C.c_void_p)]),
+    "ntf_destroy": (None, [C.c_void_p]),
+    "ntf_set_weights": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "ntf_get_weights": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "ntf_forward": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ntf_loss_grad": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                  C.c_float, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ntf_train_step": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                   C.c_float, C.c_void_p, C.c_void_p]),
 }
+
+# include/ntflow.h constants
+NTF_N_IN = 16
+NTF_HIDDEN = 128
+NTF_BINS = 8
+NTF_LAYERS = 3
+NTF_N_PARAMS = NTF_LAYERS * (3 * NTF_BINS + 1) + 3
+NTF_P_PAD = 80
+NTF_N_WEIGHTS = (NTF_N_IN * NTF_HIDDEN + NTF_HIDDEN + NTF_HIDDEN * NTF_HIDDEN + NTF_HIDDEN
+                 + NTF_HIDDEN * NTF_P_PAD + NTF_P_PAD)
+NTF_RANGE_MIN, NTF_RANGE_MAX = -20.0, 100.0
+NTF_ROW_TIME, NTF_ROW_COUNT = 1, 2
 
 _lib = None
 

@@ -12,9 +12,11 @@ ROOT = Path(__file__).resolve().parents[1]
 
 
 def declared_functions():
-    text = (ROOT / "include" / "ntprop.h").read_text()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:int32_t|void|const char\*)\s+\**(nt_\w+)\s*\(", text, re.M)))
+    names = set()
+    for header in sorted((ROOT / "include").glob("*.h")):
+        text = re.sub(r"/\*.*?\*/", "", header.read_text(), flags=re.S)
+        names |= set(re.findall(r"^\s*(?:int32_t|void|const char\*)\s+\**(ntf?_\w+)\s*\(", text, re.M))
+    return sorted(names)
 
 
 def test_header_declares_entry_points():
@@ -37,6 +39,15 @@ def test_abi_version_and_status_strings():
     assert lib.nt_abi_version() == 1
     for st in range(0, 8):
         assert lib.nt_status_string(st)
+
+
+def test_flow_constants_match_header():
+    text = (ROOT / "include" / "ntflow.h").read_text()
+    for name in ("NTF_N_IN", "NTF_HIDDEN", "NTF_BINS", "NTF_LAYERS", "NTF_P_PAD"):
+        m = re.search(rf"#define {name} (\d+)", text)
+        assert int(m.group(1)) == getattr(_abi, name), name
+    assert "ntf_train_step" in declared_functions()
+    assert _abi.NTF_N_PARAMS == 78 and _abi.NTF_N_WEIGHTS == 29008
 
 
 def test_struct_layouts():
