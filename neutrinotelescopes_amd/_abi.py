@@ -194,12 +194,19 @@ SIGNATURES = {
     "ntf_destroy": (None, [C.c_void_p]),
     "ntf_set_weights": (C.c_int32, [C.c_void_p, C.c_void_p]),
     "ntf_get_weights": (C.c_int32, [C.c_void_p, C.c_void_p]),
-    "ntf_forward": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
-                                C.c_void_p, C.c_void_p, C.c_void_p]),
-    "ntf_loss_grad": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
-                                  C.c_float, C.c_void_p, C.c_void_p, C.c_void_p]),
-    "ntf_train_step": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
-                                   C.c_float, C.c_void_p, C.c_void_p]),
+    "ntf_forward": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ntf_loss_grad": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_int64, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ntf_train_step": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_int64, C.c_float, C.c_void_p, C.c_void_p]),
+    "ntf_pair_rows": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p,
+                                  C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_float, C.c_void_p,
+                                  C.c_void_p, C.c_void_p]),
+    "ntf_time_logpdf": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64,
+                                    C.c_void_p, C.c_void_p]),
+    "ntf_segment_sum": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
+                                    C.c_int64, C.c_float, C.c_int32, C.c_void_p, C.c_void_p]),
 }
 
 # include/ntflow.h constants
@@ -213,6 +220,7 @@ NTF_N_WEIGHTS = (NTF_N_IN * NTF_HIDDEN + NTF_HIDDEN + NTF_HIDDEN * NTF_HIDDEN + 
                  + NTF_HIDDEN * NTF_P_PAD + NTF_P_PAD)
 NTF_RANGE_MIN, NTF_RANGE_MAX = -20.0, 100.0
 NTF_ROW_TIME, NTF_ROW_COUNT = 1, 2
+NTF_MAX_SCALE_KNOTS = 64
 
 _lib = None
 

@@ -262,13 +262,15 @@ __device__ float inv_layer(const float* u, float v, float& out, float gx, float 
     return gv;
 }
 
-// Row loss −[time·logpdf(t) + count·(n log λ − λ − lgamma(n+1))] from the
-// FP32 outputs p (LDS row); with kGrad, writes scale·d loss / d p into g.
+// Row loss ω·(−[time·logpdf(t) + count·(n log λ − λ − lgamma(n+1))]) from the
+// FP32 outputs p (LDS row), with log λ = p_λ + offset; with kGrad, writes
+// scale·d loss / d p into g.
 template <bool kGrad>
-__device__ float row_loss(const float* p, float t, float cnt, int flags, float* g, float scale) {
+__device__ float row_loss(const float* p, float t, float cnt, int flags, float offset, float weight, float* g,
+                          float scale) {
     const bool ft = (flags & NTF_ROW_TIME) != 0, fc = (flags & NTF_ROW_COUNT) != 0;
     const float* tail = p + kL * kSpl;
-    const float loglam = tail[2];
+    const float loglam = tail[2] + offset;
     float loss = 0.0f;
     if (kGrad) {
 #pragma unroll
@@ -305,10 +307,11 @@ __device__ float row_loss(const float* p, float t, float cnt, int flags, float* 
         if (kGrad) g[kL * kSpl + 2] = lam - cnt;
     }
     if (kGrad) {
+        const float ws = weight * scale;
 #pragma unroll
-        for (int c = 0; c < kP; ++c) g[c] *= scale;
+        for (int c = 0; c < kP; ++c) g[c] *= ws;
     }
-    return loss;
+    return loss * weight;
 }
 
 // ---------------------------------------------------------------------------
@@ -319,6 +322,7 @@ struct Rows {
     const float* t;
     const float* cnt;
     const uint8_t* flags;
+    const float* aux;          // [n][2] (log λ offset, row weight) or NULL
     int64_t n;
 };
 
@@ -332,10 +336,10 @@ __global__ __launch_bounds__(kThreads, 1) void flow_kernel(const float* __restri
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int i = lane & 15, g = lane >> 4;
-    const int64_t row0 = (int64_t)blockIdx.x * kRows;
     const int r0 = wave * 16;
+    const int64_t n_tiles = (rows.n + kRows - 1) / kRows;
 
-    // weights → LDS (bfloat16), biases (FP32), this workgroup's inputs
+    // weights → LDS (bfloat16) and biases (FP32), once per workgroup
     for (int e = tid; e < kIn * kH; e += kThreads) {
         const int f = e / kH, hcol = e % kH;
         sm.w1t[hcol][f] = f2bf(w[oW1 + e]);
@@ -347,6 +351,11 @@ __global__ __launch_bounds__(kThreads, 1) void flow_kernel(const float* __restri
         sm.b[kH + e] = w[ob2 + e];
     }
     for (int e = tid; e < kP; e += kThreads) sm.b[2 * kH + e] = w[ob3 + e];
+
+    // Row tiles: the gradient launch has one tile per workgroup (one partial
+    // record each); the forward launch is persistent over the tiles.
+    for (int64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+    const int64_t row0 = tile * kRows;
     for (int e = tid; e < kRows * kIn; e += kThreads) {
         const int r = e / kIn, f = e % kIn;
         const int64_t gr = row0 + r;
@@ -410,7 +419,9 @@ __global__ __launch_bounds__(kThreads, 1) void flow_kernel(const float* __restri
         float loss = 0.0f;
         if (gr < rows.n) {
             float gp[kP];
-            loss = row_loss<kGrad>(sm.p[r], rows.t[gr], rows.cnt[gr], rows.flags[gr], gp, scale);
+            const float off = rows.aux ? rows.aux[2 * gr] : 0.0f;
+            const float wgt = rows.aux ? rows.aux[2 * gr + 1] : 1.0f;
+            loss = row_loss<kGrad>(sm.p[r], rows.t[gr], rows.cnt[gr], rows.flags[gr], off, wgt, gp, scale);
             if (kGrad) {
 #pragma unroll
                 for (int c = 0; c < kP; ++c) sm.p[r][c] = gp[c];
@@ -428,7 +439,10 @@ __global__ __launch_bounds__(kThreads, 1) void flow_kernel(const float* __restri
         for (int r = 0; r < kRows; ++r) s += (double)sm.loss[r];
         partial_loss[blockIdx.x] = s;
     }
-    if (!kGrad) return;
+    if (!kGrad) {
+        __syncthreads();        // the next tile overwrites the LDS tiles
+        continue;
+    }
 
     float* part = partial + (int64_t)blockIdx.x * kPartW;
     float* pb = part + pBias + wave * kBiasW;
@@ -521,6 +535,8 @@ __global__ __launch_bounds__(kThreads, 1) void flow_kernel(const float* __restri
 #pragma unroll
         for (int j = 0; j < 4; ++j) part[pW1 + (4 * g + j) * kH + nb * 16 + i] = c[j];
     }
+    __syncthreads();
+    }  // tiles
 }
 
 // Sum of the partial records (fixed order) → gradient, or Adam update.
@@ -565,6 +581,129 @@ __global__ void zero_kernel(float* p, int n) {
     if (q < n) p[q] = 0.0f;
 }
 
+// ---------------------------------------------------------------------------
+// Event likelihood helpers
+// ---------------------------------------------------------------------------
+struct PairArgs {
+    float mean[5], inv_std[5];
+    float knot_d[NTF_MAX_SCALE_KNOTS], knot_g[NTF_MAX_SCALE_KNOTS];
+    int n_knots;
+    float log_yield_1gev;
+};
+
+// Conditioning of one (source, module, PMT) pair: ln d, cos θ, cos β, cos φ,
+// log10 E, z-scored; offset = ln(yield(E)·g(d)) with g interpolated in ln d
+// (clamped to the end knots).  Pair index = (s·n_mod + m)·n_pmt + k.
+__global__ __launch_bounds__(256) void pair_rows_kernel(const float* __restrict__ src, int n_src,
+                                                       const float* __restrict__ mod, int n_mod,
+                                                       const float* __restrict__ axes, int n_pmt, PairArgs a,
+                                                       float* __restrict__ x, float* __restrict__ aux) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)n_src * n_mod * n_pmt;
+    if (q >= n) return;
+    const int k = (int)(q % n_pmt);
+    const int64_t sm = q / n_pmt;
+    const int m = (int)(sm % n_mod);
+    const int si = (int)(sm / n_mod);
+    const float* S = src + 8 * (int64_t)si;
+    const float rx = mod[3 * m] - S[0], ry = mod[3 * m + 1] - S[1], rz = mod[3 * m + 2] - S[2];
+    const float d = sqrtf(rx * rx + ry * ry + rz * rz);
+    const float sx = -rx / d, sy = -ry / d, sz = -rz / d;             // module → source
+    const float ux = S[3], uy = S[4], uz = S[5];
+    const float cth = ux * sx + uy * sy + uz * sz;
+    float px = ux - cth * sx, py = uy - cth * sy, pz = uz - cth * sz;
+    float np_ = sqrtf(px * px + py * py + pz * pz);
+    if (!(np_ > 1e-6f)) {                                          // any perpendicular
+        const bool xs = fabsf(sx) < 0.9f;
+        const float ax = xs ? 1.0f : 0.0f, ay = xs ? 0.0f : 1.0f;
+        px = sy * 0.0f - sz * ay;
+        py = sz * ax - sx * 0.0f;
+        pz = sx * ay - sy * ax;
+        np_ = sqrtf(px * px + py * py + pz * pz);
+    }
+    const float e1x = px / np_, e1y = py / np_, e1z = pz / np_;
+    const float e2x = sy * e1z - sz * e1y, e2y = sz * e1x - sx * e1z, e2z = sx * e1y - sy * e1x;
+    const float ax = axes[3 * k], ay = axes[3 * k + 1], az = axes[3 * k + 2];
+    const float cb = ax * sx + ay * sy + az * sz;
+    const float a1 = ax * e1x + ay * e1y + az * e1z;
+    const float a2 = ax * e2x + ay * e2y + az * e2z;
+    const float na = sqrtf(a1 * a1 + a2 * a2);
+    const float cphi = na > 1e-12f ? a1 / na : 1.0f;
+    const float lnd = logf(d);
+    const float f[5] = {lnd, cth, cb, cphi, S[6]};
+    float* xr = x + q * kIn;
+#pragma unroll
+    for (int j = 0; j < 5; ++j) xr[j] = (f[j] - a.mean[j]) * a.inv_std[j];
+#pragma unroll
+    for (int j = 5; j < kIn; ++j) xr[j] = 0.0f;
+    float lg;
+    if (lnd <= a.knot_d[0]) {
+        lg = a.knot_g[0];
+    } else if (lnd >= a.knot_d[a.n_knots - 1]) {
+        lg = a.knot_g[a.n_knots - 1];
+    } else {
+        int j = 0;
+        while (j + 2 < a.n_knots && lnd >= a.knot_d[j + 1]) ++j;
+        const float u = (lnd - a.knot_d[j]) / (a.knot_d[j + 1] - a.knot_d[j]);
+        lg = a.knot_g[j] + u * (a.knot_g[j + 1] - a.knot_g[j]);
+    }
+    aux[2 * q] = a.log_yield_1gev + 2.302585093f * S[6] + lg;
+    aux[2 * q + 1] = 1.0f;
+}
+
+// Σ values over each chunk [chunk_off[c], chunk_off[c+1]) (elements with
+// skip != 0 left out), in double, one workgroup per chunk; then Σ of each
+// segment's chunks [seg_chunk[s], seg_chunk[s+1]) in order.  Chunks are
+// bounded in length, so events of very different sizes cost alike, and the
+// order of additions is fixed (bitwise reproducible sums).
+__global__ __launch_bounds__(256) void chunk_sum_kernel(const float* __restrict__ v,
+                                                       const uint8_t* __restrict__ skip,
+                                                       const int64_t* __restrict__ chunk_off,
+                                                       double* __restrict__ partial) {
+    __shared__ double part[256];
+    const int64_t c = blockIdx.x;
+    const int64_t a = chunk_off[c], b = chunk_off[c + 1];
+    double acc = 0.0;
+    for (int64_t q = a + threadIdx.x; q < b; q += blockDim.x)
+        if (!skip || !skip[q]) acc += (double)v[q];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[c] = part[0];
+}
+
+__global__ __launch_bounds__(256) void segment_sum_kernel(const double* __restrict__ partial,
+                                                         const int64_t* __restrict__ seg_chunk, int64_t n_seg,
+                                                         float sign, int accumulate, double* __restrict__ out) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_seg) return;
+    double acc = 0.0;
+    for (int64_t c = seg_chunk[s]; c < seg_chunk[s + 1]; ++c) acc += partial[c];
+    out[s] = (accumulate ? out[s] : 0.0) + (double)sign * acc;
+}
+
+// logpdf of each hit's time under the flow parameters of its pair.
+__global__ __launch_bounds__(256) void time_logpdf_kernel(const float* __restrict__ params, int64_t n_pairs,
+                                                         const int32_t* __restrict__ pair,
+                                                         const float* __restrict__ t, int64_t n,
+                                                         float* __restrict__ out) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int64_t pq = pair[q];
+    if (pq < 0 || pq >= n_pairs) {
+        out[q] = -INFINITY;
+        return;
+    }
+    float p[kNP];
+    const float* src = params + pq * kP;
+#pragma unroll
+    for (int c = 0; c < kNP; ++c) p[c] = src[c];
+    out[q] = -row_loss<false>(p, t[q], 0.0f, NTF_ROW_TIME, 0.0f, 1.0f, nullptr, 1.0f);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -572,6 +711,7 @@ __global__ void zero_kernel(float* p, int n) {
 // ---------------------------------------------------------------------------
 struct ntf_ctx {
     int device = 0;
+    int n_cu = 256;
     float* d_w = nullptr;
     float* d_m = nullptr;
     float* d_v = nullptr;
@@ -579,6 +719,8 @@ struct ntf_ctx {
     double* d_ploss = nullptr;
     int64_t partial_blocks = 0;
     int64_t step = 0;
+    double* d_chunk = nullptr;
+    int64_t chunk_cap = 0;
 };
 
 namespace {
@@ -603,13 +745,13 @@ bool rows_ok(const float* x, const float* t, const float* cnt, const uint8_t* fl
     return n == 0 || (x && t && cnt && flags);
 }
 
-int32_t run_grad(ntf_ctx* c, const float* x, const float* t, const float* cnt, const uint8_t* flags, int64_t n,
-                 float scale, hipStream_t s) {
+int32_t run_grad(ntf_ctx* c, const float* x, const float* t, const float* cnt, const uint8_t* flags,
+                 const float* aux, int64_t n, float scale, hipStream_t s) {
     const int64_t blocks = (n + kRows - 1) / kRows;
     const int32_t st = ensure_partials(c, blocks);
     if (st != NT_OK) return st;
     hipLaunchKernelGGL(flow_kernel<true>, dim3((unsigned)blocks), dim3(kThreads), 0, s, c->d_w,
-                       Rows{x, t, cnt, flags, n}, scale, (float*)nullptr, (float*)nullptr, c->d_partial,
+                       Rows{x, t, cnt, flags, aux, n}, scale, (float*)nullptr, (float*)nullptr, c->d_partial,
                        c->d_ploss);
     return hip_status(hipGetLastError());
 }
@@ -627,6 +769,9 @@ int32_t ntf_create(int32_t device, ntf_ctx** out) {
     ntf_ctx* c = new (std::nothrow) ntf_ctx();
     if (!c) return NT_ERR_OOM;
     c->device = device;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+        c->n_cu = cus;
     if (hipMalloc(&c->d_w, kNW * sizeof(float)) != hipSuccess || hipMalloc(&c->d_m, kNW * sizeof(float)) != hipSuccess ||
         hipMalloc(&c->d_v, kNW * sizeof(float)) != hipSuccess) {
         ntf_destroy(c);
@@ -652,6 +797,7 @@ void ntf_destroy(ntf_ctx* c) {
     if (c->d_v) (void)hipFree(c->d_v);
     if (c->d_partial) (void)hipFree(c->d_partial);
     if (c->d_ploss) (void)hipFree(c->d_ploss);
+    if (c->d_chunk) (void)hipFree(c->d_chunk);
     delete c;
 }
 
@@ -673,24 +819,25 @@ int32_t ntf_get_weights(ntf_ctx* c, float* weights) {
     return hip_status(hipMemcpy(weights, c->d_w, kNW * sizeof(float), hipMemcpyDeviceToHost));
 }
 
-int32_t ntf_forward(ntf_ctx* c, const float* x, const float* t, const float* cnt, const uint8_t* flags, int64_t n,
-                    float* d_params, float* d_row_loss, void* stream) {
+int32_t ntf_forward(ntf_ctx* c, const float* x, const float* t, const float* cnt, const uint8_t* flags,
+                    const float* aux, int64_t n, float* d_params, float* d_row_loss, void* stream) {
     if (!c || !rows_ok(x, t, cnt, flags, n)) return NT_ERR_ARG;
     if (n == 0) return NT_OK;
     if (hipSetDevice(c->device) != hipSuccess) return NT_ERR_DEVICE;
-    const int64_t blocks = (n + kRows - 1) / kRows;
+    const int64_t tiles = (n + kRows - 1) / kRows;
+    const int64_t blocks = tiles < c->n_cu ? tiles : c->n_cu;       // one resident workgroup per CU
     hipLaunchKernelGGL(flow_kernel<false>, dim3((unsigned)blocks), dim3(kThreads), 0, (hipStream_t)stream,
-                       c->d_w, Rows{x, t, cnt, flags, n}, 1.0f, d_params, d_row_loss, (float*)nullptr,
+                       c->d_w, Rows{x, t, cnt, flags, aux, n}, 1.0f, d_params, d_row_loss, (float*)nullptr,
                        (double*)nullptr);
     return hip_status(hipGetLastError());
 }
 
-int32_t ntf_loss_grad(ntf_ctx* c, const float* x, const float* t, const float* cnt, const uint8_t* flags, int64_t n,
-                      float scale, float* d_grad, double* d_loss, void* stream) {
+int32_t ntf_loss_grad(ntf_ctx* c, const float* x, const float* t, const float* cnt, const uint8_t* flags,
+                      const float* aux, int64_t n, float scale, float* d_grad, double* d_loss, void* stream) {
     if (!c || !d_grad || !rows_ok(x, t, cnt, flags, n) || n == 0) return NT_ERR_ARG;
     if (hipSetDevice(c->device) != hipSuccess) return NT_ERR_DEVICE;
     hipStream_t s = (hipStream_t)stream;
-    int32_t st = run_grad(c, x, t, cnt, flags, n, scale, s);
+    int32_t st = run_grad(c, x, t, cnt, flags, aux, n, scale, s);
     if (st != NT_OK) return st;
     const int64_t blocks = (n + kRows - 1) / kRows;
     hipLaunchKernelGGL(reduce_kernel, dim3((kNW + 255) / 256), dim3(256), 0, s, c->d_partial, (int)blocks,
@@ -698,12 +845,12 @@ int32_t ntf_loss_grad(ntf_ctx* c, const float* x, const float* t, const float* c
     return hip_status(hipGetLastError());
 }
 
-int32_t ntf_train_step(ntf_ctx* c, const float* x, const float* t, const float* cnt, const uint8_t* flags, int64_t n,
-                       float lr, double* d_loss, void* stream) {
+int32_t ntf_train_step(ntf_ctx* c, const float* x, const float* t, const float* cnt, const uint8_t* flags,
+                       const float* aux, int64_t n, float lr, double* d_loss, void* stream) {
     if (!c || !rows_ok(x, t, cnt, flags, n) || n == 0 || !(lr >= 0.0f)) return NT_ERR_ARG;
     if (hipSetDevice(c->device) != hipSuccess) return NT_ERR_DEVICE;
     hipStream_t s = (hipStream_t)stream;
-    int32_t st = run_grad(c, x, t, cnt, flags, n, 1.0f / (float)n, s);
+    int32_t st = run_grad(c, x, t, cnt, flags, aux, n, 1.0f / (float)n, s);
     if (st != NT_OK) return st;
     c->step += 1;
     const float c1 = (float)(1.0 - pow(0.9, (double)c->step));
@@ -711,6 +858,67 @@ int32_t ntf_train_step(ntf_ctx* c, const float* x, const float* t, const float* 
     const int64_t blocks = (n + kRows - 1) / kRows;
     hipLaunchKernelGGL(reduce_kernel, dim3((kNW + 255) / 256), dim3(256), 0, s, c->d_partial, (int)blocks,
                        c->d_ploss, 1.0 / (double)n, d_loss, (float*)nullptr, c->d_w, c->d_m, c->d_v, lr, c1, c2);
+    return hip_status(hipGetLastError());
+}
+
+int32_t ntf_pair_rows(ntf_ctx* c, const float* d_sources, int32_t n_sources, const float* d_modules,
+                      int32_t n_modules, const float* d_pmt_axes, int32_t n_pmts, const float* norm,
+                      const float* scale_knots, int32_t n_knots, float log_yield_1gev, float* d_x, float* d_aux,
+                      void* stream) {
+    if (!c || !d_sources || !d_modules || !d_pmt_axes || !norm || !scale_knots || !d_x || !d_aux) return NT_ERR_ARG;
+    if (n_sources < 0 || n_modules <= 0 || n_pmts <= 0 || n_knots < 2 || n_knots > NTF_MAX_SCALE_KNOTS)
+        return NT_ERR_ARG;
+    if (n_sources == 0) return NT_OK;
+    PairArgs a;
+    for (int j = 0; j < 5; ++j) {
+        if (!(norm[5 + j] > 0.0f)) return NT_ERR_ARG;
+        a.mean[j] = norm[j];
+        a.inv_std[j] = 1.0f / norm[5 + j];
+    }
+    for (int j = 0; j < n_knots; ++j) {
+        a.knot_d[j] = scale_knots[2 * j];
+        a.knot_g[j] = scale_knots[2 * j + 1];
+        if (j > 0 && !(a.knot_d[j] > a.knot_d[j - 1])) return NT_ERR_ARG;
+    }
+    a.n_knots = n_knots;
+    a.log_yield_1gev = log_yield_1gev;
+    if (hipSetDevice(c->device) != hipSuccess) return NT_ERR_DEVICE;
+    const int64_t n = (int64_t)n_sources * n_modules * n_pmts;
+    hipLaunchKernelGGL(pair_rows_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       d_sources, n_sources, d_modules, n_modules, d_pmt_axes, n_pmts, a, d_x, d_aux);
+    return hip_status(hipGetLastError());
+}
+
+int32_t ntf_segment_sum(ntf_ctx* c, const float* d_values, const uint8_t* d_skip, const int64_t* d_chunk_offsets,
+                        int64_t n_chunks, const int64_t* d_segment_chunks, int64_t n_segments, float sign,
+                        int32_t accumulate, double* d_out, void* stream) {
+    if (!c || n_segments < 0 || n_chunks < 0) return NT_ERR_ARG;
+    if (n_segments == 0) return NT_OK;
+    if (!d_values || !d_chunk_offsets || !d_segment_chunks || !d_out || n_chunks > 0x7fffffff) return NT_ERR_ARG;
+    if (hipSetDevice(c->device) != hipSuccess) return NT_ERR_DEVICE;
+    hipStream_t s = (hipStream_t)stream;
+    if (n_chunks > c->chunk_cap) {
+        if (c->d_chunk) (void)hipFree(c->d_chunk);
+        c->d_chunk = nullptr;
+        c->chunk_cap = 0;
+        if (hipMalloc(&c->d_chunk, (size_t)n_chunks * sizeof(double)) != hipSuccess) return NT_ERR_OOM;
+        c->chunk_cap = n_chunks;
+    }
+    if (n_chunks > 0)
+        hipLaunchKernelGGL(chunk_sum_kernel, dim3((unsigned)n_chunks), dim3(256), 0, s, d_values, d_skip,
+                           d_chunk_offsets, c->d_chunk);
+    hipLaunchKernelGGL(segment_sum_kernel, dim3((unsigned)((n_segments + 255) / 256)), dim3(256), 0, s, c->d_chunk,
+                       d_segment_chunks, n_segments, sign, (int)accumulate, d_out);
+    return hip_status(hipGetLastError());
+}
+
+int32_t ntf_time_logpdf(ntf_ctx* c, const float* d_params, int64_t n_pairs, const int32_t* d_pair,
+                        const float* d_t, int64_t n, float* d_logpdf, void* stream) {
+    if (!c || n < 0 || n_pairs < 0 || (n > 0 && (!d_params || !d_pair || !d_t || !d_logpdf))) return NT_ERR_ARG;
+    if (n == 0) return NT_OK;
+    if (hipSetDevice(c->device) != hipSuccess) return NT_ERR_DEVICE;
+    hipLaunchKernelGGL(time_logpdf_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       d_params, n_pairs, d_pair, d_t, n, d_logpdf);
     return hip_status(hipGetLastError());
 }
 

@@ -64,13 +64,14 @@ class FlowRows:
     t: "object"        # [n] float32
     count: "object"    # [n] float32
     flags: "object"    # [n] uint8
+    aux: "object" = None   # [n, 2] float32 (log λ offset, row weight) or None
 
     @property
     def n(self) -> int:
         return int(self.t.shape[0])
 
     @classmethod
-    def from_numpy(cls, x, t=None, count=None, flags=None, device="cuda:0"):
+    def from_numpy(cls, x, t=None, count=None, flags=None, aux=None, device="cuda:0"):
         import torch
         x = np.asarray(x, dtype=np.float32)
         n = x.shape[0]
@@ -82,15 +83,44 @@ class FlowRows:
         count = np.zeros(n, np.float32) if count is None else np.asarray(count, np.float32)
         flags = np.full(n, ROW_TIME | ROW_COUNT, np.uint8) if flags is None else np.asarray(flags, np.uint8)
         dev = torch.device(device)
+        a = None if aux is None else torch.from_numpy(np.ascontiguousarray(aux, np.float32).reshape(n, 2)).to(dev)
         return cls(torch.from_numpy(xp).to(dev), torch.from_numpy(t).to(dev), torch.from_numpy(count).to(dev),
-                   torch.from_numpy(flags).to(dev))
+                   torch.from_numpy(flags).to(dev), a)
 
     def slice(self, idx):
-        return FlowRows(self.x[idx], self.t[idx], self.count[idx], self.flags[idx])
+        return FlowRows(self.x[idx], self.t[idx], self.count[idx], self.flags[idx],
+                        None if self.aux is None else self.aux[idx])
 
     def args(self):
         return (C.c_void_p(self.x.data_ptr()), C.c_void_p(self.t.data_ptr()), C.c_void_p(self.count.data_ptr()),
-                C.c_void_p(self.flags.data_ptr()), C.c_int64(self.n))
+                C.c_void_p(self.flags.data_ptr()), None if self.aux is None else C.c_void_p(self.aux.data_ptr()),
+                C.c_int64(self.n))
+
+
+@dataclass
+class Segments:
+    """Contiguous segments of a row array split into chunks of bounded length
+    (device int64 tensors): chunk c = [chunk_offsets[c], chunk_offsets[c+1]),
+    segment s = chunks [segment_chunks[s], segment_chunks[s+1])."""
+    chunk_offsets: "object"
+    segment_chunks: "object"
+
+    @staticmethod
+    def tables(offsets, chunk: int = 8192):
+        """(chunk_offsets, segment_chunks) as numpy for segment offsets."""
+        offsets = np.asarray(offsets, np.int64)
+        n_chunks = (np.diff(offsets) + chunk - 1) // chunk
+        seg_chunks = np.concatenate([[0], np.cumsum(n_chunks)]).astype(np.int64)
+        within = np.arange(seg_chunks[-1]) - np.repeat(seg_chunks[:-1], n_chunks)
+        starts = np.repeat(offsets[:-1], n_chunks) + chunk * within
+        return np.concatenate([starts, offsets[-1:]]).astype(np.int64), seg_chunks
+
+    @classmethod
+    def from_offsets(cls, offsets, chunk: int = 8192, device="cuda:0") -> "Segments":
+        import torch
+        co, sc = cls.tables(offsets, chunk)
+        dev = torch.device(device)
+        return cls(torch.from_numpy(co).to(dev), torch.from_numpy(sc).to(dev))
 
 
 class FlowModel:
@@ -153,6 +183,46 @@ class FlowModel:
         _abi.check(self.lib.ntf_loss_grad(self.handle, *rows.args(), C.c_float(scale), C.c_void_p(g.data_ptr()),
                                           C.c_void_p(loss.data_ptr()), self._stream()), "ntf_loss_grad")
         return float(loss.cpu()[0]), g.cpu().numpy()
+
+    def pair_rows(self, sources, modules, axes, norm_mean, norm_std, scale_knots, log_yield_1gev: float):
+        """Rows (x [n, 16], aux [n, 2]) of every (source, module, PMT) pair on
+        the device; sources [n_src, 8], modules [n_mod, 3], axes [n_pmt, 3]
+        are device tensors."""
+        torch = self.torch
+        n = sources.shape[0] * modules.shape[0] * axes.shape[0]
+        x = torch.empty((n, N_IN), dtype=torch.float32, device=self.dev)
+        aux = torch.empty((n, 2), dtype=torch.float32, device=self.dev)
+        norm = np.ascontiguousarray(np.concatenate([norm_mean, norm_std]), np.float32)
+        knots = np.ascontiguousarray(scale_knots, np.float32)
+        _abi.check(self.lib.ntf_pair_rows(self.handle, C.c_void_p(sources.data_ptr()), sources.shape[0],
+                                          C.c_void_p(modules.data_ptr()), modules.shape[0],
+                                          C.c_void_p(axes.data_ptr()), axes.shape[0], norm.ctypes.data_as(C.c_void_p),
+                                          knots.ctypes.data_as(C.c_void_p), knots.shape[0], C.c_float(log_yield_1gev),
+                                          C.c_void_p(x.data_ptr()), C.c_void_p(aux.data_ptr()), self._stream()),
+                   "ntf_pair_rows")
+        return x, aux
+
+    def time_logpdf(self, params, pair, t):
+        """logpdf of times t under the outputs `params` [n_pairs, 80] of pair[i]."""
+        torch = self.torch
+        out = torch.empty(t.shape[0], dtype=torch.float32, device=self.dev)
+        _abi.check(self.lib.ntf_time_logpdf(self.handle, C.c_void_p(params.data_ptr()), C.c_int64(params.shape[0]),
+                                            C.c_void_p(pair.data_ptr()), C.c_void_p(t.data_ptr()),
+                                            C.c_int64(t.shape[0]), C.c_void_p(out.data_ptr()), self._stream()),
+                   "ntf_time_logpdf")
+        return out
+
+    def segment_sum(self, values, segments: "Segments", out, sign: float = 1.0, skip=None,
+                    accumulate: bool = False):
+        """out[s] (=/+=) sign·Σ of segment s of `values` (skip != 0 left out)."""
+        _abi.check(self.lib.ntf_segment_sum(self.handle, C.c_void_p(values.data_ptr()),
+                                            None if skip is None else C.c_void_p(skip.data_ptr()),
+                                            C.c_void_p(segments.chunk_offsets.data_ptr()),
+                                            C.c_int64(segments.chunk_offsets.shape[0] - 1),
+                                            C.c_void_p(segments.segment_chunks.data_ptr()), C.c_int64(out.shape[0]),
+                                            C.c_float(sign), int(accumulate), C.c_void_p(out.data_ptr()),
+                                            self._stream()), "ntf_segment_sum")
+        return out
 
     def train_step(self, rows: FlowRows, lr: float, loss_out=None):
         """One Adam step on the mean loss of `rows`; `loss_out` (device float64

@@ -209,26 +209,26 @@ def _lgamma(n):
     return np.vectorize(math.lgamma, otypes=[float])(np.asarray(n, dtype=float) + 1.0)
 
 
-def row_loss(p, t, n, flags, n_bins, n_layers):
+def row_loss(p, t, n, flags, n_bins, n_layers, offset=0.0, weight=1.0):
     """Per-row negative log-likelihood: −[time·logpdf(t) + count·(n log λ − λ −
     lgamma(n+1))] (extended_cascade_model.jl:132-139 with the terms switched per
-    row by `flags`)."""
+    row by `flags`), with log λ = p_λ + offset, times the row weight."""
     lp = flow_logpdf(t, p, n_bins, n_layers)
-    loglam = p[..., -1]
+    loglam = p[..., -1] + offset
     pois = n * loglam - np.exp(loglam) - _lgamma(n)
     ft = (flags & ROW_TIME) != 0
     fc = (flags & ROW_COUNT) != 0
-    return -(np.where(ft, lp, 0.0 * lp) + np.where(fc, pois, 0.0 * pois))
+    return -weight * (np.where(ft, lp, 0.0 * lp) + np.where(fc, pois, 0.0 * pois))
 
 
-def row_loss_grad(p, t, n, flags, n_bins, n_layers, h=1e-30):
+def row_loss_grad(p, t, n, flags, n_bins, n_layers, offset=0.0, weight=1.0, h=1e-30):
     """d row_loss / d p by complex step (exact to FP64 rounding)."""
     p = np.asarray(p, dtype=np.float64)
     g = np.empty_like(p)
     for k in range(p.shape[-1]):
         pc = p.astype(np.complex128)
         pc[..., k] += 1j * h
-        g[..., k] = row_loss(pc, t, n, flags, n_bins, n_layers).imag / h
+        g[..., k] = row_loss(pc, t, n, flags, n_bins, n_layers, offset, weight).imag / h
     return g
 
 
@@ -260,14 +260,14 @@ def mlp_forward(x, w, use_bf16=True):
     return {"xq": xq, "h1": h1, "h1q": h1q, "h2": h2, "h2q": h2q, "p": p}
 
 
-def loss_and_grad(x, t, n, flags, w, n_bins, n_layers, use_bf16=True):
+def loss_and_grad(x, t, n, flags, w, n_bins, n_layers, use_bf16=True, offset=0.0, weight=1.0):
     """Mean row loss over the batch and its gradient for every weight (FP64
     back-propagation through the MLP, complex-step gradient of the flow)."""
     f = mlp_forward(x, w, use_bf16)
     p = f["p"]
     B = x.shape[0]
-    loss = row_loss(p, t, n, flags, n_bins, n_layers)
-    dp = row_loss_grad(p, t, n, flags, n_bins, n_layers) / B
+    loss = row_loss(p, t, n, flags, n_bins, n_layers, offset, weight)
+    dp = row_loss_grad(p, t, n, flags, n_bins, n_layers, offset, weight) / B
     dpq = _r(dp, use_bf16)
     g = {"W3": f["h2q"].T @ dpq, "b3": dp.sum(axis=0)}
     dh2 = (dpq @ _r(w["W3"], use_bf16).T) * (f["h2"] > 0)

@@ -96,6 +96,32 @@ def test_row_gradient_matches_complex_step(flow):
     assert np.quantile(rel, 0.95) < 1e-3
 
 
+def test_offset_and_weight_rows(flow):
+    """Rows with (log λ offset, weight): loss and per-row gradient against the
+    oracle with the same offset and weight."""
+    from neutrinotelescopes_amd.flow import FlowRows
+    flat = _weights(9)
+    flow.set_weights(flat)
+    x, t, cnt, flags = _rows(48, 10)
+    rng = np.random.default_rng(11)
+    off = rng.uniform(-8, 12, 48).astype(np.float32)
+    wgt = np.exp(-off).astype(np.float32)
+    cnt = rng.poisson(np.exp(off + 0.3)).astype(np.float32)
+    rows = FlowRows.from_numpy(x, t, cnt, flags, np.column_stack([off, wgt]))
+    p, loss = flow.forward(rows)
+    p = p.cpu().numpy().astype(np.float64)[:, :78]
+    args = (t.astype(np.float64), cnt.astype(np.float64), flags, K, L, off.astype(np.float64),
+            wgt.astype(np.float64))
+    with np.errstate(all="ignore"):
+        lref = rq.row_loss(p, *args)
+        gref = rq.row_loss_grad(p, *args)
+    np.testing.assert_allclose(loss.cpu().numpy(), lref, rtol=1e-4, atol=1e-4)
+    for r in range(0, 48, 3):
+        _, g = flow.loss_grad(rows.slice(slice(r, r + 1)), 1.0)
+        gb3 = g[-80:][:78].astype(np.float64)
+        assert np.linalg.norm(gb3 - gref[r]) <= 1e-3 * max(np.linalg.norm(gref[r]), 1e-3), r
+
+
 def test_weight_gradient_matches_oracle(flow):
     from neutrinotelescopes_amd import flow as F
     from neutrinotelescopes_amd.flow import FlowRows
@@ -160,3 +186,42 @@ def test_training_learns_synthetic_density_and_rate(flow):
     assert loss1 < true_nll.mean() + 0.1
     loglam = p.cpu().numpy()[:, 77]
     assert np.median(np.abs(loglam - np.log(lam))) < 0.1
+
+
+def test_pair_rows_and_time_logpdf(flow):
+    """Device pair rows against the host features (surrogate.pmt_features) and
+    the per-hit time logpdf against the forward pass's time-row loss."""
+    import torch
+    from neutrinotelescopes_amd import surrogate as S
+    from neutrinotelescopes_amd.flow import FlowRows
+    flow.set_weights(_weights(12))
+    rng = np.random.default_rng(13)
+    casc = S.random_cascades(7, rng)
+    modules = S.array_detector()[::9]
+    axes = S.pom_axes()
+    f = S.pair_features(casc, modules).reshape(-1, 5)
+    norm = S.FeatureNorm(f.mean(0), f.std(0) + 0.1)
+    scale = S.CountScale(np.linspace(0, 6, 9), rng.normal(-20, 3, 9))
+    src = np.zeros((7, 8), np.float32)
+    src[:, :3], src[:, 3:6], src[:, 6] = casc.pos, casc.dir, np.log10(casc.energy)
+    dev = flow.dev
+    x, aux = flow.pair_rows(torch.from_numpy(src).to(dev), torch.from_numpy(modules.astype(np.float32)).to(dev),
+                            torch.from_numpy(axes.astype(np.float32)).to(dev), norm.mean, norm.std,
+                            np.column_stack([scale.log_d, scale.log_g]), float(S.log_yield(0.0)))
+    x = x.cpu().numpy()
+    np.testing.assert_allclose(x[:, :5], norm(f), atol=2e-4)
+    assert np.all(x[:, 5:] == 0)
+    a = aux.cpu().numpy()
+    np.testing.assert_allclose(a[:, 0], scale(f), rtol=1e-5, atol=1e-3)
+    assert np.all(a[:, 1] == 1.0)
+    # time logpdf from the pair outputs == −loss of time rows with the same features
+    n_pairs = x.shape[0]
+    p, _ = flow.forward(FlowRows.from_numpy(x, flags=np.full(n_pairs, 2, np.uint8)))
+    hp = rng.integers(0, n_pairs, 3000).astype(np.int32)
+    ht = rng.uniform(-30, 400, 3000).astype(np.float32)
+    lp = flow.time_logpdf(p, torch.from_numpy(hp).to(dev), torch.from_numpy(ht).to(dev)).cpu().numpy()
+    _, l = flow.forward(FlowRows.from_numpy(x[hp], ht, None, np.full(3000, 1, np.uint8)), want_params=False)
+    np.testing.assert_allclose(lp, -l.cpu().numpy(), rtol=1e-6, atol=1e-6)
+    bad = flow.time_logpdf(p, torch.tensor([-1, n_pairs], dtype=torch.int32, device=dev),
+                           torch.zeros(2, device=dev)).cpu().numpy()
+    assert np.all(np.isneginf(bad))

@@ -1,0 +1,112 @@
+"""End-to-end flow surrogate on the GPU: tables from the engine, training by
+the project's kernels, accuracy against direct propagation (and what the
+trained network gains over the untrained one), and the device likelihood
+path against a host evaluation of the same terms."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def trained(engine):
+    from neutrinotelescopes_amd import make_cascadia_medium_properties, CherenkovSpectrum
+    from neutrinotelescopes_amd import surrogate as S
+    from neutrinotelescopes_amd.propagation import PropagationEngine
+    med = make_cascadia_medium_properties(0.95)
+    spec = CherenkovSpectrum(S.WL_RANGE, 30, med)
+    eng = PropagationEngine(0)
+    tables = S.generate_tables(eng, med, spec, n_runs=32, photons_per_run=10 ** 8, seed=3)
+    sur, hist = S.train_surrogate(tables, steps=8000, lr=2e-3, seed=3)
+    yield eng, med, spec, sur, hist, tables
+    sur.model.close()
+    eng.close()
+
+
+def test_tables_and_training(trained):
+    _, _, _, _, hist, tables = trained
+    assert len(tables.t) > 10 ** 5
+    assert set(np.unique(tables.flags)) == {1, 2}
+    assert np.all(np.isfinite(tables.features)) and np.all(tables.count >= 0)
+    print("holdout loss", hist["initial_holdout"], "->", hist["final_holdout"])
+    assert hist["final_holdout"] < hist["initial_holdout"] - 1.0
+
+
+def test_trained_network_beats_untrained_against_direct(trained):
+    from neutrinotelescopes_amd import surrogate as S
+    from neutrinotelescopes_amd.flow import FlowModel
+    eng, med, spec, sur, _, _ = trained
+    rng = np.random.default_rng(21)
+    modules = S.array_detector()
+    casc = S.random_cascades(24, rng)
+    lam_dir, hits, _ = S.direct_propagation(eng, med, spec, casc, modules, 2 * 10 ** 8, seed=22)
+    nh = np.zeros((24, len(modules)))
+    np.add.at(nh, (hits[:, 0].astype(int), hits[:, 1].astype(int)), 1)
+    dist = np.linalg.norm(modules[None] - casc.pos[:, None], axis=-1)
+    nh = np.where(dist >= S.FALLBACK_DISTANCE, nh, 0)
+    err = S.module_errors(S.surrogate_expectations(sur, casc, modules), lam_dir, nh)
+    untrained = S.Surrogate(FlowModel(0, seed=77), sur.norm, sur.scale)
+    err0 = S.module_errors(S.surrogate_expectations(untrained, casc, modules), lam_dir, nh)
+    sel = np.argwhere(nh >= 400)[:24]
+    bins = np.arange(-20.0, 1000.0, 4.0)
+    lam_sur = S.surrogate_expectations(sur, casc, modules)
+    kl = S.time_kl(sur, casc, modules, lam_sur, hits, [tuple(p) for p in sel], bins)
+    kl0 = S.time_kl(untrained, casc, modules, lam_sur, hits, [tuple(p) for p in sel], bins)
+    untrained.model.close()
+    print("pairs", err["pairs"], "count error median", err["median_abs_rel_err"], "untrained",
+          err0["median_abs_rel_err"], "KL", np.median(kl), "untrained", np.median(kl0))
+    assert err["pairs"] >= 10
+    assert err["median_abs_rel_err"] < 0.3
+    assert err["median_abs_rel_err"] < err0["median_abs_rel_err"] / 2
+    assert np.median(kl) < np.median(kl0) / 4
+
+
+def test_device_likelihood_matches_host_terms(trained):
+    """surrogate_llh (device rows, one forward over pairs, per-hit logpdf
+    from the pair outputs) against the same terms built on the host from
+    pmt_features and evaluated row by row."""
+    from neutrinotelescopes_amd import surrogate as S
+    eng, med, spec, sur, _, _ = trained
+    rng = np.random.default_rng(31)
+    modules = S.array_detector()
+    casc = S.random_cascades(5, rng)
+    lam, hits, _ = S.direct_propagation(eng, med, spec, casc, modules, 10 ** 7, seed=32)
+    obs = S.observe(lam, hits, np.random.default_rng(33))
+    ev = S.to_device(casc, modules, obs, str(sur.model.dev))
+    llh = S.surrogate_llh(sur, ev).cpu().numpy()
+    f = S.pair_features(casc, modules).reshape(-1, 5)
+    near = np.repeat(ev.near.reshape(-1), 16)
+    _, lc = sur.model.forward(sur.rows(f, None, obs.counts.reshape(-1), np.full(len(f), 2, np.uint8)),
+                              want_params=False)
+    hp = ((obs.hits[:, 0].astype(int) * len(modules) + obs.hits[:, 1].astype(int)) * 16 + obs.hits[:, 2].astype(int))
+    _, lt = sur.model.forward(sur.rows(f[hp], obs.hits[:, 3], None, np.full(len(hp), 1, np.uint8)),
+                              want_params=False)
+    host = np.zeros(5)
+    np.add.at(host, np.repeat(np.arange(5), len(modules) * 16), np.where(near, 0, -lc.double().cpu().numpy()))
+    np.add.at(host, obs.hits[:, 0].astype(int), np.where(near[hp], 0, -lt.double().cpu().numpy()))
+    print("llh", llh, "host", host)
+    # pair features differ between the FP32 device and FP64 host paths only by rounding
+    np.testing.assert_allclose(llh, host, rtol=2e-4, atol=1.0)
+
+
+def test_fallback_near_modules(trained):
+    """A cascade 3 m from a module: that pair takes the direct-propagation
+    terms and the event likelihood stays finite."""
+    from neutrinotelescopes_amd import surrogate as S
+    eng, med, spec, sur, _, _ = trained
+    modules = S.array_detector()
+    rng = np.random.default_rng(41)
+    casc = S.random_cascades(3, rng)
+    casc.pos[0] = modules[57] + np.array([3.0, 0.0, 0.0])
+    casc.energy[0] = 2e4
+    lam, hits, _ = S.direct_propagation(eng, med, spec, casc, modules, 10 ** 7, seed=42)
+    obs = S.observe(lam, hits, np.random.default_rng(43))
+    ev = S.to_device(casc, modules, obs, str(sur.model.dev))
+    assert ev.near[0, 57] and ev.near.sum() >= 1
+    fb = S.fallback_terms(eng, med, spec, casc, modules, obs.counts, ev.near, ev.near_hits, photons=10 ** 7,
+                          seed=44)
+    assert fb[0] != 0 and np.all(fb[1:] == 0) and np.isfinite(fb[0])
+    llh = S.surrogate_llh(sur, ev, fb).cpu().numpy()
+    assert np.all(np.isfinite(llh))
