@@ -145,6 +145,25 @@ def serving_benchmark(duration_s: float = 8.0, seed: int = 11):
             "rounds": st["rounds"], "device_busy_fraction": st["busy_ms"] / 1e3 / res["wall_s"]}
 
 
+def surrogate_benchmark():
+    """Flow surrogate: 128 engine photon-table launches (cascades log10 E
+    uniform 2-6, 2e8 photons each, 192 modules at 1-300 m), 24000 Adam steps
+    of batch 16384 on the bf16-MFMA training kernels, per-module expected-count
+    error and time-density KL against direct propagation (64 cascades, 2e8
+    photons each, module pairs with >= 400 simulated hits), and event
+    likelihood throughput for 1024 cascades (log-uniform 1e3-1e6 GeV) on the
+    10-string x 20-module x 16-PMT array against direct propagation of their
+    full Cherenkov light yield."""
+    from neutrinotelescopes_amd.surrogate import surrogate_study
+    r = surrogate_study()
+    keep = ("config", "table_photons", "table_seconds", "train_steps", "train_step_ms", "holdout_loss_initial",
+            "holdout_loss_final", "error_pairs", "count_err_median", "count_err_mean", "count_bias",
+            "count_err_median_untrained", "time_kl_median", "time_kl_median_untrained", "events",
+            "llh_seconds", "llh_flow_seconds", "llh_fallback_seconds", "llh_pairs", "llh_hits", "fallback_pairs",
+            "surrogate_events_per_s", "direct_photons_per_s", "direct_seconds", "direct_events_per_s", "speedup")
+    return {k: r[k] for k in keep if k in r}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +174,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving (p50/p99) measurement")
     ap.add_argument("--serving-only", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="skip the flow-surrogate study (tables, training, error, likelihood speed-up)")
+    ap.add_argument("--surrogate-only", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device-to-device copy 5 times (reference for "
                          "FETCH_SIZE/WRITE_SIZE counter scaling)")
@@ -169,6 +191,9 @@ def main():
     import torch
     if args.serving_only:
         print(json.dumps({"serving": serving_benchmark()}), flush=True)
+        return
+    if args.surrogate_only:
+        print(json.dumps({"surrogate": surrogate_benchmark()}), flush=True)
         return
     if args.copy_calibration:
         a = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
@@ -265,6 +290,8 @@ def main():
     eng.close()
     if not args.no_serving and args.workload == "cascade":
         out["serving"] = serving_benchmark()
+    if not args.no_surrogate and args.workload == "cascade":
+        out["surrogate"] = surrogate_benchmark()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload)
         out["gpu_vs_cpu"] = value / out["cpu_baseline"]["value"]

@@ -208,15 +208,17 @@ class TableSet:
         return np.column_stack([off, np.where(self.flags & 2, np.exp(-off), 1.0)]).astype(np.float32)
 
 
-def _group_rank(keys: np.ndarray, rng) -> np.ndarray:
-    """Random rank of each element inside its key group."""
-    order = np.lexsort((rng.random(keys.size), keys))
-    sk = keys[order]
-    start = np.r_[0, np.flatnonzero(np.diff(sk)) + 1]
-    ranks_sorted = np.arange(sk.size) - np.repeat(start, np.diff(np.r_[start, sk.size]))
-    out = np.empty(keys.size, np.int64)
-    out[order] = ranks_sorted
-    return out
+def thin_hits(hits_mod: np.ndarray, n_mod: int, max_hits: int, rng, pos, tres, w):
+    """About `max_hits` hits per module: each hit of a module with n > max_hits
+    hits is kept with probability max_hits/n and its weight divided by that
+    probability (unbiased for every weighted sum)."""
+    counts = np.bincount(hits_mod, minlength=n_mod)
+    if counts.max(initial=0) <= max_hits:
+        return pos, tres, w, hits_mod
+    p = np.minimum(1.0, max_hits / np.maximum(counts, 1))
+    keep = rng.random(hits_mod.size) < p[hits_mod]
+    m = hits_mod[keep]
+    return pos[keep], tres[keep], np.asarray(w[keep], np.float64) / p[m], m
 
 
 def rows_from_hits(hits_pos, hits_tres, hits_w, hits_mod, mod_pos, src_pos, src_dir, log10e, scale, rng,
@@ -232,14 +234,8 @@ def rows_from_hits(hits_pos, hits_tres, hits_w, hits_mod, mod_pos, src_pos, src_
     axes0 = pom_axes()
     cosop = cos_opening()
     hits_mod = np.asarray(hits_mod, np.int64)
-    w = np.asarray(hits_w, np.float64).copy()
-    # at most max_hits hits per module, reweighted (unbiased)
-    counts = np.bincount(hits_mod, minlength=n_mod)
-    if counts.max(initial=0) > max_hits:
-        keep = _group_rank(hits_mod, rng) < max_hits
-        factor = np.maximum(counts / max_hits, 1.0)
-        w = w * factor[hits_mod]
-        hits_pos, hits_tres, w, hits_mod = hits_pos[keep], hits_tres[keep], w[keep], hits_mod[keep]
+    w = np.asarray(hits_w, np.float64)
+    hits_pos, hits_tres, w, hits_mod = thin_hits(hits_mod, n_mod, max_hits, rng, hits_pos, hits_tres, w)
     rel = hits_pos - mod_pos[hits_mod]
     rel /= np.linalg.norm(rel, axis=1, keepdims=True)
     rots = random_rotations(n_mod * orientations, rng).reshape(n_mod, orientations, 3, 3)
@@ -314,7 +310,7 @@ def generate_tables(engine, medium, spectrum, n_runs: int, photons_per_run: int,
     engine.set_medium(medium)
     engine.set_spectrum(spectrum)
     engine.set_targets(make_targets(positions))
-    out = None
+    parts = []
     t0 = time.perf_counter()
     t_prop = 0.0
     photons_total = 0
@@ -332,12 +328,14 @@ def generate_tables(engine, medium, spectrum, n_runs: int, photons_per_run: int,
         photons_total += hits.n_simulated
         scale = cascade_photons(e, medium) / hits.n_simulated
         mod = hits.module_id.astype(np.int64) - 1
-        ts = rows_from_hits(hits.position.astype(np.float64), hits.tres.astype(np.float64),
-                            hits.total_weight.astype(np.float64), mod, positions, np.zeros(3), u, le, scale, rng,
-                            orientations, time_rows, max_hits)
-        out = ts if out is None else out.concat(ts)
+        pos, tres, w, mod_k = thin_hits(mod, len(positions), max_hits, rng, hits.position, hits.tres,
+                                        hits.total_weight)
+        ts = rows_from_hits(pos.astype(np.float64), tres.astype(np.float64), w, mod_k, positions, np.zeros(3), u, le,
+                            scale, rng, orientations, time_rows, max_hits)
+        parts.append(ts)
         if export is not None:
             export.extend(_reference_layout(hits, mod, positions, u, e, scale))
+    out = TableSet(*(np.concatenate([getattr(p, k) for p in parts]) for k in ("features", "t", "count", "flags")))
     out.info = {"runs": n_runs, "photons_per_run": photons_per_run, "photons": photons_total,
                 "seconds": time.perf_counter() - t0, "propagate_seconds": t_prop,
                 "modules_per_run": len(positions)}
