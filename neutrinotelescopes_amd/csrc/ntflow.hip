@@ -539,8 +539,41 @@ __global__ __launch_bounds__(kThreads, 1) void flow_kernel(const float* __restri
     }  // tiles
 }
 
-// Sum of the partial records (fixed order) → gradient, or Adam update.
-__global__ __launch_bounds__(256) void reduce_kernel(const float* __restrict__ partial, int nblocks,
+constexpr int kSplits = 16;           // record ranges summed in parallel by the first reduction pass
+
+// Source offset of weight q in a partial record and its number of sub-entries
+// (biases: one per wavefront).
+__device__ __forceinline__ int record_source(int q, int& nsub) {
+    nsub = 1;
+    if (q < ob1) return pW1 + (q - oW1);
+    if (q < oW2) { nsub = 4; return pBias + (q - ob1); }
+    if (q < ob2) return pW2 + (q - oW2);
+    if (q < oW3) { nsub = 4; return pBias + kH + (q - ob2); }
+    if (q < ob3) return pW3 + (q - oW3);
+    nsub = 4;
+    return pBias + 2 * kH + (q - ob3);
+}
+
+// First pass: tmp[split][q] = Σ over records [split·nb/S, (split+1)·nb/S) in order.
+__global__ __launch_bounds__(256) void reduce_partials_kernel(const float* __restrict__ partial, int nblocks,
+                                                             float* __restrict__ tmp) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int split = blockIdx.y;
+    if (q >= kNW) return;
+    int nsub;
+    const int src = record_source(q, nsub);
+    const int b0 = (int)((int64_t)nblocks * split / kSplits), b1 = (int)((int64_t)nblocks * (split + 1) / kSplits);
+    float acc = 0.0f;
+    for (int b = b0; b < b1; ++b) {
+        const float* rec = partial + (int64_t)b * kPartW + src;
+        for (int s = 0; s < nsub; ++s) acc += rec[s * kBiasW];
+    }
+    tmp[(int64_t)split * kNW + q] = acc;
+}
+
+// Second pass: gradient = Σ_split tmp (fixed order), then either the gradient
+// or the Adam update; thread 0 also sums the per-block losses.
+__global__ __launch_bounds__(256) void reduce_kernel(const float* __restrict__ tmp, int nblocks,
                                                     const double* __restrict__ partial_loss, double loss_scale,
                                                     double* __restrict__ loss_out, float* __restrict__ grad_out,
                                                     float* __restrict__ w, float* __restrict__ m,
@@ -552,18 +585,9 @@ __global__ __launch_bounds__(256) void reduce_kernel(const float* __restrict__ p
         *loss_out = s * loss_scale;
     }
     if (q >= kNW) return;
-    int src, nsub = 1;
-    if (q < ob1) src = pW1 + (q - oW1);
-    else if (q < oW2) { src = pBias + (q - ob1); nsub = 4; }
-    else if (q < ob2) src = pW2 + (q - oW2);
-    else if (q < oW3) { src = pBias + kH + (q - ob2); nsub = 4; }
-    else if (q < ob3) src = pW3 + (q - oW3);
-    else { src = pBias + 2 * kH + (q - ob3); nsub = 4; }
     float gsum = 0.0f;
-    for (int b = 0; b < nblocks; ++b) {
-        const float* rec = partial + (int64_t)b * kPartW;
-        for (int s = 0; s < nsub; ++s) gsum += rec[src + s * kBiasW];
-    }
+#pragma unroll
+    for (int sp = 0; sp < kSplits; ++sp) gsum += tmp[(int64_t)sp * kNW + q];
     if (grad_out) {
         grad_out[q] = gsum;
         return;
@@ -721,6 +745,7 @@ struct ntf_ctx {
     int64_t step = 0;
     double* d_chunk = nullptr;
     int64_t chunk_cap = 0;
+    float* d_tmp = nullptr;              // kSplits × NTF_N_WEIGHTS first-pass sums
 };
 
 namespace {
@@ -773,7 +798,8 @@ int32_t ntf_create(int32_t device, ntf_ctx** out) {
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
         c->n_cu = cus;
     if (hipMalloc(&c->d_w, kNW * sizeof(float)) != hipSuccess || hipMalloc(&c->d_m, kNW * sizeof(float)) != hipSuccess ||
-        hipMalloc(&c->d_v, kNW * sizeof(float)) != hipSuccess) {
+        hipMalloc(&c->d_v, kNW * sizeof(float)) != hipSuccess ||
+        hipMalloc(&c->d_tmp, (size_t)kSplits * kNW * sizeof(float)) != hipSuccess) {
         ntf_destroy(c);
         return NT_ERR_OOM;
     }
@@ -798,6 +824,7 @@ void ntf_destroy(ntf_ctx* c) {
     if (c->d_partial) (void)hipFree(c->d_partial);
     if (c->d_ploss) (void)hipFree(c->d_ploss);
     if (c->d_chunk) (void)hipFree(c->d_chunk);
+    if (c->d_tmp) (void)hipFree(c->d_tmp);
     delete c;
 }
 
@@ -840,7 +867,9 @@ int32_t ntf_loss_grad(ntf_ctx* c, const float* x, const float* t, const float* c
     int32_t st = run_grad(c, x, t, cnt, flags, aux, n, scale, s);
     if (st != NT_OK) return st;
     const int64_t blocks = (n + kRows - 1) / kRows;
-    hipLaunchKernelGGL(reduce_kernel, dim3((kNW + 255) / 256), dim3(256), 0, s, c->d_partial, (int)blocks,
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3((kNW + 255) / 256, kSplits), dim3(256), 0, s, c->d_partial,
+                       (int)blocks, c->d_tmp);
+    hipLaunchKernelGGL(reduce_kernel, dim3((kNW + 255) / 256), dim3(256), 0, s, c->d_tmp, (int)blocks,
                        c->d_ploss, 1.0, d_loss, d_grad, c->d_w, c->d_m, c->d_v, 0.0f, 1.0f, 1.0f);
     return hip_status(hipGetLastError());
 }
@@ -856,7 +885,9 @@ int32_t ntf_train_step(ntf_ctx* c, const float* x, const float* t, const float* 
     const float c1 = (float)(1.0 - pow(0.9, (double)c->step));
     const float c2 = (float)(1.0 - pow(0.999, (double)c->step));
     const int64_t blocks = (n + kRows - 1) / kRows;
-    hipLaunchKernelGGL(reduce_kernel, dim3((kNW + 255) / 256), dim3(256), 0, s, c->d_partial, (int)blocks,
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3((kNW + 255) / 256, kSplits), dim3(256), 0, s, c->d_partial,
+                       (int)blocks, c->d_tmp);
+    hipLaunchKernelGGL(reduce_kernel, dim3((kNW + 255) / 256), dim3(256), 0, s, c->d_tmp, (int)blocks,
                        c->d_ploss, 1.0 / (double)n, d_loss, (float*)nullptr, c->d_w, c->d_m, c->d_v, lr, c1, c2);
     return hip_status(hipGetLastError());
 }
