@@ -149,8 +149,9 @@ def surrogate_benchmark():
     """Flow surrogate: 128 engine photon-table launches (cascades log10 E
     uniform 2-6, 2e8 photons each, 192 modules at 1-300 m), 24000 Adam steps
     of batch 16384 on the bf16-MFMA training kernels, per-module expected-count
-    error and time-density KL against direct propagation (64 cascades, 2e8
-    photons each, module pairs with >= 400 simulated hits), and event
+    error and time-density KL against direct propagation (64 cascades; pairs
+    selected with >= 400 hits in one 2e8-photon run, truth from an independent
+    6e8-photon run), and event
     likelihood throughput for 1024 cascades (log-uniform 1e3-1e6 GeV) on the
     10-string x 20-module x 16-PMT array against direct propagation of their
     full Cherenkov light yield."""
@@ -158,7 +159,7 @@ def surrogate_benchmark():
     r = surrogate_study()
     keep = ("config", "table_photons", "table_seconds", "train_steps", "train_step_ms", "holdout_loss_initial",
             "holdout_loss_final", "error_pairs", "count_err_median", "count_err_mean", "count_bias",
-            "count_err_median_untrained", "time_kl_median", "time_kl_median_untrained", "events",
+            "count_err_median_untrained", "truth_rel_stat_err_median", "time_kl_median", "time_kl_median_untrained", "events",
             "llh_seconds", "llh_flow_seconds", "llh_fallback_seconds", "llh_pairs", "llh_hits", "fallback_pairs",
             "surrogate_events_per_s", "direct_photons_per_s", "direct_seconds", "direct_events_per_s", "speedup")
     return {k: r[k] for k in keep if k in r}

@@ -814,13 +814,19 @@ def surrogate_study(n_runs: int = 128, photons_per_run: int = 2 * 10 ** 8, train
     # --- error against high-statistics direct propagation
     ne = min(n_error_events, n_events)
     ev = Cascades(casc.pos[:ne], casc.dir[:ne], casc.energy[:ne])
-    lam_dir, hits_dir, dinfo = direct_propagation(eng, med, spec, ev, modules, error_photons, seed + 2)
+    # selection on one direct run, truth from an independent one (no selection bias)
+    _, hits_sel, _ = direct_propagation(eng, med, spec, ev, modules, error_photons, seed + 12)
+    lam_dir, hits_dir, dinfo = direct_propagation(eng, med, spec, ev, modules, 3 * error_photons, seed + 2)
     nh = np.zeros((ne, len(modules)))
-    np.add.at(nh, (hits_dir[:, 0].astype(int), hits_dir[:, 1].astype(int)), 1)
+    np.add.at(nh, (hits_sel[:, 0].astype(int), hits_sel[:, 1].astype(int)), 1)
+    nh_truth = np.zeros((ne, len(modules)))
+    np.add.at(nh_truth, (hits_dir[:, 0].astype(int), hits_dir[:, 1].astype(int)), 1)
     dist = np.linalg.norm(modules[None] - ev.pos[:, None], axis=-1)
     nh_far = np.where(dist >= FALLBACK_DISTANCE, nh, 0)
     lam_sur = surrogate_expectations(sur, ev, modules)
     err = module_errors(lam_sur, lam_dir, nh_far)
+    sel_pairs = nh_far >= 400
+    truth_stat = float(np.median(1.0 / np.sqrt(np.maximum(nh_truth[sel_pairs], 1)))) if sel_pairs.any() else None
     untrained = Surrogate(FlowModel(device, seed=seed + 99, n_in=N_FEATURES), sur.norm, sur.scale)
     err0 = module_errors(surrogate_expectations(untrained, ev, modules), lam_dir, nh_far)
     sel = np.argwhere(nh_far >= 400)
@@ -869,6 +875,7 @@ def surrogate_study(n_runs: int = 128, photons_per_run: int = 2 * 10 ** 8, train
         "error_pairs": err["pairs"], "count_err_median": err["median_abs_rel_err"],
         "count_err_mean": err["mean_abs_rel_err"], "count_bias": err["bias"],
         "count_err_median_untrained": err0["median_abs_rel_err"],
+        "truth_rel_stat_err_median": truth_stat,
         "time_kl_median": float(np.median(kl)) if kl.size else None,
         "time_kl_median_untrained": float(np.median(kl0)) if kl0.size else None,
         "events": n_events, "llh_seconds": linfo["seconds"], "llh_flow_seconds": linfo["flow_seconds"],
