@@ -309,3 +309,39 @@ def make_hits_from_photons(hits: PhotonHits, setup: PhotonPropSetup, orientation
         pmt[sel] = check_pmt_hit(hits.position[sel], tmap[int(mid)], orientation)
     hits.extra["pmt_id"] = pmt
     return hits.select(pmt > 0)
+
+
+def resample_simulation(times, module_id, total_weight, pmt_id=None, downsample: float = 1.0,
+                        rng: np.random.Generator | None = None):
+    """Weighted photons → one realisation of detected hits, pipeline.jl:86-116:
+    per (module, PMT) group (per module when `pmt_id` is None) draw
+    n = min(Poisson(Σw·downsample), #photons) and sample n of the group's
+    times without replacement with probabilities ∝ weight (photons with
+    weight 0 never).  Returns (times, module_id, pmt_id or None)."""
+    rng = rng or np.random.default_rng()
+    times = np.asarray(times)
+    module_id = np.asarray(module_id)
+    w = np.asarray(total_weight, dtype=np.float64)
+    pmt = None if pmt_id is None else np.asarray(pmt_id)
+    key = module_id.astype(np.int64) * 65536 + (0 if pmt is None else pmt.astype(np.int64))
+    order = np.argsort(key, kind="stable")
+    ks = key[order]
+    starts = np.r_[0, np.flatnonzero(np.diff(ks)) + 1] if ks.size else np.zeros(0, np.int64)
+    ends = np.r_[starts[1:], ks.size] if ks.size else np.zeros(0, np.int64)
+    out_t, out_m, out_p = [], [], []
+    for a, b in zip(starts, ends):
+        idx = order[a:b]
+        idx = idx[w[idx] > 0]
+        if idx.size == 0:
+            continue
+        ws = w[idx]
+        n = min(int(rng.poisson(ws.sum() * downsample)), idx.size)
+        if n == 0:
+            continue
+        pick = idx[rng.choice(idx.size, size=n, replace=False, p=ws / ws.sum())]
+        out_t.append(times[pick])
+        out_m.append(module_id[pick])
+        if pmt is not None:
+            out_p.append(pmt[pick])
+    cat = lambda parts, like: np.concatenate(parts) if parts else np.zeros(0, like.dtype)
+    return cat(out_t, times), cat(out_m, module_id), (cat(out_p, pmt) if pmt is not None else None)

@@ -140,3 +140,38 @@ def test_segment_chunk_tables():
     chunk_sums = np.array([v[co[c]:co[c + 1]].sum() for c in range(len(co) - 1)])
     seg = np.array([chunk_sums[sc[s]:sc[s + 1]].sum() for s in range(len(offs) - 1)])
     np.testing.assert_array_equal(seg, [v[offs[s]:offs[s + 1]].sum() for s in range(len(offs) - 1)])
+
+
+def test_resample_simulation_counts_and_subsets():
+    """pipeline.jl:86-116: per (module, PMT) Poisson(Σw) hits drawn without
+    replacement ∝ weight from the group's photons."""
+    from neutrinotelescopes_amd import resample_simulation
+    rng = np.random.default_rng(7)
+    n = 4000
+    mod = rng.integers(1, 4, n)
+    pmt = rng.integers(1, 3, n)
+    w = rng.uniform(0.0, 0.02, n)
+    w[::10] = 0.0
+    t = np.arange(n, dtype=float)
+    sums = {}
+    for m in (1, 2, 3):
+        for p in (1, 2):
+            sums[(m, p)] = w[(mod == m) & (pmt == p)].sum()
+    counts = {k: [] for k in sums}
+    for rep in range(300):
+        tt, mm, pp = resample_simulation(t, mod, w, pmt, rng=np.random.default_rng(rep))
+        assert len(np.unique(tt)) == len(tt)                          # without replacement
+        idx = tt.astype(int)
+        assert np.all(mod[idx] == mm) and np.all(pmt[idx] == pp)      # labels follow the photon
+        assert np.all(w[idx] > 0)                                     # zero weight never drawn
+        for k in sums:
+            counts[k].append(np.sum((mm == k[0]) & (pp == k[1])))
+    for k, lam in sums.items():
+        c = np.array(counts[k])
+        assert abs(c.mean() - lam) < 4 * math.sqrt(lam / 300) + 0.05, (k, c.mean(), lam)
+    # heavier photons are drawn more often: Σw = 0.55, n ~ min(Poisson(0.55), 2);
+    # E[#photon 0] = P(n=1)·0.5/0.55 + P(n=2) = 0.394, E[#photon 1] = 0.135
+    hits = np.concatenate([resample_simulation(np.arange(2.0), np.ones(2, int), np.array([0.5, 0.05]), None, 1.0,
+                                               np.random.default_rng(s))[0] for s in range(2000)])
+    assert abs((hits == 0).mean() * len(hits) / 2000 - 0.394) < 0.04
+    assert abs((hits == 1).mean() * len(hits) / 2000 - 0.135) < 0.03
