@@ -469,7 +469,7 @@ __global__ __launch_bounds__(kThreads, NTP_MIN_WAVES_PER_SIMD) void propagate_ke
 
             for (int k = 0; k < m.max_steps; ++k) {
                 const uint4 r = rng.block(1u + (uint32_t)k);
-                const float step = -logf(u01_oc(r.x)) * sca_len;
+                const float step = -log_unit(u01_oc(r.x)) * sca_len;
                 // first listed target hit within this step (photon_prop_cuda.jl:457-468)
                 float dt = 0.0f;
                 const int hit_t = first_hit(p, pos, dir, step, dt, tests, ctests);

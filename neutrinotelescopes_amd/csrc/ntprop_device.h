@@ -45,6 +45,13 @@ struct Rng {
     }
 };
 
+// ln u for u in (0, 1] (no denormals, no infinities): the hardware log2
+// (v_log_f32) times ln 2 split in two parts, within ~2 ulp of ln u.
+__device__ __forceinline__ float log_unit(float u) {
+    const float y = __builtin_amdgcn_logf(u);
+    return __builtin_fmaf(y, 0.693145751953125f, y * 1.42860682030941723212e-6f);
+}
+
 __device__ __forceinline__ float clampf(float x, float lo, float hi) {
     return fminf(fmaxf(x, lo), hi);
 }
@@ -135,7 +142,8 @@ __device__ __forceinline__ float sl_cos(float g, float xi) {
 __device__ __forceinline__ float3 update_direction(float3 dir, float eta, float u_phi, float u_mix,
                                                    const Medium& m) {
     const float c = (u_mix < m.sl_fraction) ? sl_cos(m.g, eta) : hg_cos(m.g, eta);
-    const float s = sqrtf(__builtin_fmaf(-c, c, 1.0f));
+    // v_sqrt_f32 (1 ulp): the argument lies in [0, 1]
+    const float s = __builtin_amdgcn_sqrtf(__builtin_fmaf(-c, c, 1.0f));
     const float phi = 0.0f + u_phi * kTwoPi;
     float sp, cp;
     sincosf(phi, &sp, &cp);

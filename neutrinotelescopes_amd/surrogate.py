@@ -684,7 +684,7 @@ def fallback_terms(engine, medium, spectrum, casc: Cascades, modules: np.ndarray
     np.add.at(lam, pos, w)
     n_obs = counts.reshape(-1)[all_keys]
     with np.errstate(divide="ignore"):
-        pois = np.where(n_obs > 0, n_obs * np.log(lam), 0.0) - lam - np.vectorize(math.lgamma)(n_obs + 1.0)
+        pois = n_obs * np.log(np.where(lam > 0, lam, 1.0)) - lam - np.vectorize(math.lgamma)(n_obs + 1.0)
     pois = np.where((lam == 0) & (n_obs > 0), -1e30, pois)
     np.add.at(out, all_keys // (n_mod * 16), pois)
     # time densities: per-PMT weighted histograms, looked up at the observed times
