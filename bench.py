@@ -110,7 +110,7 @@ def cpu_baseline(workload: str, target_s: float = 12.0):
                       f"(CPU oracle, g++ -O3, OpenMP)"}
 
 
-def serving_benchmark(duration_s: float = 8.0, seed: int = 11):
+def serving_benchmark(duration_s: float = 8.0, seed: int = 11, round_photons: int = 10_000_000):
     """Serving configuration: Poisson stream of cascade requests (position
     uniform in a 300 m cylinder, isotropic direction, E log-uniform 1e2-1e6 GeV,
     1e4·E/100 GeV photons) against a 7-string × 20-module cluster, arrival
@@ -125,7 +125,7 @@ def serving_benchmark(duration_s: float = 8.0, seed: int = 11):
     eng.set_targets(cluster_targets())
     rng = np.random.default_rng(seed)
     mean_photons = 1e4 * (1e6 - 1e2) / math.log(1e4) / 100.0
-    with PhotonServer(eng, device_budget=32 << 30, host_budget=8 << 30, round_photons=20_000_000) as srv:
+    with PhotonServer(eng, device_budget=32 << 30, host_budget=8 << 30, round_photons=round_photons) as srv:
         # capacity: a burst submitted at once
         burst = cascade_requests(64, med, rng)
         t0 = time.perf_counter()
@@ -142,13 +142,15 @@ def serving_benchmark(duration_s: float = 8.0, seed: int = 11):
     return {"config": "poisson_cascades_1e2-1e6GeV_7x20_cluster", "requests": res["requests"],
             "arrival_rate_hz": rate, "p50_ms": res["p50_ms"], "p99_ms": res["p99_ms"],
             "photons_per_s": res["photons_per_s"], "capacity_photons_per_s": cap_rate,
-            "rounds": st["rounds"], "device_busy_fraction": st["busy_ms"] / 1e3 / res["wall_s"]}
+            "rounds": st["rounds"], "round_photons": round_photons,
+            "device_busy_fraction": st["busy_ms"] / 1e3 / res["wall_s"]}
 
 
 def surrogate_benchmark():
     """Flow surrogate: 128 engine photon-table launches (cascades log10 E
-    uniform 2-6, 2e8 photons each, 192 modules at 1-300 m), 24000 Adam steps
-    of batch 16384 on the bf16-MFMA training kernels, per-module expected-count
+    uniform 2-6, 2e8 photons each, 192 modules at 1-300 m), 48000 Adam steps
+    of batch 16384 on the bf16-MFMA training kernels (count rows weighted ×2),
+    per-module expected-count
     error and time-density KL against direct propagation (64 cascades; pairs
     selected with >= 400 hits in one 2e8-photon run, truth from an independent
     6e8-photon run), and event

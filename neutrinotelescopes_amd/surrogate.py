@@ -788,10 +788,10 @@ def direct_rate(engine, medium, spectrum, modules, energy: float = 3e4, photons:
     return h.n_simulated / (time.perf_counter() - t0)
 
 
-def surrogate_study(n_runs: int = 128, photons_per_run: int = 2 * 10 ** 8, train_steps: int = 24000,
+def surrogate_study(n_runs: int = 128, photons_per_run: int = 2 * 10 ** 8, train_steps: int = 48000,
                     n_events: int = 1024, n_error_events: int = 64, error_photons: int = 2 * 10 ** 8,
                     data_photons: int = 10 ** 7, seed: int = 0, device: int = 0, log=None, lr: float = 2e-3,
-                    count_weight: float = 1.0) -> dict:
+                    count_weight: float = 2.0) -> dict:
     """Tables → training → error against direct propagation → event
     likelihood throughput against direct propagation."""
     from .flow import FlowModel
