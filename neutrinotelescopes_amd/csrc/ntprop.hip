@@ -467,8 +467,25 @@ __global__ __launch_bounds__(kThreads, NTP_MIN_WAVES_PER_SIMD) void propagate_ke
             const float sca_len = scattering_length(wl, mi);
             const float c_grp = group_velocity(wl, mi);
 
+            // Scattering steps read the Philox stream of blocks 1, 2, ... three
+            // words per step (four steps per three blocks): step, HG/SL uniform
+            // and azimuth from the words' top 24 bits, the mixture uniform from
+            // their low bytes.  k is wave-uniform, so is the block schedule.
+            uint4 buf = make_uint4(0u, 0u, 0u, 0u);
+            uint32_t next_block = 1u;
             for (int k = 0; k < m.max_steps; ++k) {
-                const uint4 r = rng.block(1u + (uint32_t)k);
+                const int phase = k & 3;
+                uint4 nxt = buf;
+                if (phase != 3) nxt = rng.block(next_block++);
+                uint4 r;
+                switch (phase) {
+                    case 0: r.x = nxt.x; r.y = nxt.y; r.z = nxt.z; break;
+                    case 1: r.x = buf.w; r.y = nxt.x; r.z = nxt.y; break;
+                    case 2: r.x = buf.z; r.y = buf.w; r.z = nxt.x; break;
+                    default: r.x = buf.y; r.y = buf.z; r.z = buf.w; break;
+                }
+                buf = nxt;
+                r.w = ((r.x & 0xFFu) << 8) | ((r.y & 0xFFu) << 16) | ((r.z & 0xFFu) << 24);
                 const float step = -log_unit(u01_oc(r.x)) * sca_len;
                 // first listed target hit within this step (photon_prop_cuda.jl:457-468)
                 float dt = 0.0f;

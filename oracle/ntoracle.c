@@ -570,9 +570,24 @@ static void propagate_one(const nt_source* s, uint32_t si, uint64_t p, const nt_
     float dist = 0.0f;
     float sca_len = ntor_scattering_length(wl, m);
     float c_grp = ntor_group_velocity(wl, m);
+    /* Scattering steps read the Philox stream of blocks 1, 2, ... three words
+       per step (four steps per three blocks): step length, HG/SL uniform and
+       azimuth from the top 24 bits of the three words, the mixture uniform
+       from their low bytes (bits 0-7 of words 1, 2, 3 as bits 8-15, 16-23,
+       24-31). */
+    uint32_t buf[4];
+    uint32_t next_block = 1u;
+    int used = 4;
     for (int k = 0; k < m->max_steps; ++k) {
         uint32_t r[4];
-        draw_block(seed, si, p, 1u + (uint32_t)k, r);
+        for (int j = 0; j < 3; ++j) {
+            if (used == 4) {
+                draw_block(seed, si, p, next_block++, buf);
+                used = 0;
+            }
+            r[j] = buf[used++];
+        }
+        r[3] = ((r[0] & 0xFFu) << 8) | ((r[1] & 0xFFu) << 16) | ((r[2] & 0xFFu) << 24);
         float step = -logf(u01_oc(r[0])) * sca_len;
         int hit = 0, hit_t = -1;
         float dt = 0.0f;
