@@ -17,8 +17,10 @@
  *  - Random numbers: Philox4x32-10 (Salmon et al., SC'11).  key = (seed_lo32,
  *    seed_hi32); counter = (block, photon_lo32, photon_hi32, source_index),
  *    where photon is the photon's index inside its source.  Block 0 holds the
- *    initial-state draws, block 1+k the draws of scattering step k, block
- *    0x80000000+t the draws of Gamma rejection trial t.  A photon's history is
+ *    initial-state draws; scattering steps read the words of blocks 1, 2, ...
+ *    in order, three per step (step length, HG/SL uniform and azimuth from the
+ *    top 24 bits, the mixture uniform from the three low bytes); block
+ *    0x80000000+t holds the draws of Gamma rejection trial t.  A photon's history is
  *    therefore a pure function of (seed, source_index, photon index), whatever
  *    the launch shape or the other sources in the batch.  The reference seeds
  *    CUDA.jl's device RNG with seed + thread index (photon_prop_cuda.jl:423).
