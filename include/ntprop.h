@@ -290,6 +290,42 @@ void nt_server_destroy(nt_server* srv);
 /* Grid used by nt_launch (blocks, threads per block). */
 int32_t nt_launch_shape(nt_ctx* ctx, int32_t* blocks, int32_t* threads);
 
+/* ---- source builder ------------------------------------------------------
+   Particles → extended-Cherenkov sources on the device, one thread per
+   particle: the reference's ExtendedCherenkovEmitter(particle, medium,
+   wl_range; oversample) (lightyield.jl:471-481), i.e.
+     long_a   = Float32(alpha + beta·log10 E),  long_b = Float32(b)
+                (LongitudinalParameterisation, lightyield.jl:139-155, 204-213;
+                 alpha, beta, b of the particle type, lightyield.jl:135-137)
+     long_lrad = Float32(model.lrad)
+     photons  ~ Poisson(model.photons_per_m · track_alpha · E^track_beta ·
+                model.oversample)   (total_lightyield, lightyield.jl:350-366)
+   The Poisson draw uses Philox4x32-10 with key = seed, counter =
+   (trial, particle index lo, hi, 0x50000000): inversion for a mean below 10,
+   Hörmann's transformed rejection (PTRS, 1993) above, in double precision.
+   (The reference draws with PoissonRandom.pois_rand, whose streams are not
+   reproducible here.)  d_particles and d_sources are device arrays. */
+enum { NT_PARTICLE_EMINUS = 0, NT_PARTICLE_EPLUS = 1, NT_PARTICLE_GAMMA = 2 };
+
+typedef struct nt_particle {         /* 40 bytes */
+    float position[3];               /* m */
+    float direction[3];              /* unit vector */
+    float time;                      /* ns */
+    float energy;                    /* GeV */
+    int32_t type;                    /* NT_PARTICLE_* */
+    int32_t reserved;
+} nt_particle;
+
+typedef struct nt_cascade_model {
+    double photons_per_m;            /* frank_tamm_norm(wl_range, medium) */
+    double lrad;                     /* radiation_length / density · 10 (m) */
+    double oversample;               /* > 0 */
+} nt_cascade_model;
+
+int32_t nt_build_cascade_sources(nt_ctx* ctx, const nt_particle* d_particles, int32_t n,
+                                 const nt_cascade_model* model, uint64_t seed, nt_source* d_sources,
+                                 void* stream);
+
 /* ---- device-function evaluation (test hook) ----------------------------
    Runs one device function of the propagation kernel on n inputs (host
    arrays; row-major, `in_width` floats per row) and writes `out_width` floats

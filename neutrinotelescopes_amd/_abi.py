@@ -145,6 +145,18 @@ assert C.sizeof(nt_source) == 64
 assert C.sizeof(nt_medium) == 48
 assert C.sizeof(nt_target) == 160
 
+class nt_particle(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("direction", C.c_float * 3), ("time", C.c_float),
+                ("energy", C.c_float), ("type", C.c_int32), ("reserved", C.c_int32)]
+
+
+class nt_cascade_model(C.Structure):
+    _fields_ = [("photons_per_m", C.c_double), ("lrad", C.c_double), ("oversample", C.c_double)]
+
+
+NT_PARTICLE_EMINUS, NT_PARTICLE_EPLUS, NT_PARTICLE_GAMMA = 0, 1, 2
+
+
 class nt_server_config(C.Structure):
     _fields_ = [
         ("device_budget_bytes", C.c_uint64),
@@ -189,6 +201,8 @@ SIGNATURES = {
     "nt_server_destroy": (None, [C.c_void_p]),
     "nt_eval": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
                             C.c_int64]),
+    "nt_build_cascade_sources": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(nt_cascade_model),
+                                             C.c_uint64, C.c_void_p, C.c_void_p]),
     # include/ntflow.h (flow surrogate)
     "ntf_create": (C.c_int32, [C.c_int32, C.POINTER(C.c_void_p)]),
     "ntf_destroy": (None, [C.c_void_p]),

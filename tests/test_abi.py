@@ -2,6 +2,8 @@
 (no device calls: runs without a GPU)."""
 import ctypes as C
 import re
+import shutil
+import subprocess
 from pathlib import Path
 
 import pytest
@@ -55,6 +57,32 @@ def test_struct_layouts():
     assert C.sizeof(_abi.nt_medium) == 48
     assert C.sizeof(_abi.nt_target) == 160
     assert _abi.nt_hits.capacity.offset == 12 * 8
+    assert C.sizeof(_abi.nt_particle) == 40 and C.sizeof(_abi.nt_cascade_model) == 24
+    assert _abi.nt_particle.type.offset == 32
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="needs a C compiler")
+def test_struct_layouts_match_c_compiler(tmp_path):
+    """sizeof and field offsets of every ABI struct as a C compiler lays them
+    out from the header, against the ctypes mirror."""
+    structs = ["nt_source", "nt_medium", "nt_target", "nt_spectrum", "nt_hits", "nt_particle", "nt_cascade_model"]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ntprop.h"', "int main(void) {"]
+    for st in structs:
+        lines.append(f'printf("{st} %zu\\n", sizeof({st}));')
+        for name, *_ in getattr(_abi, st)._fields_:
+            lines.append(f'printf("{st}.{name} %zu\\n", offsetof({st}, {name}));')
+    lines.append("return 0; }")
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-std=c11", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                   text=True).stdout.splitlines())
+    for st in structs:
+        cls = getattr(_abi, st)
+        assert int(got[st]) == C.sizeof(cls), st
+        for name, *_ in cls._fields_:
+            assert int(got[f"{st}.{name}"]) == getattr(cls, name).offset, (st, name)
 
 
 def test_library_missing_fails_loudly(tmp_path):

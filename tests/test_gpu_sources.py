@@ -1,0 +1,118 @@
+"""Device-side source builder (nt_build_cascade_sources) against the FP64
+oracle on the same Philox draws, the Poisson law at a million particles,
+and built sources feeding the propagation kernel."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _med():
+    from neutrinotelescopes_amd import make_cascadia_medium_properties
+    return make_cascadia_medium_properties(0.95)
+
+
+def test_device_sources_match_oracle(engine):
+    from neutrinotelescopes_amd import LongitudinalParameterisation, PEMinus, PEPlus, PGamma
+    from neutrinotelescopes_amd import injectors as I
+    from oracle import poisson as P
+    from _oracle import philox
+    med = _med()
+    rng = np.random.default_rng(5)
+    inj = I.VolumeInjector(I.Cylinder(np.zeros(3), 500.0, 300.0), I.LogUniform(1e-5, 1e7), (PEMinus, PEPlus, PGamma),
+                           (1.0, 1.0, 1.0))
+    ps = inj.sample(3000, rng)
+    seed = 0x1234_5678_9ABC
+    print("source builder: 3000 particles, seed", hex(seed), flush=True)
+    srcs = I.build_cascade_sources(ps, med, oversample=1.0, seed=seed, engine=engine)
+    model = I.cascade_model(med)
+    code = {PEMinus: 0, PEPlus: 1, PGamma: 2}
+    same_k = same_a = 0
+    small = 0
+    for i, (p, s) in enumerate(zip(ps, srcs)):
+        e32 = float(np.float32(p.energy))
+        k, a, b = P.cascade_source(e32, code[p.type], model.photons_per_m, 1.0, philox, seed, i)
+        small += model.photons_per_m * P.TYPE_PARAMS[code[p.type]][3] * e32 ** P.TYPE_PARAMS[code[p.type]][4] < 10
+        same_k += s.photons == k
+        same_a += np.float32(s.long_param.a) == np.float32(a)
+        assert abs(np.float32(s.long_param.a) - np.float32(a)) <= 2 * abs(np.spacing(np.float32(a)))
+        lp = LongitudinalParameterisation.from_energy(e32, med, p.type)
+        assert np.float32(s.long_param.b) == np.float32(lp.b) and np.float32(s.long_param.lrad) == np.float32(lp.lrad)
+        np.testing.assert_array_equal(s.position, np.float32(p.position))
+        np.testing.assert_array_equal(s.direction, np.float32(p.direction))
+        assert s.time == np.float32(p.time)
+    print("photon counts equal", same_k, "profile a equal", same_a, "inversion branch", small)
+    assert small >= 50                    # both Poisson branches are exercised
+    assert same_k >= 2997 and same_a >= 2997
+
+
+def test_poisson_law_at_full_size(engine):
+    """2^20 particles at one energy per branch: the photon counts have the
+    Poisson mean and variance (size-independent check)."""
+    import torch
+    from neutrinotelescopes_amd import injectors as I
+    med = _med()
+    model = I.cascade_model(med)
+    n = 1 << 20
+    for energy in (1e-5, 1e-3, 2.0):
+        mu = model.photons_per_m * 5.3207078881 * float(np.float32(energy)) ** 1.00000211
+        print("poisson law: n", n, "energy", energy, "mean", mu, flush=True)
+        rec = I.pack_particles(np.zeros((n, 3)), np.tile([0, 0, 1.0], (n, 1)), 0.0, energy, 0)
+        d = torch.from_numpy(rec.view(np.uint8)).cuda()
+        out = I.build_sources_device(d, n, model, 99, engine)
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record()
+        I.build_sources_device(d, n, model, 99, engine)
+        t1.record()
+        t1.synchronize()
+        print("  build time %.3f ms" % t0.elapsed_time(t1), flush=True)
+        k = out.view(torch.int64).view(n, 8)[:, 1].double()
+        m, v = k.mean().item(), k.var().item()
+        assert abs(m - mu) < 5 * math.sqrt(mu / n), (energy, m, mu)
+        assert abs(v - mu) < 5 * math.sqrt((mu + 2 * mu * mu) / n), (energy, v, mu)
+
+
+def test_seed_determinism_and_errors(engine):
+    import torch
+    from neutrinotelescopes_amd import _abi
+    from neutrinotelescopes_amd import injectors as I
+    med = _med()
+    model = I.cascade_model(med)
+    rec = I.pack_particles(np.zeros((4096, 3)), np.tile([1.0, 0, 0], (4096, 1)), 0.0, 0.01, 2)
+    d = torch.from_numpy(rec.view(np.uint8)).cuda()
+    a = I.build_sources_device(d, 4096, model, 7, engine).clone()
+    b = I.build_sources_device(d, 4096, model, 7, engine)
+    c = I.build_sources_device(d, 4096, model, 8, engine)
+    assert torch.equal(a, b) and not torch.equal(a, c)
+    bad = _abi.nt_cascade_model(model.photons_per_m, model.lrad, 0.0)
+    with pytest.raises(RuntimeError):
+        I.build_sources_device(d, 4096, bad, 7, engine)
+    assert I.build_sources_device(d[:0], 0, model, 7, engine).numel() == 0
+
+
+def test_built_sources_propagate(engine):
+    """Sources built on the device are the records the propagation kernel
+    takes: a 10 GeV cascade 10 m from a module gives the same hits as the
+    host-built emitter with the same photon count."""
+    from neutrinotelescopes_amd import (CherenkovSpectrum, ExtendedCherenkovEmitter, LongitudinalParameterisation,
+                                        Particle, PEMinus, PhotonPropSetup, propagate_photons)
+    from neutrinotelescopes_amd.detection import MultiPMTDetector, make_pom_pmt_coordinates
+    from neutrinotelescopes_amd import injectors as I
+    med = _med()
+    p = Particle(np.array([0.0, 3.0, 10.0], np.float32), np.array([0.0, 0.0, -1.0], np.float32), 0.0, 10.0, PEMinus)
+    srcs = I.build_cascade_sources([p], med, oversample=1.0, seed=3, engine=engine)
+    assert srcs[0].photons > 10 ** 6
+    host = ExtendedCherenkovEmitter(p.position, p.direction, 0.0, srcs[0].photons,
+                                    LongitudinalParameterisation.from_energy(10.0, med, PEMinus))
+    tgt = MultiPMTDetector(np.zeros(3, np.float32), 0.21, math.pi * 0.0375 ** 2, make_pom_pmt_coordinates(), 1)
+    spec = CherenkovSpectrum((300.0, 800.0), 30, med)
+    a = propagate_photons(PhotonPropSetup(srcs, [tgt], med, spec, seed=4))
+    b = propagate_photons(PhotonPropSetup([host], [tgt], med, spec, seed=4))
+    print("hits", len(a.time), len(b.time))
+    assert len(a.time) > 20
+    order_a, order_b = np.argsort(a.photon_id), np.argsort(b.photon_id)
+    np.testing.assert_array_equal(a.photon_id[order_a], b.photon_id[order_b])
+    np.testing.assert_array_equal(a.time[order_a], b.time[order_b])
+    np.testing.assert_array_equal(a.position[order_a], b.position[order_b])
