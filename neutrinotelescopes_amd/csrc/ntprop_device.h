@@ -52,6 +52,19 @@ __device__ __forceinline__ float log_unit(float u) {
     return __builtin_fmaf(y, 0.693145751953125f, y * 1.42860682030941723212e-6f);
 }
 
+// sin and cos of 2π·u for u in [0, 1]: v_sin_f32 / v_cos_f32 take the angle
+// in revolutions, so 2π·u is never rounded to FP32; over all 24-bit uniforms
+// the error is at most 1.3e-7 (sincosf of the rounded FP32 angle: 4.2e-7).
+__device__ __forceinline__ void sincos_rev(float u, float* s, float* c) {
+    *s = __builtin_amdgcn_sinf(u);
+    *c = __builtin_amdgcn_cosf(u);
+}
+
+// x^y for x >= 0 (normal or zero) and y > 0: hardware exp2(y·log2 x).
+__device__ __forceinline__ float pow_pos(float x, float y) {
+    return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+}
+
 __device__ __forceinline__ float clampf(float x, float lo, float hi) {
     return fminf(fmaxf(x, lo), hi);
 }
@@ -113,6 +126,16 @@ __device__ __forceinline__ float3 sph_to_cart(float theta, float phi) {
     return make_float3(cp * st, sp * st, ct);
 }
 
+// sph_to_cart for θ given by its cosine (sin θ = √(1 − cos²θ), θ in [0, π])
+// and φ = 2π·u_phi: the reference's acos → sincos round trip becomes one
+// square root.
+__device__ __forceinline__ float3 sph_to_cart_cos(float ct, float u_phi) {
+    const float st = __builtin_amdgcn_sqrtf(fmaxf(__builtin_fmaf(-ct, ct, 1.0f), 0.0f));
+    float sp, cp;
+    sincos_rev(u_phi, &sp, &cp);
+    return make_float3(cp * st, sp * st, ct);
+}
+
 // rot_from_ez_fast, utils.jl:228-243 (the |a_z| == 1 branch as written)
 __device__ __forceinline__ float3 rot_from_ez(float3 a, float3 o) {
     if (fabsf(a.z) == 1.0f) return make_float3(o.x, copysignf(o.y, a.z), copysignf(o.z, a.z));
@@ -135,7 +158,7 @@ __device__ __forceinline__ float hg_cos(float g, float eta) {
 // Simplified-Liu: cosθ = 2 ξ^{1/(1+α)} − 1, α = 2g/(1−g)  (extension)
 __device__ __forceinline__ float sl_cos(float g, float xi) {
     const float alpha = 2.0f * g / (1.0f - g);
-    return clampf(2.0f * powf(xi, 1.0f / (1.0f + alpha)) - 1.0f, -1.0f, 1.0f);
+    return clampf(2.0f * pow_pos(xi, 1.0f / (1.0f + alpha)) - 1.0f, -1.0f, 1.0f);
 }
 
 // update_direction, photon_prop_cuda.jl:204-222
@@ -144,17 +167,14 @@ __device__ __forceinline__ float3 update_direction(float3 dir, float eta, float 
     const float c = (u_mix < m.sl_fraction) ? sl_cos(m.g, eta) : hg_cos(m.g, eta);
     // v_sqrt_f32 (1 ulp): the argument lies in [0, 1]
     const float s = __builtin_amdgcn_sqrtf(__builtin_fmaf(-c, c, 1.0f));
-    const float phi = 0.0f + u_phi * kTwoPi;
     float sp, cp;
-    sincosf(phi, &sp, &cp);
+    sincos_rev(u_phi, &sp, &cp);       // φ = 2π·u_phi
     return rot_from_ez(dir, make_float3(cp * s, sp * s, c));
 }
 
 // initialize_direction_isotropic, photon_prop_cuda.jl:100-104
 __device__ __forceinline__ float3 iso_dir(float u_cos, float u_phi) {
-    const float theta = acosf(-1.0f + u_cos * 2.0f);
-    const float phi = 0.0f + u_phi * kTwoPi;
-    return sph_to_cart(theta, phi);
+    return sph_to_cart_cos(-1.0f + u_cos * 2.0f, u_phi);     // θ = acos(2u − 1)
 }
 
 // sample_cherenkov_direction (photon_prop_cuda.jl:107-125) with
@@ -162,14 +182,12 @@ __device__ __forceinline__ float3 iso_dir(float u_cos, float u_phi) {
 // (utils.jl:249-260)
 __device__ __forceinline__ float3 cherenkov_dir(float wl, float u_phi, float u_tcos, float u_tphi,
                                                 float3 src_dir, const Medium& m) {
-    const float theta_c = acosf(1.0f / refractive_index(wl, m));
-    const float phi = 0.0f + u_phi * kTwoPi;
-    const float3 ph = sph_to_cart(theta_c, phi);
+    const float3 ph = sph_to_cart_cos(1.0f / refractive_index(wl, m), u_phi);   // θc = acos(1/n)
     const float a = 0.39f, b = 2.61f;
-    const float I = m.track_norm;
-    const float t = -logf(1.0f - u_tcos * I) / b;
-    const float ct = fmaxf(1.0f - powf(t, 1.0f / a), -1.0f);
-    const float3 track = sph_to_cart(acosf(ct), kTwoPi * u_tphi);
+    const float I = m.track_norm;                    // < 1: the log argument lies in (0, 1]
+    const float t = -log_unit(1.0f - u_tcos * I) / b;
+    const float ct = fmaxf(1.0f - pow_pos(t, 1.0f / a), -1.0f);
+    const float3 track = sph_to_cart_cos(ct, u_tphi);
     return rot_from_ez(src_dir, rot_from_ez(track, ph));
 }
 
@@ -225,15 +243,14 @@ __device__ __forceinline__ float rand_gamma(float d, float c, float scale, const
                                             int* trials) {
     for (uint32_t t = 0;; ++t) {
         const uint4 r = rng.block(kGammaBlock + t);
-        float sn, cs;
-        sincosf(kTwoPi * u01(r.y), &sn, &cs);
-        const float x = sqrtf(-2.0f * logf(u01_oc(r.x))) * cs;
+        const float cs = __builtin_amdgcn_cosf(u01(r.y));      // cos 2πu
+        const float x = __builtin_amdgcn_sqrtf(-2.0f * log_unit(u01_oc(r.x))) * cs;
         float v = __builtin_fmaf(c, x, 1.0f);
         if (v <= 0.0f) continue;
         v = v * v * v;
         const float u = u01_oc(r.z);
         const float xsq = x * x;
-        if (logf(u) < 0.5f * xsq + d * ((1.0f - v) + logf(v))) {
+        if (log_unit(u) < 0.5f * xsq + d * ((1.0f - v) + logf(v))) {
             if (trials) *trials = (int)t + 1;
             return d * v * scale;
         }

@@ -169,7 +169,7 @@ def test_rotation_and_directions_device(medium_engine, oracle):
     u = rng.random((n, 2)).astype(np.float32)
     y = eng.eval(_abi.NT_FN_ISO_DIR, u, 3)
     theta = np.arccos((np.float32(-1) + u[:, 0] * np.float32(2)).astype(np.float64))
-    phi = (u[:, 1] * np.float32(6.2831855)).astype(np.float64)
+    phi = 2 * np.pi * u[:, 1].astype(np.float64)        # the kernel takes φ in revolutions: 2πu unrounded
     f64 = np.stack([np.cos(phi) * np.sin(theta), np.sin(phi) * np.sin(theta), np.cos(theta)], 1)
     # acos is ill-conditioned near ±1: sinθ error is bounded by ulp(cosθ)/sinθ
     sin_t = np.maximum(np.sin(theta), 1e-3)[:, None]
