@@ -101,9 +101,11 @@ struct Epilogue {
 
 // Target clusters.  Targets sharing (x, y) form a vertical string; a photon
 // step is tested against a string's bounding cylinder first and against its
-// spheres only when the step's segment comes within reach.  Per cluster:
-//   geometry float4 (axis x, axis y, lowest centre z, highest centre z)
-//   largest radius (float), member range int2 (first, count)
+// spheres only when the step's segment comes within reach.  Per cluster, two
+// float4 (one 32-byte scalar load):
+//   [2c]     axis x, axis y, lowest centre z, highest centre z
+//   [2c + 1] reach r0, first member, member count (int bits), 0
+// with r0 = r_max + 2^-10·(r_max + 1) + 1e-3 m rounded up to FP32.
 // Members (targets reordered by cluster): float4 (x, y, z, r^2) and their
 // list index, so the first listed target still wins.
 struct KParams {
@@ -112,9 +114,7 @@ struct KParams {
     const SlotDesc* __restrict__ slots;      // null: single output (hits, counters)
     const float4* __restrict__ targets;      // x, y, z, r^2 in list order
     const uint32_t* __restrict__ module_ids; // per list index
-    const float4* __restrict__ clusters;     // geometry per cluster
-    const float* __restrict__ cluster_rmax;  // largest radius per cluster
-    const int2* __restrict__ cluster_range;  // (first member, count) per cluster
+    const float4* __restrict__ clusters;     // two records per cluster
     const float4* __restrict__ members;      // x, y, z, r^2 in cluster order
     const uint32_t* __restrict__ member_idx; // list index of each member
     const float* __restrict__ knots;
@@ -149,7 +149,9 @@ __device__ __forceinline__ bool sphere_hit(float3 pos, float3 dir, float step, f
 // m = 2^-10·(step + r_max + 1) + 1e-3 m: the FP32 discriminant of the sphere
 // test can accept rays passing up to ~sqrt(5·2^-23)·|p − c| beyond the radius,
 // and |p − c| ≤ step + r for any accepted hit, so a culled target can never be
-// one the exact test would accept.
+// one the exact test would accept.  The bound tests are conservative
+// geometry, not the reference's arithmetic, and use fused forms; their
+// roundings are ~1e-7 relative, far inside the 2^-10 and 1e-3 m slack.
 __device__ __forceinline__ int first_hit(const KParams& p, float3 pos, float3 dir, float step, float& dt,
                                          uint32_t& tests, uint32_t& ctests) {
     int best = 0x7fffffff;
@@ -167,33 +169,27 @@ __device__ __forceinline__ int first_hit(const KParams& p, float3 pos, float3 di
         return best == 0x7fffffff ? -1 : best;
     }
     typedef const __attribute__((address_space(4))) uint32_t* const_u32_ptr;
-    typedef const __attribute__((address_space(4))) float* const_f32_ptr;
-    typedef int i2 __attribute__((ext_vector_type(2)));
-    typedef const __attribute__((address_space(4))) i2* const_i2_ptr;
     const const_f4_ptr cl = (const_f4_ptr)(const void*)p.clusters;
-    const const_f32_ptr cl_r = (const_f32_ptr)(const void*)p.cluster_rmax;
-    const const_i2_ptr cl_k = (const_i2_ptr)(const void*)p.cluster_range;
     const const_f4_ptr mem = (const_f4_ptr)(const void*)p.members;
     const const_u32_ptr midx = (const_u32_ptr)(const void*)p.member_idx;
-    const float z1 = pos.z + dir.z * step;
+    const float z1 = __builtin_fmaf(dir.z, step, pos.z);
     const float zmin = fminf(pos.z, z1), zmax = fmaxf(pos.z, z1);
     const float dx = dir.x * step, dy = dir.y * step;
-    const float dd = dx * dx + dy * dy;
+    const float dd = __builtin_fmaf(dx, dx, dy * dy);
     const float inv_dd = dd > 0.0f ? __builtin_amdgcn_rcpf(dd) : 0.0f;
     for (int c = 0; c < p.n_clusters; ++c) {
-        const f4 c0 = cl[c];
-        const float rmax = cl_r[c];
-        const i2 range = cl_k[c];
-        const int first = range.x;
-        const int count = range.y;
+        const f4 c0 = cl[2 * c];
+        const f4 c1 = cl[2 * c + 1];
+        const int first = __float_as_int(c1.y);
+        const int count = __float_as_int(c1.z);
         if (first < 0 || count < 0 || first + count > p.n_targets) continue;  // corrupt table guard
         ++ctests;
-        const float re = rmax + 0x1p-10f * (step + rmax + 1.0f) + 1e-3f;
+        const float re = __builtin_fmaf(step, 0x1p-10f, c1.x);
         if (zmax < c0.z - re || zmin > c0.w + re) continue;
         const float wx = c0.x - pos.x, wy = c0.y - pos.y;
-        const float t = fminf(fmaxf((wx * dx + wy * dy) * inv_dd, 0.0f), 1.0f);
-        const float ex = wx - t * dx, ey = wy - t * dy;
-        if (ex * ex + ey * ey > re * re) continue;
+        const float t = fminf(fmaxf(__builtin_fmaf(wx, dx, wy * dy) * inv_dd, 0.0f), 1.0f);
+        const float ex = __builtin_fmaf(-t, dx, wx), ey = __builtin_fmaf(-t, dy, wy);
+        if (__builtin_fmaf(ex, ex, ey * ey) > re * re) continue;
         for (int k = first; k < first + count; ++k) {
             const f4 mv = mem[k];
             if (zmax < mv.z - re || zmin > mv.z + re) continue;
@@ -692,11 +688,9 @@ struct nt_ctx {
     bool culling = true;
     int n_clusters = 0;
     float4* d_clusters = nullptr;
-    float* d_cluster_rmax = nullptr;
-    int2* d_cluster_range = nullptr;
     float4* d_members = nullptr;
     uint32_t* d_member_idx = nullptr;
-    int64_t clusters_cap = 0, rmax_cap = 0, range_cap = 0, members_cap = 0, member_idx_cap = 0;
+    int64_t clusters_cap = 0, members_cap = 0, member_idx_cap = 0;
 
     int n_sources = 0;
     DevSource* d_sources = nullptr;
@@ -783,8 +777,6 @@ int32_t launch_plan(nt_ctx* c, uint64_t seed, const LaunchPlan& lp, hipStream_t 
     kp.targets = c->d_targets;
     kp.module_ids = c->d_module_ids;
     kp.clusters = c->d_clusters;
-    kp.cluster_rmax = c->d_cluster_rmax;
-    kp.cluster_range = c->d_cluster_range;
     kp.members = c->d_members;
     kp.member_idx = c->d_member_idx;
     kp.n_clusters = c->culling ? c->n_clusters : 0;
@@ -991,7 +983,7 @@ void nt_destroy(nt_ctx* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     void* ptrs[] = {c->d_knots, c->d_targets, c->d_module_ids, c->d_sources, c->d_offsets, c->d_counters,
-                    c->d_clusters, c->d_cluster_rmax, c->d_cluster_range, c->d_members,
+                    c->d_clusters, c->d_members,
                     c->d_member_idx, c->d_epi_targets, c->d_abs, c->d_acc, c->d_target_of_module};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
@@ -1115,8 +1107,6 @@ int32_t nt_set_targets(nt_ctx* c, const nt_target* t, int32_t n) {
         cluster_of[i] = g;
     }
     std::vector<float4> cl, mem;
-    std::vector<float> cl_rmax;
-    std::vector<int2> cl_range;
     std::vector<uint32_t> midx;
     for (const auto& grp : groups) {
         float zlo = INFINITY, zhi = -INFINITY, rmax = 0.0f;
@@ -1126,20 +1116,21 @@ int32_t nt_set_targets(nt_ctx* c, const nt_target* t, int32_t n) {
             rmax = std::max(rmax, t[i].radius);
         }
         const int first = (int)mem.size(), count = (int)grp.size();
+        const double reach = (double)rmax + 0x1p-10 * ((double)rmax + 1.0) + 1e-3;
+        float r0 = (float)reach;
+        if ((double)r0 < reach) r0 = std::nextafter(r0, INFINITY);
+        float first_bits, count_bits;
+        std::memcpy(&first_bits, &first, 4);
+        std::memcpy(&count_bits, &count, 4);
         cl.push_back(make_float4(tg[grp[0]].x, tg[grp[0]].y, zlo, zhi));
-        cl_rmax.push_back(rmax);
-        cl_range.push_back(make_int2(first, count));
+        cl.push_back(make_float4(r0, first_bits, count_bits, 0.0f));
         for (int i : grp) {
             mem.push_back(tg[i]);
             midx.push_back((uint32_t)i);
         }
     }
     const int nc = (int)groups.size();
-    st = grow(&c->d_clusters, std::max(nc, 1), &c->clusters_cap);
-    if (st) return st;
-    st = grow(&c->d_cluster_rmax, std::max(nc, 1), &c->rmax_cap);
-    if (st) return st;
-    st = grow(&c->d_cluster_range, std::max(nc, 1), &c->range_cap);
+    st = grow(&c->d_clusters, std::max(2 * nc, 1), &c->clusters_cap);
     if (st) return st;
     st = grow(&c->d_members, std::max(n, 1), &c->members_cap);
     if (st) return st;
@@ -1147,8 +1138,6 @@ int32_t nt_set_targets(nt_ctx* c, const nt_target* t, int32_t n) {
     if (st) return st;
     if (nc) {
         HIP_TRY(hipMemcpy(c->d_clusters, cl.data(), sizeof(float4) * cl.size(), hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(c->d_cluster_rmax, cl_rmax.data(), sizeof(float) * nc, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(c->d_cluster_range, cl_range.data(), sizeof(int2) * nc, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(c->d_members, mem.data(), sizeof(float4) * mem.size(), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(c->d_member_idx, midx.data(), sizeof(uint32_t) * midx.size(), hipMemcpyHostToDevice));
     }

@@ -19,6 +19,15 @@ constexpr uint32_t kGammaBlock = 0x80000000u;   // first Philox block of Gamma t
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon et al., SC'11).  One call yields four 32-bit words.
 // ---------------------------------------------------------------------------
+// a ^ b ^ c in one instruction (v_bitop3_b32, truth table 0x96)
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+    return a ^ b ^ c;
+#endif
+}
+
 __device__ __forceinline__ uint4 philox(uint4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
@@ -26,7 +35,7 @@ __device__ __forceinline__ uint4 philox(uint4 c, uint32_t k0, uint32_t k1) {
         const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
         const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
         const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
-        c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+        c = make_uint4(xor3(hi1, c.y, k0), lo1, xor3(hi0, c.w, k1), lo0);
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;
     }
@@ -63,6 +72,19 @@ __device__ __forceinline__ void sincos_rev(float u, float* s, float* c) {
 // x^y for x >= 0 (normal or zero) and y > 0: hardware exp2(y·log2 x).
 __device__ __forceinline__ float pow_pos(float x, float y) {
     return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+}
+
+// n / d, correctly rounded, for finite normal operands whose quotient and
+// reciprocal stay normal (|n|, |d|, |n/d| within [2^-100, 2^100]): the
+// compiler's IEEE division sequence — reciprocal, one Newton step, two residual
+// corrections — without its range pre-scaling and special-value fix-up, which
+// leave such operands unchanged.  Bit-identical to n / d there.
+__device__ __forceinline__ float div_normal(float n, float d) {
+    float r = __builtin_amdgcn_rcpf(d);
+    r = __builtin_fmaf(__builtin_fmaf(-d, r, 1.0f), r, r);
+    float q = n * r;
+    q = __builtin_fmaf(__builtin_fmaf(-d, q, n), r, q);
+    return __builtin_fmaf(__builtin_fmaf(-d, q, n), r, q);
 }
 
 __device__ __forceinline__ float clampf(float x, float lo, float hi) {
@@ -140,7 +162,8 @@ __device__ __forceinline__ float3 sph_to_cart_cos(float ct, float u_phi) {
 __device__ __forceinline__ float3 rot_from_ez(float3 a, float3 o) {
     if (fabsf(a.z) == 1.0f) return make_float3(o.x, copysignf(o.y, a.z), copysignf(o.z, a.z));
     const float a1sq = a.x * a.x, a2sq = a.y * a.y;
-    const float fact = (1.0f - a.z) / (a2sq + a1sq);
+    // 1 − a_z in [2^-24, 2], a_1² + a_2² = (1 − a_z)(1 + a_z) up to rounding: normal operands
+    const float fact = div_normal(1.0f - a.z, a2sq + a1sq);
     const float a1a2 = a.x * a.y;
     const float x = (__builtin_fmaf(-a1sq, fact, 1.0f) * o.x - a1a2 * fact * o.y) + a.x * o.z;
     const float y = ((-a1a2) * fact * o.x + __builtin_fmaf(-a2sq, fact, 1.0f) * o.y) + a.y * o.z;
@@ -150,7 +173,8 @@ __device__ __forceinline__ float3 rot_from_ez(float3 a, float3 o) {
 
 // cuda_hg_scattering_func, photon_prop_cuda.jl:70-77
 __device__ __forceinline__ float hg_cos(float g, float eta) {
-    const float t = __builtin_fmaf(-g, g, 1.0f) / __builtin_fmaf(g, __builtin_fmaf(2.0f, eta, -1.0f), 1.0f);
+    // numerator 1 − g² and denominator 1 + g(2η − 1) ≥ 1 − |g| are normal for |g| < 1
+    const float t = div_normal(__builtin_fmaf(-g, g, 1.0f), __builtin_fmaf(g, __builtin_fmaf(2.0f, eta, -1.0f), 1.0f));
     const float c = (1.0f / (2.0f * g)) * (__builtin_fmaf(g, g, 1.0f) - t * t);
     return clampf(c, -1.0f, 1.0f);
 }
