@@ -466,31 +466,20 @@ __global__ __launch_bounds__(kThreads, NTP_MIN_WAVES_PER_SIMD) void propagate_ke
             // Scattering steps read the Philox stream of blocks 1, 2, ... three
             // words per step (four steps per three blocks): step, HG/SL uniform
             // and azimuth from the words' top 24 bits, the mixture uniform from
-            // their low bytes.  k is wave-uniform, so is the block schedule.
-            uint4 buf = make_uint4(0u, 0u, 0u, 0u);
-            uint32_t next_block = 1u;
-            for (int k = 0; k < m.max_steps; ++k) {
-                const int phase = k & 3;
-                uint4 nxt = buf;
-                if (phase != 3) nxt = rng.block(next_block++);
-                uint4 r;
-                switch (phase) {
-                    case 0: r.x = nxt.x; r.y = nxt.y; r.z = nxt.z; break;
-                    case 1: r.x = buf.w; r.y = nxt.x; r.z = nxt.y; break;
-                    case 2: r.x = buf.z; r.y = buf.w; r.z = nxt.x; break;
-                    default: r.x = buf.y; r.y = buf.z; r.z = buf.w; break;
-                }
-                buf = nxt;
-                r.w = ((r.x & 0xFFu) << 8) | ((r.y & 0xFFu) << 16) | ((r.z & 0xFFu) << 24);
-                const float step = -log_unit(u01_oc(r.x)) * sca_len;
+            // their low bytes.  The loop is unrolled by four so that every
+            // step's words are fixed registers; the block index is wave-uniform.
+            // A step returns true when the photon hit a target (state at the hit).
+            auto scatter_step = [&](uint32_t wx, uint32_t wy, uint32_t wz) -> bool {
+                const uint32_t ww = ((wx & 0xFFu) << 8) | ((wy & 0xFFu) << 16) | ((wz & 0xFFu) << 24);
+                const float step = -log_unit(u01_oc(wx)) * sca_len;
                 // first listed target hit within this step (photon_prop_cuda.jl:457-468)
                 float dt = 0.0f;
                 const int hit_t = first_hit(p, pos, dir, step, dt, tests, ctests);
                 if (hit_t < 0) {
                     pos = make_float3(pos.x + dir.x * step, pos.y + dir.y * step, pos.z + dir.z * step);
                     dist += step;
-                    dir = update_direction(dir, u01(r.y), u01(r.z), u01(r.w), m);
-                    continue;
+                    dir = update_direction(dir, u01(wy), u01(wz), u01(ww), m);
+                    return false;
                 }
                 pos = make_float3(pos.x + dir.x * dt, pos.y + dir.y * dt, pos.z + dir.z * dt);
                 dist += dt;
@@ -499,7 +488,18 @@ __global__ __launch_bounds__(kThreads, NTP_MIN_WAVES_PER_SIMD) void propagate_ke
                 time += dist / c_grp;
                 mid = p.module_ids[hit_t];
                 hit = true;
-                break;
+                return true;
+            };
+            const int max_steps = m.max_steps;
+            int k = 0;
+            for (uint32_t blk = 1u; k < max_steps; blk += 3u) {
+                const uint4 b0 = rng.block(blk);
+                if (scatter_step(b0.x, b0.y, b0.z) || ++k >= max_steps) break;
+                const uint4 b1 = rng.block(blk + 1u);
+                if (scatter_step(b0.w, b1.x, b1.y) || ++k >= max_steps) break;
+                const uint4 b2 = rng.block(blk + 2u);
+                if (scatter_step(b1.z, b1.w, b2.x) || ++k >= max_steps) break;
+                if (scatter_step(b2.y, b2.z, b2.w) || ++k >= max_steps) break;
             }
         }
         simulated += __popcll(__ballot(active));
