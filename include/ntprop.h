@@ -345,7 +345,9 @@ enum {
     NT_FN_SPECTRUM = 11,        /* u -> wl                          (photon_prop_cuda.jl:128-130) */
     NT_FN_PHILOX = 12,          /* ctr4 key2 (uint32 bits) -> out4  */
     NT_FN_GAMMA = 13,           /* a b lrad seed_lo seed_hi photon (bits) -> depth trials */
-    NT_FN_COUNT = 14
+    NT_FN_SCA_LEN_FIT = 14,     /* wl -> L_sca as the kernel takes it: FP64 polynomial fit of the
+                                   formula over the spectrum's range, the formula outside it */
+    NT_FN_COUNT = 15
 };
 int32_t nt_eval(nt_ctx* ctx, int32_t fn, const float* in, int32_t in_width,
                 float* out, int32_t out_width, int64_t n);

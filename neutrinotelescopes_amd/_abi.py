@@ -51,7 +51,8 @@ NT_MAX_KNOTS = 1024
 
 (NT_FN_INTERSECT, NT_FN_HG_COS, NT_FN_SL_COS, NT_FN_ROT_FROM_EZ, NT_FN_SPH_TO_CART,
  NT_FN_ISO_DIR, NT_FN_REFRACTIVE_INDEX, NT_FN_GROUP_VELOCITY, NT_FN_SCA_LEN,
- NT_FN_CHERENKOV_DIR, NT_FN_UPDATE_DIR, NT_FN_SPECTRUM, NT_FN_PHILOX, NT_FN_GAMMA) = range(14)
+ NT_FN_CHERENKOV_DIR, NT_FN_UPDATE_DIR, NT_FN_SPECTRUM, NT_FN_PHILOX, NT_FN_GAMMA,
+ NT_FN_SCA_LEN_FIT) = range(15)
 
 
 class nt_source(C.Structure):

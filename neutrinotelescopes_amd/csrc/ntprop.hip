@@ -123,6 +123,7 @@ struct KParams {
     int64_t total;
     Medium medium;
     Spectrum spectrum;
+    WlPoly sca_poly;                         // scattering length over the spectrum's range
     int n_sources;
     int n_targets;
     int n_clusters;                          // 0: test every target in list order
@@ -460,7 +461,7 @@ __global__ __launch_bounds__(kThreads, NTP_MIN_WAVES_PER_SIMD) void propagate_ke
                     break;
             }
             idir = dir;
-            const float sca_len = scattering_length(wl, mi);
+            const float sca_len = scattering_length_fit(wl, &kp->sca_poly, mi);
             const float c_grp = group_velocity(wl, mi);
 
             // Scattering steps read the Philox stream of blocks 1, 2, ... three
@@ -583,7 +584,8 @@ __global__ __launch_bounds__(256) void epilogue_kernel(Epilogue ep, HitCols h, c
 // Device-function evaluation kernel (test hook)
 // ---------------------------------------------------------------------------
 __global__ void eval_kernel(int fn, const float* __restrict__ in, int iw, float* __restrict__ out,
-                            int ow, int64_t n, Medium m, Spectrum s, const float* __restrict__ knots) {
+                            int ow, int64_t n, Medium m, Spectrum s, const float* __restrict__ knots,
+                            WlPoly sca_poly) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float* x = in + i * iw;
@@ -627,6 +629,7 @@ __global__ void eval_kernel(int fn, const float* __restrict__ in, int iw, float*
         case NT_FN_REFRACTIVE_INDEX: y[0] = refractive_index(x[0], m); break;
         case NT_FN_GROUP_VELOCITY: y[0] = group_velocity(x[0], m); break;
         case NT_FN_SCA_LEN: y[0] = scattering_length(x[0], m); break;
+        case NT_FN_SCA_LEN_FIT: y[0] = scattering_length_fit(x[0], &sca_poly, m); break;
         case NT_FN_CHERENKOV_DIR: {
             const float3 v = cherenkov_dir(x[0], x[1], x[2], x[3], make_float3(x[4], x[5], x[6]), m);
             y[0] = v.x; y[1] = v.y; y[2] = v.z;
@@ -677,6 +680,8 @@ struct nt_ctx {
     bool has_medium = false, has_spectrum = false, has_targets = false, has_sources = false;
     Medium medium{};
     Spectrum spectrum{};
+    float spec_lo = 0.0f, spec_hi = 0.0f;  // wavelength range the spectrum produces
+    WlPoly sca_poly{};
     float* d_knots = nullptr;
     int knots_cap = 0;
 
@@ -786,6 +791,7 @@ int32_t launch_plan(nt_ctx* c, uint64_t seed, const LaunchPlan& lp, hipStream_t 
     kp.total = lp.total;
     kp.medium = c->medium;
     kp.spectrum = c->spectrum;
+    kp.sca_poly = c->sca_poly;
     kp.n_sources = lp.n_sources;
     kp.n_targets = c->n_targets;
     kp.seed_lo = (uint32_t)seed;
@@ -992,6 +998,90 @@ void nt_destroy(nt_ctx* c) {
     delete c;
 }
 
+// FP64 value of the scattering-length formula (medium.jl:331-347) with the
+// reference's FP32 constants and concentrations.
+static double sca_len_f64(double wl, float vcs, float vcl) {
+    const double x = 550.0 / wl;
+    const double c = ((double)0.0017f * std::pow(x, (double)4.3f) +
+                      (double)1.34f * (double)vcs * std::pow(x, (double)1.7f)) +
+                     (double)0.312f * (double)vcl * std::pow(x, (double)0.3f);
+    return 1.0 / c;
+}
+
+// Polynomial fit of the scattering length over the spectrum's wavelength
+// range: Chebyshev interpolation at d + 1 nodes, converted to monomials in t,
+// for the smallest degree d <= 20 whose Horner evaluation (fma, as on the
+// device) stays within 1e-12 relative of the formula on a 2001-point grid and
+// the range ends.  No degree meeting the bound leaves the fit invalid.
+static WlPoly fit_scattering_length(const Medium& m, float lo, float hi) {
+    WlPoly P{};
+    if (!(lo > 0.0f) || !(hi >= lo)) return P;
+    // widen by a relative 1e-6 (rounded outward) to take in the FP32 rounding
+    // of spectrum samples at the ends
+    const double dlo = (double)lo * (1.0 - 1e-6), dhi = (double)hi * (1.0 + 1e-6);
+    float flo = (float)dlo, fhi = (float)dhi;
+    if ((double)flo > dlo) flo = std::nextafter(flo, 0.0f);
+    if ((double)fhi < dhi) fhi = std::nextafter(fhi, INFINITY);
+    P.lo = flo;
+    P.hi = fhi;
+    const double a = flo, b = fhi;
+    if (a == b) {
+        P.c[0] = sca_len_f64(a, m.vcs, m.vcl);
+        P.valid = 1;
+        return P;
+    }
+    P.ta = 2.0 / (b - a);
+    P.tb = -(a + b) / (b - a);
+    for (int d = 6; d < kScaPolyCoefs; ++d) {
+        const int nn = d + 1;
+        double f[kScaPolyCoefs], ch[kScaPolyCoefs] = {};
+        for (int k = 0; k < nn; ++k) {
+            const double t = std::cos(M_PI * (k + 0.5) / nn);
+            f[k] = sca_len_f64(0.5 * (a + b) + 0.5 * (b - a) * t, m.vcs, m.vcl);
+        }
+        for (int j = 0; j < nn; ++j) {
+            double sum = 0.0;
+            for (int k = 0; k < nn; ++k) sum += f[k] * std::cos(M_PI * j * (k + 0.5) / nn);
+            ch[j] = (j == 0 ? 1.0 : 2.0) * sum / nn;
+        }
+        // Σ ch_j T_j(t) as monomials: T_{j+1} = 2t·T_j − T_{j−1}
+        double mono[kScaPolyCoefs] = {}, tm1[kScaPolyCoefs] = {}, tj[kScaPolyCoefs] = {};
+        tm1[0] = 1.0;                        // T_0
+        mono[0] = ch[0];
+        if (nn > 1) { tj[1] = 1.0; mono[1] += ch[1]; }
+        for (int j = 1; j + 1 < nn; ++j) {
+            double tn[kScaPolyCoefs] = {};
+            for (int q = 0; q < kScaPolyCoefs; ++q) {
+                tn[q] = -tm1[q] + (q > 0 ? 2.0 * tj[q - 1] : 0.0);
+            }
+            for (int q = 0; q < kScaPolyCoefs; ++q) {
+                mono[q] += ch[j + 1] * tn[q];
+                tm1[q] = tj[q];
+                tj[q] = tn[q];
+            }
+        }
+        double worst = 0.0;
+        for (int k = 0; k <= 2002; ++k) {
+            const double wl = k <= 2000 ? a + (b - a) * k / 2000.0 : (k == 2001 ? (double)lo : (double)hi);
+            const double t = std::fma(wl, P.ta, P.tb);
+            double y = mono[kScaPolyCoefs - 1];
+            for (int j = kScaPolyCoefs - 2; j >= 0; --j) y = std::fma(y, t, mono[j]);
+            worst = std::max(worst, std::fabs(y / sca_len_f64(wl, m.vcs, m.vcl) - 1.0));
+        }
+        if (worst <= 1e-12) {
+            for (int q = 0; q < kScaPolyCoefs; ++q) P.c[q] = mono[q];
+            P.valid = 1;
+            return P;
+        }
+    }
+    return WlPoly{};
+}
+
+static void refresh_sca_poly(nt_ctx* c) {
+    c->sca_poly = (c->has_medium && c->has_spectrum) ? fit_scattering_length(c->medium, c->spec_lo, c->spec_hi)
+                                                     : WlPoly{};
+}
+
 int32_t nt_set_medium(nt_ctx* c, const nt_medium* m) {
     if (!c || !m) return NT_ERR_ARG;
     if (!(m->mean_scattering_angle > 0.0f && m->mean_scattering_angle < 1.0f) ||
@@ -1015,6 +1105,7 @@ int32_t nt_set_medium(nt_ctx* c, const nt_medium* m) {
     // a constant: evaluated once here in FP32 with the host's libm
     d.track_norm = 1.0f - expf(-2.61f * powf(2.0f, 0.39f));
     c->has_medium = true;
+    refresh_sca_poly(c);
     if (c->epi_enabled) {
         HIP_TRY(hipSetDevice(c->device));
         return rebuild_epilogue(c);
@@ -1064,6 +1155,16 @@ int32_t nt_set_spectrum(nt_ctx* c, const nt_spectrum* s) {
     }
     c->spectrum = d;
     c->has_spectrum = true;
+    if (s->kind == NT_SPEC_TABLE) {
+        c->spec_lo = *std::min_element(s->knots, s->knots + s->n_knots);
+        c->spec_hi = *std::max_element(s->knots, s->knots + s->n_knots);
+    } else if (s->kind == NT_SPEC_MONOCHROMATIC) {
+        c->spec_lo = c->spec_hi = s->wavelength;
+    } else {
+        c->spec_lo = s->wl_range[0];
+        c->spec_hi = s->wl_range[1];
+    }
+    refresh_sca_poly(c);
     return NT_OK;
 }
 
@@ -1322,7 +1423,7 @@ int32_t nt_eval(nt_ctx* c, int32_t fn, const float* in, int32_t iw, float* out, 
         if (!c->has_spectrum) { sp = Spectrum{}; sp.kind = NT_SPEC_MONOCHROMATIC; sp.wl = 400.0f; }
         if (!kn) kn = d_in;  // never read unless a table spectrum is set
         hipLaunchKernelGGL(eval_kernel, dim3((unsigned)nb), dim3(tpb), 0, c->stream, fn, d_in, iw, d_out, ow, n,
-                           c->medium, sp, kn);
+                           c->medium, sp, kn, c->sca_poly);
         e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e == hipSuccess) e = hipMemcpy(out, d_out, sizeof(float) * ow * n, hipMemcpyDeviceToHost);

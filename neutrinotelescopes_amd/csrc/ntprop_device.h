@@ -137,6 +137,31 @@ __device__ __forceinline__ float scattering_length(float wl, const Medium& m) {
     return m.sca_len_scale == 1.0f ? l : l * m.sca_len_scale;
 }
 
+// The scattering length as the kernel takes it: a polynomial in
+// t = ta·λ + tb ∈ [−1, 1] (FP64, Horner with fma), fitted on the host to the
+// FP64 value of the formula above over the wavelength range the spectrum can
+// produce (relative error ≤ 1e-12, so the FP32 result is the correctly rounded
+// value of the formula rather than the ~2 ulp of its FP32 evaluation); outside
+// that range, or when no fit met the bound, the FP32 formula.
+constexpr int kScaPolyCoefs = 21;
+struct WlPoly {
+    float lo, hi;                    // fitted wavelength range (nm)
+    int valid, pad;
+    double ta, tb;
+    double c[kScaPolyCoefs];         // monomial coefficients in t (unused ones zero)
+};
+
+template <typename P>
+__device__ __forceinline__ float scattering_length_fit(float wl, P poly, const Medium& m) {
+    if (!poly->valid || !(wl >= poly->lo && wl <= poly->hi)) return scattering_length(wl, m);
+    const double t = __builtin_fma((double)wl, poly->ta, poly->tb);
+    double y = poly->c[kScaPolyCoefs - 1];
+#pragma unroll
+    for (int j = kScaPolyCoefs - 2; j >= 0; --j) y = __builtin_fma(y, t, poly->c[j]);
+    const float l = (float)y;
+    return m.sca_len_scale == 1.0f ? l : l * m.sca_len_scale;
+}
+
 // ---------------------------------------------------------------------------
 // Directions: utils.jl, photon_prop_cuda.jl
 // ---------------------------------------------------------------------------

@@ -197,6 +197,49 @@ def test_medium_functions_device(medium_engine, oracle):
     assert np.abs(sl - slo).max() <= 2 * ulp32(sl).max()
 
 
+def _sca_len_f64_fp32_constants(wl, vcs, vcl):
+    """The scattering-length formula (medium.jl:331-347) in FP64 with the
+    reference's FP32 literals and concentrations."""
+    f = lambda v: np.float64(np.float32(v))
+    x = f(550.0) / wl
+    return 1.0 / (f(0.0017) * x ** f(4.3) + f(1.34) * f(vcs) * x ** f(1.7) + f(0.312) * f(vcl) * x ** f(0.3))
+
+
+@pytest.mark.parametrize("kind", ["table_300_800", "inv_lambda2_300_600"])
+def test_scattering_length_fit_device(medium_engine, oracle, kind):
+    """The kernel takes the scattering length from an FP64 polynomial fitted
+    over the spectrum's wavelength range: correctly rounded (within 1 ulp of
+    the FP64 formula, equal to its FP32 rounding but for values within 1e-12
+    of a rounding boundary); outside the range, the FP32 formula itself."""
+    eng, med = medium_engine
+    spec = (nt.CherenkovSpectrum((300.0, 800.0), 30, med) if kind.startswith("table")
+            else nt.InvLambda2Spectrum((300.0, 600.0)))
+    lo, hi = ((float(spec.knots.min()), float(spec.knots.max())) if kind.startswith("table") else (300.0, 600.0))
+    rng = np.random.default_rng(7)
+    wl = np.concatenate([np.float32([lo, hi]), rng.uniform(lo, hi, 20000).astype(np.float32)])
+    vcs, vcl = med.vol_conc_small_part, med.vol_conc_large_part
+    try:
+        eng.set_spectrum(spec)
+        fit = eng.eval(_abi.NT_FN_SCA_LEN_FIT, wl, 1)[:, 0]
+        ref = eng.eval(_abi.NT_FN_SCA_LEN, wl, 1)[:, 0]
+        outside = np.float32([lo * 0.9, hi * 1.1])
+        assert np.array_equal(eng.eval(_abi.NT_FN_SCA_LEN_FIT, outside, 1)[:, 0],
+                              eng.eval(_abi.NT_FN_SCA_LEN, outside, 1)[:, 0])
+    finally:
+        eng.set_spectrum(nt.Monochromatic(400.0))
+    exact = _sca_len_f64_fp32_constants(wl.astype(np.float64), vcs, vcl)
+    err = errs_in_ulps(fit, exact, 0.0)
+    print(kind, "fit: max err %.3f ulp, correctly rounded %.5f; FP32 formula: max err %.3f ulp"
+          % (err.max(), np.mean(fit == exact.astype(np.float32)), errs_in_ulps(ref, exact, 0.0).max()))
+    assert err.max() <= 1.0
+    assert np.mean(fit == exact.astype(np.float32)) >= 0.999
+    # the oracle's FP64 formula (real-valued constants) sits within the
+    # constants' FP32 rounding of it
+    o64 = np.array([oracle.ntor_scattering_length_f64(float(w), float(np.float32(vcs)), float(np.float32(vcl)))
+                    for w in wl[:2000]])
+    assert errs_in_ulps(fit[:2000], o64, 0.0).max() <= 4
+
+
 def test_cherenkov_and_update_direction_device(medium_engine, oracle):
     eng, med = medium_engine
     rng = np.random.default_rng(4)
