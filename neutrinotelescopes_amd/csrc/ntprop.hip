@@ -492,13 +492,15 @@ __global__ __launch_bounds__(kThreads, NTP_MIN_WAVES_PER_SIMD) void propagate_ke
                 return true;
             };
             const int max_steps = m.max_steps;
+            PhiloxRun run;
+            run.init(rng.k0, rng.k1, rng.plo, rng.phi, rng.src);
             int k = 0;
             for (uint32_t blk = 1u; k < max_steps; blk += 3u) {
-                const uint4 b0 = rng.block(blk);
+                const uint4 b0 = run.block(blk);
                 if (scatter_step(b0.x, b0.y, b0.z) || ++k >= max_steps) break;
-                const uint4 b1 = rng.block(blk + 1u);
+                const uint4 b1 = run.block(blk + 1u);
                 if (scatter_step(b0.w, b1.x, b1.y) || ++k >= max_steps) break;
-                const uint4 b2 = rng.block(blk + 2u);
+                const uint4 b2 = run.block(blk + 2u);
                 if (scatter_step(b1.z, b1.w, b2.x) || ++k >= max_steps) break;
                 if (scatter_step(b2.y, b2.z, b2.w) || ++k >= max_steps) break;
             }

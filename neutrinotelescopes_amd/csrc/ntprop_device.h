@@ -20,7 +20,7 @@ constexpr uint32_t kGammaBlock = 0x80000000u;   // first Philox block of Gamma t
 // Philox4x32-10 (Salmon et al., SC'11).  One call yields four 32-bit words.
 // ---------------------------------------------------------------------------
 // a ^ b ^ c in one instruction (v_bitop3_b32, truth table 0x96)
-__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+__host__ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
 #if defined(__HIP_DEVICE_COMPILE__)
     return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
 #else
@@ -28,7 +28,7 @@ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
 #endif
 }
 
-__device__ __forceinline__ uint4 philox(uint4 c, uint32_t k0, uint32_t k1) {
+__host__ __device__ __forceinline__ uint4 philox(uint4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
@@ -41,6 +41,48 @@ __device__ __forceinline__ uint4 philox(uint4 c, uint32_t k0, uint32_t k1) {
     }
     return c;
 }
+
+// Philox4x32-10 of the counters (b, lo, hi, src) of one photon for a run of
+// block indices b, split so that work repeats neither across blocks nor
+// across lanes: with b, src and the key wave-uniform and (lo, hi) fixed per
+// photon, round 0's second product and round 1's first product depend on the
+// photon only (computed once, in `init`), round 0's first product and round
+// 1's second product on the wave only (scalar unit).  block(b) equals
+// philox(make_uint4(b, lo, hi, src), k0, k1).
+struct PhiloxRun {
+    uint32_t k0, k1, src;     // wave-uniform
+    uint32_t B, C, D;         // per photon: lo(M1·hi), hi(M0·A), lo(M0·A), A = hi(M1·hi) ^ lo ^ k0
+    __host__ __device__ __forceinline__ void init(uint32_t key0, uint32_t key1, uint32_t lo, uint32_t hi,
+                                                  uint32_t s) {
+        k0 = key0; k1 = key1; src = s;
+        const uint64_t q1 = (uint64_t)0xCD9E8D57u * hi;
+        const uint32_t A = (uint32_t)(q1 >> 32) ^ lo ^ key0;
+        B = (uint32_t)q1;
+        const uint64_t q0 = (uint64_t)0xD2511F53u * A;
+        C = (uint32_t)(q0 >> 32);
+        D = (uint32_t)q0;
+    }
+    __host__ __device__ __forceinline__ uint4 block(uint32_t b) const {
+        // rounds 0 and 1
+        const uint64_t p0 = (uint64_t)0xD2511F53u * b;
+        const uint32_t z1 = (uint32_t)(p0 >> 32) ^ src ^ k1;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * z1;
+        const uint32_t ka = k0 + 0x9E3779B9u, kb = k1 + 0xBB67AE85u;
+        uint4 c = make_uint4(((uint32_t)(p1 >> 32) ^ ka) ^ B, (uint32_t)p1, ((uint32_t)p0 ^ kb) ^ C, D);
+        // rounds 2..9
+        uint32_t r0 = k0 + 2u * 0x9E3779B9u, r1 = k1 + 2u * 0xBB67AE85u;
+#pragma unroll
+        for (int r = 2; r < 10; ++r) {
+            const uint64_t m0 = (uint64_t)0xD2511F53u * c.x;
+            const uint64_t m1 = (uint64_t)0xCD9E8D57u * c.z;
+            c = make_uint4(xor3((uint32_t)(m1 >> 32), c.y, r0), (uint32_t)m1, xor3((uint32_t)(m0 >> 32), c.w, r1),
+                           (uint32_t)m0);
+            r0 += 0x9E3779B9u;
+            r1 += 0xBB67AE85u;
+        }
+        return c;
+    }
+};
 
 // [0,1) and (0,1] uniforms carrying 24 random bits
 __device__ __forceinline__ float u01(uint32_t x) { return (float)(x >> 8) * 0x1p-24f; }
