@@ -148,10 +148,11 @@ def serving_benchmark(duration_s: float = 8.0, seed: int = 11, round_photons: in
 
 def surrogate_benchmark():
     """Flow surrogate: 128 engine photon-table launches (cascades log10 E
-    uniform 2-6, 2e8 photons each, 192 modules at 1-300 m), 48000 Adam steps
-    of batch 16384 on the bf16-MFMA training kernels (count rows weighted ×2),
+    uniform 2-6, 2e8 photons each, 192 modules at 1-300 m), 192000 Adam steps
+    (lr 1e-3, ×0.1 over the last third, weights averaged over that third) of
+    batch 16384 on the bf16-MFMA training kernels (count rows weighted ×2),
     per-module expected-count
-    error and time-density KL against direct propagation (64 cascades; pairs
+    error and time-density KL against direct propagation (128 cascades; pairs
     selected with >= 400 hits in one 2e8-photon run, truth from an independent
     6e8-photon run), and event
     likelihood throughput for 1024 cascades (log-uniform 1e3-1e6 GeV) on the

@@ -410,10 +410,18 @@ class Surrogate:
 
 def train_surrogate(tables: TableSet, steps: int = 3000, batch: int = 16384, lr: float = 1e-3,
                     seed: int = 0, model=None, log_every: int = 0, holdout: float = 0.1,
-                    count_weight: float = 1.0):
+                    count_weight: float = 1.0, average_every: int = 64, count_power: float = 1.0):
     """Adam on the mean row loss (train_model!, extended_cascade_model.jl:
-    237-295; lr decays ×0.1 over the last third).  Returns the surrogate and
-    a history dict (train/holdout loss)."""
+    237-295; lr decays ×0.1 over the last third).  With `average_every` > 0
+    the returned weights are the mean of snapshots taken every that many steps
+    during the decayed phase (Polyak-Ruppert averaging: the jitter of the
+    last Adam steps averages out instead of landing in the expected counts).
+    Count rows are weighted by count_weight · λ_ref^-count_power (λ_ref the
+    CountScale reference): 1 makes every row's Poisson gradient (λ − n)/λ_ref
+    of order one, 0.5 scales it like the row's statistical significance
+    (λ − n)/√λ_ref, so bright rows that are known precisely count for more than
+    faint ones dominated by Poisson noise.
+    Returns the surrogate and a history dict (train/holdout loss)."""
     import torch
 
     from .flow import FlowModel, FlowRows
@@ -428,7 +436,9 @@ def train_surrogate(tables: TableSet, steps: int = 3000, batch: int = 16384, lr:
     model = model or FlowModel(0, seed=seed, n_in=N_FEATURES)
     dev = model.dev
     aux = tables.aux(scale)
-    aux[:, 1] *= np.where(tables.flags & 2, count_weight, 1.0).astype(np.float32)
+    is_cnt = (tables.flags & 2) != 0
+    aux[:, 1] = np.where(is_cnt, count_weight * np.exp(-count_power * aux[:, 0].astype(np.float64)),
+                         aux[:, 1]).astype(np.float32)
     rows = FlowRows.from_numpy(norm(tables.features[train]), tables.t[train], tables.count[train],
                                tables.flags[train], aux[train], device=dev)
     hold_rows = FlowRows.from_numpy(norm(tables.features[hold]), tables.t[hold], tables.count[hold],
@@ -444,11 +454,17 @@ def train_surrogate(tables: TableSet, steps: int = 3000, batch: int = 16384, lr:
     hist["initial_holdout"] = holdout_loss()
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
+    avg, n_avg = None, 0
     for step in range(steps):
         frac = step / max(steps, 1)
-        lr_s = lr if frac < 2 / 3 else lr * 0.1
+        decayed = frac >= 2 / 3
+        lr_s = lr * 0.1 if decayed else lr
         idx = torch.randint(0, rows.n, (min(batch, rows.n),), generator=gen, device=dev)
         model.train_step(rows.slice(idx), lr_s, loss_buf)
+        if average_every and decayed and (steps - 1 - step) % average_every == 0:
+            w = model.get_weights().astype(np.float64)
+            avg = w if avg is None else avg + w
+            n_avg += 1
         if log_every and (step % log_every == 0 or step == steps - 1):
             hist["step"].append(step)
             hist["train"].append(float(loss_buf.cpu()[0]))
@@ -456,6 +472,10 @@ def train_surrogate(tables: TableSet, steps: int = 3000, batch: int = 16384, lr:
     torch.cuda.synchronize(dev)
     hist["seconds"] = time.perf_counter() - t0
     hist["steps"] = steps
+    if n_avg > 1:
+        hist["last_holdout"] = holdout_loss()
+        model.set_weights((avg / n_avg).astype(np.float32))
+        hist["averaged_snapshots"] = n_avg
     hist["final_holdout"] = holdout_loss()
     return Surrogate(model, norm, scale), hist
 
@@ -788,10 +808,10 @@ def direct_rate(engine, medium, spectrum, modules, energy: float = 3e4, photons:
     return h.n_simulated / (time.perf_counter() - t0)
 
 
-def surrogate_study(n_runs: int = 128, photons_per_run: int = 2 * 10 ** 8, train_steps: int = 48000,
-                    n_events: int = 1024, n_error_events: int = 64, error_photons: int = 2 * 10 ** 8,
-                    data_photons: int = 10 ** 7, seed: int = 0, device: int = 0, log=None, lr: float = 2e-3,
-                    count_weight: float = 2.0) -> dict:
+def surrogate_study(n_runs: int = 128, photons_per_run: int = 2 * 10 ** 8, train_steps: int = 192000,
+                    n_events: int = 1024, n_error_events: int = 128, error_photons: int = 2 * 10 ** 8,
+                    data_photons: int = 10 ** 7, seed: int = 0, device: int = 0, log=None, lr: float = 1e-3,
+                    count_weight: float = 2.0, count_power: float = 1.0, batch: int = 16384) -> dict:
     """Tables → training → error against direct propagation → event
     likelihood throughput against direct propagation."""
     from .flow import FlowModel
@@ -805,7 +825,8 @@ def surrogate_study(n_runs: int = 128, photons_per_run: int = 2 * 10 ** 8, train
     tables = generate_tables(eng, med, spec, n_runs, photons_per_run, seed)
     say(f"tables: {len(tables.t)} rows from {tables.info['photons']:.3g} photons in {tables.info['seconds']:.1f} s"
         f" ({tables.info['propagate_seconds']:.1f} s propagation)")
-    sur, hist = train_surrogate(tables, steps=train_steps, seed=seed, lr=lr, count_weight=count_weight)
+    sur, hist = train_surrogate(tables, steps=train_steps, seed=seed, lr=lr, count_weight=count_weight,
+                                count_power=count_power, batch=batch)
     say(f"training: {train_steps} steps in {hist['seconds']:.1f} s, holdout loss {hist['initial_holdout']:.3f}"
         f" -> {hist['final_holdout']:.3f}")
     modules = array_detector()
@@ -872,6 +893,7 @@ def surrogate_study(n_runs: int = 128, photons_per_run: int = 2 * 10 ** 8, train
         "train_steps": train_steps, "train_seconds": hist["seconds"],
         "train_step_ms": hist["seconds"] / max(train_steps, 1) * 1e3,
         "holdout_loss_initial": hist["initial_holdout"], "holdout_loss_final": hist["final_holdout"],
+        "holdout_loss_last_step": hist.get("last_holdout"),
         "error_pairs": err["pairs"], "count_err_median": err["median_abs_rel_err"],
         "count_err_mean": err["mean_abs_rel_err"], "count_bias": err["bias"],
         "count_err_median_untrained": err0["median_abs_rel_err"],
