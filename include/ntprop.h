@@ -326,6 +326,38 @@ int32_t nt_build_cascade_sources(nt_ctx* ctx, const nt_particle* d_particles, in
                                  const nt_cascade_model* model, uint64_t seed, nt_source* d_sources,
                                  void* stream);
 
+/* ---- particle injector ----------------------------------------------------
+   VolumeInjector (injectors.jl:93-111) on the device, one thread per
+   particle, so that a request generator can feed nt_build_cascade_sources
+   without a host round trip.  Particle i draws two Philox4x32-10 blocks with
+   key = seed, counter = (block, i lo, i hi, 0x60000000); uniforms are the
+   words' top 24 bits, u = (w >> 8)·2^-24, and all arithmetic is FP64, rounded
+   to FP32 at the end:
+     block 0 word 0-2  position: cylinder (injectors.jl:59-70)
+                         z = c_z + (u0 − 1/2)·height, r = sqrt(u1·radius)
+                         (as the reference draws it), φ = 2π·u2;
+                       cuboid (injectors.jl:72-78) c + (u − 1/2)·size;
+                       fixed (injectors.jl:57) the centre
+     block 0 word 3    energy, log-uniform: e_min·(e_max/e_min)^u3
+     block 1 word 0-1  direction (UniformAngularDistribution,
+                       injectors.jl:82-89): φ = 2π·u, cos θ = 2u − 1
+     block 1 word 2    type: the first k with u < type_cdf[k]
+   time = 0 (injectors.jl:109).  d_particles is a device array of n records. */
+enum { NT_VOLUME_CYLINDER = 0, NT_VOLUME_CUBOID = 1, NT_VOLUME_FIXED = 2 };
+
+typedef struct nt_injector {         /* 60 bytes */
+    int32_t volume;                  /* NT_VOLUME_* */
+    int32_t reserved;
+    float center[3];                 /* m */
+    float height, radius;            /* cylinder */
+    float size[3];                   /* cuboid edge lengths */
+    float e_min, e_max;              /* GeV; log-uniform, e_min == e_max: fixed */
+    float type_cdf[3];               /* cumulative probabilities of e-, e+, gamma; type_cdf[2] = 1 */
+} nt_injector;
+
+int32_t nt_inject_particles(nt_ctx* ctx, const nt_injector* inj, int64_t n, uint64_t seed,
+                            nt_particle* d_particles, void* stream);
+
 /* ---- device-function evaluation (test hook) ----------------------------
    Runs one device function of the propagation kernel on n inputs (host
    arrays; row-major, `in_width` floats per row) and writes `out_width` floats

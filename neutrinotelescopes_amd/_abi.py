@@ -156,6 +156,13 @@ class nt_cascade_model(C.Structure):
 
 
 NT_PARTICLE_EMINUS, NT_PARTICLE_EPLUS, NT_PARTICLE_GAMMA = 0, 1, 2
+NT_VOLUME_CYLINDER, NT_VOLUME_CUBOID, NT_VOLUME_FIXED = 0, 1, 2
+
+
+class nt_injector(C.Structure):
+    _fields_ = [("volume", C.c_int32), ("reserved", C.c_int32), ("center", C.c_float * 3), ("height", C.c_float),
+                ("radius", C.c_float), ("size", C.c_float * 3), ("e_min", C.c_float), ("e_max", C.c_float),
+                ("type_cdf", C.c_float * 3)]
 
 
 class nt_server_config(C.Structure):
@@ -202,6 +209,8 @@ SIGNATURES = {
     "nt_server_destroy": (None, [C.c_void_p]),
     "nt_eval": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
                             C.c_int64]),
+    "nt_inject_particles": (C.c_int32, [C.c_void_p, C.POINTER(nt_injector), C.c_int64, C.c_uint64, C.c_void_p,
+                                        C.c_void_p]),
     "nt_build_cascade_sources": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(nt_cascade_model),
                                              C.c_uint64, C.c_void_p, C.c_void_p]),
     # include/ntflow.h (flow surrogate)

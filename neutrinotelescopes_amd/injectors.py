@@ -3,6 +3,7 @@
 Reference (paths relative to the reference repository root):
   Cylinder, Cuboid, FixedPosition, VolumeInjector, UniformAngularDistribution
                                                src/event_generation/injectors.jl:30-111
+                                               (host sampling, and nt_inject_particles on the device)
   ExtendedCherenkovEmitter(particle, medium, wl_range; oversample)
                                                src/photon_propagation/lightyield.jl:471-481
 """
@@ -97,6 +98,54 @@ class VolumeInjector:
         t = rng.choice(len(self.types), size=n, p=np.asarray(self.type_probs) / np.sum(self.type_probs))
         d = self.angular_dist.sample(n, rng)
         return [Particle(pos[i], d[i], 0.0, float(e[i]), self.types[t[i]]) for i in range(n)]
+
+
+def injector_to_abi(inj: VolumeInjector) -> _abi.nt_injector:
+    """The injector as the device injector's record (nt_inject_particles):
+    its volume, a log-uniform energy range and the cumulative type
+    probabilities in the order e-, e+, gamma."""
+    if not isinstance(inj.e_dist, LogUniform):
+        raise ValueError("the device injector draws log-uniform energies")
+    r = _abi.nt_injector()
+    v = inj.volume
+    if isinstance(v, Cylinder):
+        r.volume, r.height, r.radius = _abi.NT_VOLUME_CYLINDER, float(v.height), float(v.radius)
+        c = v.center
+    elif isinstance(v, Cuboid):
+        r.volume = _abi.NT_VOLUME_CUBOID
+        r.size[0], r.size[1], r.size[2] = float(v.l_x), float(v.l_y), float(v.l_z)
+        c = v.center
+    elif isinstance(v, FixedPosition):
+        r.volume, c = _abi.NT_VOLUME_FIXED, v.position
+    else:
+        raise ValueError(f"unsupported volume {type(v).__name__}")
+    for q in range(3):
+        r.center[q] = float(c[q])
+    r.e_min, r.e_max = float(inj.e_dist.lo), float(inj.e_dist.hi)
+    p = np.zeros(3)
+    w = np.asarray(inj.type_probs, np.float64) / np.sum(inj.type_probs)
+    for t, pw in zip(inj.types, w):
+        p[_TYPE_CODE[t]] += pw
+    cdf = np.cumsum(p)
+    cdf[-1] = 1.0
+    for q in range(3):
+        r.type_cdf[q] = float(cdf[q])
+    return r
+
+
+def inject_particles_device(inj: VolumeInjector, n: int, seed: int, engine, device: int = 0, stream=None):
+    """n particles of `inj` drawn on the device (nt_inject_particles): a torch
+    uint8 tensor of n nt_particle records in HBM, ready for
+    build_sources_device."""
+    import torch
+    rec = injector_to_abi(inj)
+    dev = torch.device(f"cuda:{device}")
+    out = torch.empty(max(n, 1) * C.sizeof(_abi.nt_particle), dtype=torch.uint8, device=dev)
+    st = stream if stream is not None else torch.cuda.current_stream(dev)
+    _abi.check(engine.lib.nt_inject_particles(engine.handle, C.byref(rec), int(n), C.c_uint64(seed & (2 ** 64 - 1)),
+                                              C.c_void_p(out.data_ptr()), C.c_void_p(st.cuda_stream)),
+               "nt_inject_particles")
+    return out[:n * C.sizeof(_abi.nt_particle)]
 
 
 def cascade_model(medium: WaterProperties, wl_range=(300.0, 800.0), oversample: float = 1.0) -> _abi.nt_cascade_model:

@@ -58,6 +58,7 @@ def test_struct_layouts():
     assert C.sizeof(_abi.nt_target) == 160
     assert _abi.nt_hits.capacity.offset == 12 * 8
     assert C.sizeof(_abi.nt_particle) == 40 and C.sizeof(_abi.nt_cascade_model) == 24
+    assert C.sizeof(_abi.nt_injector) == 60
     assert _abi.nt_particle.type.offset == 32
 
 
@@ -65,7 +66,8 @@ def test_struct_layouts():
 def test_struct_layouts_match_c_compiler(tmp_path):
     """sizeof and field offsets of every ABI struct as a C compiler lays them
     out from the header, against the ctypes mirror."""
-    structs = ["nt_source", "nt_medium", "nt_target", "nt_spectrum", "nt_hits", "nt_particle", "nt_cascade_model"]
+    structs = ["nt_source", "nt_medium", "nt_target", "nt_spectrum", "nt_hits", "nt_particle", "nt_cascade_model",
+               "nt_injector"]
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ntprop.h"', "int main(void) {"]
     for st in structs:
         lines.append(f'printf("{st} %zu\\n", sizeof({st}));')

@@ -116,3 +116,85 @@ def test_built_sources_propagate(engine):
     np.testing.assert_array_equal(a.photon_id[order_a], b.photon_id[order_b])
     np.testing.assert_array_equal(a.time[order_a], b.time[order_b])
     np.testing.assert_array_equal(a.position[order_a], b.position[order_b])
+
+
+def _injector(volume):
+    from neutrinotelescopes_amd import PEMinus, PEPlus, PGamma
+    from neutrinotelescopes_amd import injectors as I
+    vol = {"cylinder": I.Cylinder(np.array([10.0, -5.0, -100.0]), 300.0, 150.0),
+           "cuboid": I.Cuboid(np.array([1.0, 2.0, 3.0]), 20.0, 40.0, 60.0),
+           "fixed": I.FixedPosition(np.array([0.0, 0.0, 15.0]))}[volume]
+    return I.VolumeInjector(vol, I.LogUniform(1e2, 1e6), (PEMinus, PEPlus, PGamma), (1.0, 1.0, 2.0))
+
+
+@pytest.mark.parametrize("volume", ["cylinder", "cuboid", "fixed"])
+def test_device_injector_matches_oracle(engine, volume):
+    """nt_inject_particles against the FP64 oracle on the same Philox words:
+    positions, directions and energies equal after FP32 rounding (but for
+    values within FP64 libm differences of a rounding boundary), types equal."""
+    from neutrinotelescopes_amd import injectors as I
+    from oracle import injector as J
+    from _oracle import philox
+    inj = _injector(volume)
+    rec = I.injector_to_abi(inj)
+    n, seed = 3000, 0xABCDEF0123
+    print("injector:", volume, "n", n, "seed", hex(seed), flush=True)
+    p = I.inject_particles_device(inj, n, seed, engine).cpu().numpy().view(I.PARTICLE_DTYPE)
+    o = J.inject({"volume": rec.volume, "center": list(rec.center), "height": rec.height, "radius": rec.radius,
+                  "size": list(rec.size), "e_min": rec.e_min, "e_max": rec.e_max, "type_cdf": list(rec.type_cdf)},
+                 n, seed, philox)
+    o32 = o.astype(np.float32)
+    same = ((p["position"] == o32[:, :3]).all(1) & (p["direction"] == o32[:, 3:6]).all(1)
+            & (p["energy"] == o32[:, 7]))
+    print("  identical records", same.mean())
+    assert same.mean() >= 0.999
+    np.testing.assert_allclose(p["position"], o[:, :3], rtol=1e-6, atol=1e-4)
+    np.testing.assert_allclose(p["direction"], o[:, 3:6], atol=1e-6)
+    np.testing.assert_allclose(p["energy"], o[:, 7], rtol=1e-6)
+    assert np.array_equal(p["type"], o[:, 8].astype(np.int32)) and np.all(p["time"] == 0)
+
+
+def test_device_injector_law_at_full_size(engine):
+    """2^20 particles: r²/radius, z, cos θ, φ and ln E uniform, type
+    frequencies 1:1:2 (injectors.jl:59-111), directions of unit length."""
+    from scipy import stats
+    from neutrinotelescopes_amd import injectors as I
+    n = 1 << 20
+    p = I.inject_particles_device(_injector("cylinder"), n, 5, engine).cpu().numpy().view(I.PARTICLE_DTYPE)
+    x = p["position"].astype(np.float64) - [10.0, -5.0, -100.0]
+    d = p["direction"].astype(np.float64)
+    sub = slice(0, 200000)
+    assert stats.kstest(((x[:, 0] ** 2 + x[:, 1] ** 2) / 150.0)[sub], "uniform").pvalue > 1e-3
+    assert stats.kstest((x[:, 2] / 300.0 + 0.5)[sub], "uniform").pvalue > 1e-3
+    assert stats.kstest(((d[:, 2] + 1) / 2)[sub], "uniform").pvalue > 1e-3
+    assert stats.kstest(((np.arctan2(d[:, 1], d[:, 0]) + np.pi) / (2 * np.pi))[sub], "uniform").pvalue > 1e-3
+    assert stats.kstest((np.log(p["energy"].astype(np.float64) / 1e2) / np.log(1e4))[sub], "uniform").pvalue > 1e-3
+    assert np.abs(np.linalg.norm(d, axis=1) - 1).max() < 1e-6
+    frac = np.bincount(p["type"], minlength=3) / n
+    np.testing.assert_allclose(frac, [0.25, 0.25, 0.5], atol=5 * math.sqrt(0.25 / n))
+
+
+def test_request_chain_on_device(engine):
+    """Request generator: particles injected and turned into sources on the
+    device, then propagated; the hits belong to the injected cascades."""
+    from neutrinotelescopes_amd import (CherenkovSpectrum, ExtendedCherenkovEmitter, LongitudinalParameterisation,
+                                        PhotonPropSetup, propagate_photons)
+    from neutrinotelescopes_amd.detection import MultiPMTDetector, make_pom_pmt_coordinates
+    from neutrinotelescopes_amd import injectors as I
+    med = _med()
+    inj = I.VolumeInjector(I.Cylinder(np.array([0.0, 0.0, 0.0]), 20.0, 100.0), I.LogUniform(1.0, 3.0))
+    n = 16
+    parts = I.inject_particles_device(inj, n, 11, engine)
+    src = I.build_sources_device(parts, n, I.cascade_model(med), 12, engine).cpu().numpy().view(I.SOURCE_DTYPE)
+    pv = parts.cpu().numpy().view(I.PARTICLE_DTYPE)
+    assert np.array_equal(src["position"], pv["position"]) and np.all(src["photons"] > 10 ** 5)
+    emitters = [ExtendedCherenkovEmitter(r["position"].copy(), r["direction"].copy(), float(r["time"]),
+                                         int(r["photons"]), LongitudinalParameterisation(
+                                             float(r["long_a"]), float(r["long_b"]), float(r["long_lrad"])))
+                for r in src]
+    tgt = MultiPMTDetector(np.zeros(3, np.float32), 0.21, math.pi * 0.0375 ** 2, make_pom_pmt_coordinates(), 1)
+    hits = propagate_photons(PhotonPropSetup(emitters, [tgt], med, CherenkovSpectrum((300.0, 800.0), 30, med),
+                                             seed=13))
+    print("chain: photons", int(src["photons"].sum()), "hits", len(hits.time))
+    assert len(hits.time) > 0 and hits.source_index.max() < n
+    assert hits.n_simulated == int(src["photons"].sum())

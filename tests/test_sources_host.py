@@ -133,3 +133,53 @@ def test_particles_to_abi_layout():
     assert raw.nbytes == 2 * C.sizeof(_abi.nt_particle)
     assert list(arr[0].position) == [1.0, 2.0, 3.0] and arr[0].time == 4.5 and arr[0].energy == 1e3
     assert arr[0].type == _abi.NT_PARTICLE_GAMMA and arr[1].type == _abi.NT_PARTICLE_EPLUS
+
+
+def _rec(volume=0, center=(0.0, 0.0, -100.0), height=200.0, radius=400.0, size=(0.0, 0.0, 0.0), e=(1e2, 1e6),
+         cdf=(0.25, 0.25, 1.0)):
+    return {"volume": volume, "center": center, "height": height, "radius": radius, "size": size,
+            "e_min": e[0], "e_max": e[1], "type_cdf": cdf}
+
+
+def test_injector_oracle_follows_the_reference_law():
+    """The device injector's FP64 restatement draws the same law as the host
+    VolumeInjector (injectors.jl:59-111): z uniform over the height,
+    r² / radius uniform (r = sqrt(u·radius)), cos θ and φ uniform, ln E
+    uniform, types by their probabilities."""
+    from oracle import injector as J
+    n = 20000
+    x = J.inject(_rec(), n, 77, philox)
+    r2 = x[:, 0] ** 2 + x[:, 1] ** 2
+    assert np.all(np.abs(x[:, 2] + 100.0) <= 100.0)
+    assert stats.kstest(r2 / 400.0, "uniform").pvalue > 1e-3
+    assert stats.kstest((np.arctan2(x[:, 1], x[:, 0]) + np.pi) / (2 * np.pi), "uniform").pvalue > 1e-3
+    np.testing.assert_allclose(np.linalg.norm(x[:, 3:6], axis=1), 1.0, atol=1e-12)
+    assert stats.kstest((x[:, 5] + 1) / 2, "uniform").pvalue > 1e-3
+    assert stats.kstest(np.log(x[:, 7] / 1e2) / np.log(1e4), "uniform").pvalue > 1e-3
+    assert abs(np.mean(x[:, 8] == 2) - 0.75) < 5 * math.sqrt(0.75 * 0.25 / n)
+    assert np.all(x[:, 8] != 1) and np.all(x[:, 6] == 0)
+    # the host mirror draws the same law (two-sample KS)
+    inj = I.VolumeInjector(I.Cylinder(np.array([0.0, 0.0, -100.0]), 200.0, 400.0), I.LogUniform(1e2, 1e6),
+                           (PEMinus, PGamma), (0.25, 0.75))
+    ps = inj.sample(n, np.random.default_rng(8))
+    pos = np.array([p.position for p in ps])
+    assert stats.ks_2samp(pos[:, 0] ** 2 + pos[:, 1] ** 2, r2).pvalue > 1e-3
+    assert stats.ks_2samp(pos[:, 2], x[:, 2]).pvalue > 1e-3
+    assert stats.ks_2samp(np.log([p.energy for p in ps]), np.log(x[:, 7])).pvalue > 1e-3
+    # cuboid and fixed position
+    xc = J.inject(_rec(volume=1, center=(1.0, 2.0, 3.0), size=(2.0, 4.0, 6.0)), 2000, 3, philox)
+    assert np.all(np.abs(xc[:, :3] - [1, 2, 3]) <= [1, 2, 3])
+    xf = J.inject(_rec(volume=2, center=(5.0, 0.0, 0.0), e=(10.0, 10.0)), 50, 3, philox)
+    assert np.all(xf[:, :3] == [5.0, 0.0, 0.0]) and np.allclose(xf[:, 7], 10.0)
+
+
+def test_injector_record_from_host_injector():
+    from neutrinotelescopes_amd import _abi
+    inj = I.VolumeInjector(I.Cylinder(np.array([1.0, 2.0, 3.0]), 50.0, 20.0), I.LogUniform(1e3, 1e5),
+                           (PGamma, PEPlus), (3.0, 1.0))
+    r = I.injector_to_abi(inj)
+    assert r.volume == _abi.NT_VOLUME_CYLINDER and r.height == 50.0 and r.radius == 20.0
+    assert list(r.center) == [1.0, 2.0, 3.0] and (r.e_min, r.e_max) == (1e3, 1e5)
+    np.testing.assert_allclose(list(r.type_cdf), [0.0, 0.25, 1.0])
+    with pytest.raises(ValueError):
+        I.injector_to_abi(I.VolumeInjector(I.Cylinder(np.zeros(3), 1.0, 1.0), "not log-uniform"))
