@@ -188,6 +188,14 @@ int32_t nt_set_targets(nt_ctx* ctx, const nt_target* targets, int32_t n_targets)
    here one launch for all). */
 int32_t nt_set_sources(nt_ctx* ctx, const nt_source* sources, int32_t n_sources);
 
+/* nt_set_sources for records already in device memory (e.g. written by
+   nt_build_cascade_sources): the same validation and preparation on the
+   device, in stream order, then one 16-byte read-back (total photons and an
+   error flag).  kind_hint: the common NT_SRC_* of all records (selects the
+   kernel variant), or -1 when kinds are mixed or unknown. */
+int32_t nt_set_sources_device(nt_ctx* ctx, const nt_source* d_sources, int32_t n_sources, int32_t kind_hint,
+                              void* stream);
+
 /* Synchronous drop-in for run_photon_prop_no_local_cache
    (photon_prop_cuda.jl:726-759): propagate every photon of the sources set by
    nt_set_sources and copy the hits into caller-owned HOST columns.

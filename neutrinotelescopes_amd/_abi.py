@@ -193,6 +193,7 @@ SIGNATURES = {
     "nt_set_spectrum": (C.c_int32, [C.c_void_p, C.POINTER(nt_spectrum)]),
     "nt_set_targets": (C.c_int32, [C.c_void_p, C.POINTER(nt_target), C.c_int32]),
     "nt_set_sources": (C.c_int32, [C.c_void_p, C.POINTER(nt_source), C.c_int32]),
+    "nt_set_sources_device": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]),
     "nt_propagate": (C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(nt_hits), C.POINTER(C.c_int64),
                                  C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "nt_launch": (C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(nt_hits), C.c_void_p, C.c_void_p]),

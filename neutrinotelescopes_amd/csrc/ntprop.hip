@@ -905,6 +905,70 @@ int32_t rebuild_epilogue(nt_ctx* c) {
 int64_t round_up64(int64_t n) { return (n + 63) & ~int64_t(63); }
 
 // Device record of an nt_source (offset, key, base and slot left at 0).
+// nt_set_sources_device: per record the preparation of make_dev_source (same
+// FP32 operations, IEEE square root and division), photon counts rounded up
+// to multiples of 64 into offs[i + 1]; err[0] = 1 + index of an invalid record.
+__global__ __launch_bounds__(256) void prepare_sources_kernel(const nt_source* __restrict__ in, int n,
+                                                             DevSource* __restrict__ out, int64_t* offs,
+                                                             int* err) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) offs[0] = 0;
+    if (i >= n) return;
+    const nt_source a = in[i];
+    DevSource d{};
+    bool ok = a.photons >= 0 && a.photons <= (int64_t(1) << 56) && a.kind >= NT_SRC_POINTLIKE_ISOTROPIC &&
+              a.kind <= NT_SRC_POINTLIKE_CHERENKOV;
+    d.kind = a.kind;
+    d.rng_index = (uint32_t)i;
+    d.photons = a.photons;
+    for (int k = 0; k < 3; ++k) { d.pos[k] = a.position[k]; d.dir[k] = a.direction[k]; }
+    d.t0 = a.time;
+    d.t1 = a.time_end;
+    d.angle = a.angle;
+    if (ok && a.kind == NT_SRC_EXTENDED_CHERENKOV) {
+        ok = a.long_a > 1.0f && a.long_b > 0.0f && a.long_lrad > 0.0f;
+        d.gamma_d = (float)((double)a.long_a - 1.0 / 3.0);
+        d.gamma_c = 1.0f / sqrtf(9.0f * d.gamma_d);
+        d.gamma_scale = 1.0f / a.long_b;
+        d.lrad = a.long_lrad;
+    }
+    if (!ok) {
+        atomicMin(err, i + 1);
+        d.photons = 0;
+    }
+    out[i] = d;
+    offs[i + 1] = ok ? ((a.photons + 63) & ~int64_t(63)) : 0;
+}
+
+// In-place inclusive scan of offs[1..n] by one block (chunks per thread, a
+// block scan of the chunk sums, then the chunks again), then each record's
+// offset; total[0] = offs[n].
+__global__ __launch_bounds__(1024) void scan_offsets_kernel(int64_t* offs, int n, DevSource* __restrict__ recs,
+                                                           int64_t* total) {
+    __shared__ int64_t part[1024];
+    const int t = threadIdx.x;
+    const int chunk = (n + 1023) / 1024;
+    const int lo = 1 + t * chunk, hi = min(n + 1, lo + chunk);
+    int64_t sum = 0;
+    for (int j = lo; j < hi; ++j) sum += offs[j];
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int64_t v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int64_t run = t ? part[t - 1] : 0;
+    for (int j = lo; j < hi; ++j) {
+        run += offs[j];
+        offs[j] = run;
+    }
+    __syncthreads();
+    for (int j = t; j < n; j += 1024) recs[j].offset = offs[j];
+    if (t == 0) total[0] = part[1023];
+}
+
 int32_t make_dev_source(const nt_source& a, int i, DevSource& d) {
     if (a.photons < 0 || a.photons > (int64_t(1) << 56) || a.kind < NT_SRC_POINTLIKE_ISOTROPIC ||
         a.kind > NT_SRC_POINTLIKE_CHERENKOV) {
@@ -1250,6 +1314,49 @@ int32_t nt_set_targets(nt_ctx* c, const nt_target* t, int32_t n) {
     c->host_targets.assign(t, t + n);
     c->has_targets = true;
     if (c->epi_enabled) return rebuild_epilogue(c);
+    return NT_OK;
+}
+
+int32_t nt_set_sources_device(nt_ctx* c, const nt_source* d_src, int32_t n, int32_t kind_hint, void* stream) {
+    if (!c || n < 0 || (n > 0 && !d_src) || kind_hint < -1 || kind_hint > NT_SRC_POINTLIKE_CHERENKOV)
+        return NT_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    int64_t cap = c->sources_cap, cap2 = c->sources_cap ? c->sources_cap + 1 : 0;
+    int32_t st = grow(&c->d_sources, std::max(n, 1), &cap);
+    if (st) return st;
+    st = grow(&c->d_offsets, (int64_t)n + 1, &cap2);
+    if (st) return st;
+    c->sources_cap = (int)std::min(cap, cap2 - 1);
+    hipStream_t s = (hipStream_t)stream;
+    // error flag and total: the context's counter buffer is free between launches
+    int64_t* scratch = c->d_counters;
+    int* err = reinterpret_cast<int*>(scratch);
+    const int32_t none = 0x7fffffff;
+    HIP_TRY(hipMemcpyAsync(err, &none, sizeof none, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(prepare_sources_kernel, dim3((std::max(n, 1) + 255) / 256), dim3(256), 0, s, d_src, n,
+                       c->d_sources, c->d_offsets, err);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(scan_offsets_kernel, dim3(1), dim3(1024), 0, s, c->d_offsets, n, c->d_sources,
+                       scratch + 1);
+    HIP_TRY(hipGetLastError());
+    int64_t back[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(back, scratch, sizeof back, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    int32_t bad;
+    std::memcpy(&bad, &back[0], sizeof bad);
+    if (bad != none) {
+        snprintf(g_err, sizeof g_err, "source %d: photons must be in [0, 2^56], kind known, and extended "
+                 "Cherenkov longitudinal parameters a > 1, b > 0, lrad > 0", bad - 1);
+        c->n_sources = 0;
+        c->total = 0;
+        c->has_sources = false;
+        return NT_ERR_ARG;
+    }
+    c->n_sources = n;
+    c->total = back[1];
+    c->variant = kind_hint >= 0 ? kind_hint : kKernelVariants - 1;
+    c->has_sources = true;
     return NT_OK;
 }
 

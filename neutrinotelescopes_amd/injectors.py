@@ -201,6 +201,19 @@ def build_sources_device(d_particles, n: int, model: _abi.nt_cascade_model, seed
     return d_out[:n * C.sizeof(_abi.nt_source)]
 
 
+def set_sources_device(engine, d_sources, n: int, kind_hint: int = _abi.NT_SRC_EXTENDED_CHERENKOV, stream=None):
+    """Make n nt_source records in device memory (a torch uint8 tensor, e.g.
+    from build_sources_device) the engine's sources without a host copy
+    (nt_set_sources_device)."""
+    import torch
+    if d_sources.dtype != torch.uint8 or d_sources.numel() != n * C.sizeof(_abi.nt_source):
+        raise ValueError("d_sources must hold n nt_source records as uint8")
+    st = stream if stream is not None else torch.cuda.current_stream(d_sources.device)
+    _abi.check(engine.lib.nt_set_sources_device(engine.handle, C.c_void_p(d_sources.data_ptr()), int(n),
+                                                int(kind_hint), C.c_void_p(st.cuda_stream)),
+               "nt_set_sources_device")
+
+
 def build_cascade_sources(particles, medium: WaterProperties, wl_range=(300.0, 800.0), oversample: float = 1.0,
                           seed: int = 0, device: int = 0, engine=None) -> list:
     """ExtendedCherenkovEmitter for every particle, built on the GPU

@@ -198,3 +198,41 @@ def test_request_chain_on_device(engine):
     print("chain: photons", int(src["photons"].sum()), "hits", len(hits.time))
     assert len(hits.time) > 0 and hits.source_index.max() < n
     assert hits.n_simulated == int(src["photons"].sum())
+
+
+def test_sources_set_from_device_equal_host_path(engine):
+    """nt_set_sources_device prepares the kernel's source records on the
+    device exactly as nt_set_sources does on the host: identical hits for the
+    same seed; an invalid record is rejected with its index."""
+    import torch
+    from neutrinotelescopes_amd import _abi, CherenkovSpectrum
+    from neutrinotelescopes_amd.detection import MultiPMTDetector, make_pom_pmt_coordinates
+    from neutrinotelescopes_amd.propagation import PropagationEngine
+    from neutrinotelescopes_amd import injectors as I
+    med = _med()
+    eng = PropagationEngine(0)
+    try:
+        inj = I.VolumeInjector(I.Cylinder(np.zeros(3), 20.0, 100.0), I.LogUniform(1.0, 3.0))
+        n = 40
+        parts = I.inject_particles_device(inj, n, 21, eng)
+        d_src = I.build_sources_device(parts, n, I.cascade_model(med), 22, eng)
+        eng.set_medium(med)
+        eng.set_spectrum(CherenkovSpectrum((300.0, 800.0), 30, med))
+        eng.set_targets([MultiPMTDetector(np.zeros(3, np.float32), 0.21, math.pi * 0.0375 ** 2,
+                                          make_pom_pmt_coordinates(), 1)])
+        I.set_sources_device(eng, d_src, n)
+        torch.cuda.synchronize()
+        a = eng.propagate(seed=23).sorted()
+        arr = (_abi.nt_source * n).from_buffer_copy(d_src.cpu().numpy().tobytes())
+        _abi.check(eng.lib.nt_set_sources(eng.handle, arr, n), "nt_set_sources")
+        b = eng.propagate(seed=23).sorted()
+        print("device-set sources: hits", len(a.time), "simulated", a.n_simulated)
+        assert len(a.time) > 0 and a.n_simulated == b.n_simulated == sum(r.photons for r in arr)
+        for col in ("photon_id", "source_index", "time", "position", "direction", "wavelength"):
+            np.testing.assert_array_equal(getattr(a, col), getattr(b, col))
+        bad = d_src.clone()
+        bad.view(torch.int64).view(n, 8)[7, 1] = -5            # photons of record 7
+        with pytest.raises(RuntimeError, match="source 7"):
+            I.set_sources_device(eng, bad, n)
+    finally:
+        eng.close()
