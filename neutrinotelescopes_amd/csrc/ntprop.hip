@@ -1196,6 +1196,7 @@ int32_t nt_set_spectrum(nt_ctx* c, const nt_spectrum* s) {
             snprintf(g_err, sizeof g_err, "spectrum table needs 2..%d knots and a valid addressing", NT_MAX_KNOTS);
             return NT_ERR_ARG;
         }
+        c->has_spectrum = false;             // knots change from here on (see nt_set_targets)
         int64_t cap = c->knots_cap;
         int32_t st = grow(&c->d_knots, s->n_knots, &cap);
         if (st) return st;
@@ -1248,6 +1249,9 @@ int32_t nt_set_targets(nt_ctx* c, const nt_target* t, int32_t n) {
     }
     std::lock_guard<std::mutex> lk(c->mu);
     HIP_TRY(hipSetDevice(c->device));
+    // buffers change from here on: a failure below leaves the context without
+    // targets rather than with freed or half-written tables
+    c->has_targets = false;
     int64_t cap = c->targets_cap, cap2 = c->targets_cap;
     int32_t st = grow(&c->d_targets, n, &cap);
     if (st) return st;
@@ -1322,6 +1326,7 @@ int32_t nt_set_sources_device(nt_ctx* c, const nt_source* d_src, int32_t n, int3
         return NT_ERR_ARG;
     std::lock_guard<std::mutex> lk(c->mu);
     HIP_TRY(hipSetDevice(c->device));
+    c->has_sources = false;                  // records change from here on (see nt_set_targets)
     int64_t cap = c->sources_cap, cap2 = c->sources_cap ? c->sources_cap + 1 : 0;
     int32_t st = grow(&c->d_sources, std::max(n, 1), &cap);
     if (st) return st;
@@ -1373,6 +1378,7 @@ int32_t nt_set_sources(nt_ctx* c, const nt_source* s, int32_t n) {
     }
     std::lock_guard<std::mutex> lk(c->mu);
     HIP_TRY(hipSetDevice(c->device));
+    c->has_sources = false;                  // records change from here on (see nt_set_targets)
     int64_t cap = c->sources_cap, cap2 = c->sources_cap ? c->sources_cap + 1 : 0;
     int32_t st = grow(&c->d_sources, std::max(n, 1), &cap);
     if (st) return st;
