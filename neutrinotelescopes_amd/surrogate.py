@@ -811,7 +811,8 @@ def direct_rate(engine, medium, spectrum, modules, energy: float = 3e4, photons:
 def surrogate_study(n_runs: int = 128, photons_per_run: int = 2 * 10 ** 8, train_steps: int = 192000,
                     n_events: int = 1024, n_error_events: int = 128, error_photons: int = 2 * 10 ** 8,
                     data_photons: int = 10 ** 7, seed: int = 0, device: int = 0, log=None, lr: float = 1e-3,
-                    count_weight: float = 2.0, count_power: float = 1.0, batch: int = 16384) -> dict:
+                    count_weight: float = 2.0, count_power: float = 1.0, batch: int = 16384,
+                    orientations: int = 4) -> dict:
     """Tables → training → error against direct propagation → event
     likelihood throughput against direct propagation."""
     from .flow import FlowModel
@@ -822,7 +823,7 @@ def surrogate_study(n_runs: int = 128, photons_per_run: int = 2 * 10 ** 8, train
     med = make_cascadia_medium_properties(0.95)
     spec = CherenkovSpectrum(WL_RANGE, 30, med)
     eng = PropagationEngine(device)
-    tables = generate_tables(eng, med, spec, n_runs, photons_per_run, seed)
+    tables = generate_tables(eng, med, spec, n_runs, photons_per_run, seed, orientations=orientations)
     say(f"tables: {len(tables.t)} rows from {tables.info['photons']:.3g} photons in {tables.info['seconds']:.1f} s"
         f" ({tables.info['propagate_seconds']:.1f} s propagation)")
     sur, hist = train_surrogate(tables, steps=train_steps, seed=seed, lr=lr, count_weight=count_weight,
