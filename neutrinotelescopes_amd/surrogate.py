@@ -327,14 +327,17 @@ def generate_tables(engine, medium, spectrum, n_runs: int, photons_per_run: int,
         t_prop += time.perf_counter() - tp
         photons_total += hits.n_simulated
         scale = cascade_photons(e, medium) / hits.n_simulated
-        mod = hits.module_id.astype(np.int64) - 1
-        pos, tres, w, mod_k = thin_hits(mod, len(positions), max_hits, rng, hits.position, hits.tres,
-                                        hits.total_weight)
+        # hits arrive in device append order; photon order makes the thinning,
+        # the time-row draws and so the whole table a function of the seed
+        order = np.argsort(hits.photon_id)
+        mod = hits.module_id[order].astype(np.int64) - 1
+        hpos, htres, hw = hits.position[order], hits.tres[order], hits.total_weight[order]
+        pos, tres, w, mod_k = thin_hits(mod, len(positions), max_hits, rng, hpos, htres, hw)
         ts = rows_from_hits(pos.astype(np.float64), tres.astype(np.float64), w, mod_k, positions, np.zeros(3), u, le,
                             scale, rng, orientations, time_rows, max_hits)
         parts.append(ts)
         if export is not None:
-            export.extend(_reference_layout(hits, mod, positions, u, e, scale))
+            export.extend(_reference_layout(hpos, htres, hw, mod, positions, u, e, scale))
     out = TableSet(*(np.concatenate([getattr(p, k) for p in parts]) for k in ("features", "t", "count", "flags")))
     out.info = {"runs": n_runs, "photons_per_run": photons_per_run, "photons": photons_total,
                 "seconds": time.perf_counter() - t0, "propagate_seconds": t_prop,
@@ -342,7 +345,7 @@ def generate_tables(engine, medium, spectrum, n_runs: int, photons_per_run: int,
     return out
 
 
-def _reference_layout(hits, mod, positions, u, energy, scale):
+def _reference_layout(hpos, htres, hw, mod, positions, u, energy, scale):
     """Per-module 'photons' datasets of photon_tables.jl:111-120 in the
     reference's frame (module at the origin, cascade on its +z axis)."""
     from .detection import cart_to_sph
@@ -360,12 +363,12 @@ def _reference_layout(hits, mod, positions, u, energy, scale):
         else:
             vx = np.array([[0, -v[2], v[1]], [v[2], 0, -v[0]], [-v[1], v[0], 0]])
             R = np.eye(3) + vx + vx @ vx * (1 - c) / (v @ v)
-        pos = (hits.position[sel] - positions[m]) @ R.T
+        pos = (hpos[sel] - positions[m]) @ R.T
         ur = R @ u
         th, ph = cart_to_sph(*ur)
         out.append({"attrs": {"energy": energy, "distance": d, "dir_costheta": float(math.cos(th)),
                               "dir_phi": float(ph)},
-                    "data": np.column_stack([hits.tres[sel], pos, hits.total_weight[sel] * scale])})
+                    "data": np.column_stack([htres[sel], pos, hw[sel] * scale])})
     return out
 
 
