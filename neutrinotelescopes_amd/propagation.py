@@ -121,6 +121,7 @@ class PropagationEngine:
 
     def __init__(self, device: int = 0):
         self.lib = _abi.load_library()
+        self.device = int(device)
         h = C.c_void_p()
         _abi.check(self.lib.nt_create(int(device), C.byref(h)), "nt_create")
         self.handle = h
