@@ -173,8 +173,6 @@ __device__ __forceinline__ int first_hit(const KParams& p, float3 pos, float3 di
     const const_f4_ptr cl = (const_f4_ptr)(const void*)p.clusters;
     const const_f4_ptr mem = (const_f4_ptr)(const void*)p.members;
     const const_u32_ptr midx = (const_u32_ptr)(const void*)p.member_idx;
-    const float z1 = __builtin_fmaf(dir.z, step, pos.z);
-    const float zmin = fminf(pos.z, z1), zmax = fmaxf(pos.z, z1);
     const float dx = dir.x * step, dy = dir.y * step;
     const float dd = __builtin_fmaf(dx, dx, dy * dy);
     const float inv_dd = dd > 0.0f ? __builtin_amdgcn_rcpf(dd) : 0.0f;
@@ -186,11 +184,15 @@ __device__ __forceinline__ int first_hit(const KParams& p, float3 pos, float3 di
         if (first < 0 || count < 0 || first + count > p.n_targets) continue;  // corrupt table guard
         ++ctests;
         const float re = __builtin_fmaf(step, 0x1p-10f, c1.x);
-        if (zmax < c0.z - re || zmin > c0.w + re) continue;
+        // distance of the step's xy projection from the string axis first:
+        // for strings long compared with a step it is the selective test
         const float wx = c0.x - pos.x, wy = c0.y - pos.y;
         const float t = fminf(fmaxf(__builtin_fmaf(wx, dx, wy * dy) * inv_dd, 0.0f), 1.0f);
         const float ex = __builtin_fmaf(-t, dx, wx), ey = __builtin_fmaf(-t, dy, wy);
         if (__builtin_fmaf(ex, ex, ey * ey) > re * re) continue;
+        const float z1 = __builtin_fmaf(dir.z, step, pos.z);
+        const float zmin = fminf(pos.z, z1), zmax = fmaxf(pos.z, z1);
+        if (zmax < c0.z - re || zmin > c0.w + re) continue;
         for (int k = first; k < first + count; ++k) {
             const f4 mv = mem[k];
             if (zmax < mv.z - re || zmin > mv.z + re) continue;
