@@ -383,7 +383,9 @@ __global__ __launch_bounds__(kThreads, NTP_MIN_WAVES_PER_SIMD) void propagate_ke
     float* s_knots = lds;                                          // n_knots (padded to 4)
     uint32_t* s_stage = reinterpret_cast<uint32_t*>(lds + ((p.spectrum.n + 3) & ~3));
 
+    __shared__ double s_poly[kScaPolyCoefs];                       // scattering-length fit coefficients
     for (int i = threadIdx.x; i < p.spectrum.n; i += kThreads) s_knots[i] = p.knots[i];
+    if (threadIdx.x < kScaPolyCoefs) s_poly[threadIdx.x] = p.sca_poly.c[threadIdx.x];
     __syncthreads();
 
     const int lane = threadIdx.x & 63;
@@ -463,7 +465,7 @@ __global__ __launch_bounds__(kThreads, NTP_MIN_WAVES_PER_SIMD) void propagate_ke
                     break;
             }
             idir = dir;
-            const float sca_len = scattering_length_fit(wl, &kp->sca_poly, mi);
+            const float sca_len = scattering_length_fit(wl, &kp->sca_poly, s_poly, mi);
             const float c_grp = group_velocity(wl, mi);
 
             // Scattering steps read the Philox stream of blocks 1, 2, ... three

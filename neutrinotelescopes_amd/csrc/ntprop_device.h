@@ -193,15 +193,21 @@ struct WlPoly {
     double c[kScaPolyCoefs];         // monomial coefficients in t (unused ones zero)
 };
 
-template <typename P>
-__device__ __forceinline__ float scattering_length_fit(float wl, P poly, const Medium& m) {
+// `coef` may be a copy of poly->c (the kernel keeps one in LDS).
+template <typename P, typename Cf>
+__device__ __forceinline__ float scattering_length_fit(float wl, P poly, Cf coef, const Medium& m) {
     if (!poly->valid || !(wl >= poly->lo && wl <= poly->hi)) return scattering_length(wl, m);
     const double t = __builtin_fma((double)wl, poly->ta, poly->tb);
-    double y = poly->c[kScaPolyCoefs - 1];
+    double y = coef[kScaPolyCoefs - 1];
 #pragma unroll
-    for (int j = kScaPolyCoefs - 2; j >= 0; --j) y = __builtin_fma(y, t, poly->c[j]);
+    for (int j = kScaPolyCoefs - 2; j >= 0; --j) y = __builtin_fma(y, t, coef[j]);
     const float l = (float)y;
     return m.sca_len_scale == 1.0f ? l : l * m.sca_len_scale;
+}
+
+template <typename P>
+__device__ __forceinline__ float scattering_length_fit(float wl, P poly, const Medium& m) {
+    return scattering_length_fit(wl, poly, poly->c, m);
 }
 
 // ---------------------------------------------------------------------------
