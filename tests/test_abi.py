@@ -97,3 +97,7 @@ def test_null_arguments_rejected_without_device():
     assert lib.nt_set_medium(None, None) == _abi.NT_ERR_ARG
     assert lib.nt_create(0, None) == _abi.NT_ERR_ARG
     assert lib.nt_eval(None, 0, None, 1, None, 1, 1) == _abi.NT_ERR_ARG
+    inj, model = _abi.nt_injector(), _abi.nt_cascade_model()
+    assert lib.nt_inject_particles(None, C.byref(inj), 1, 0, None, None) == _abi.NT_ERR_ARG
+    assert lib.nt_build_cascade_sources(None, None, 1, C.byref(model), 0, None, None) == _abi.NT_ERR_ARG
+    assert lib.nt_set_sources_device(None, None, 1, -1, None) == _abi.NT_ERR_ARG
