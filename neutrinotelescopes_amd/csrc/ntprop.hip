@@ -475,7 +475,6 @@ __global__ __launch_bounds__(kThreads, NTP_MIN_WAVES_PER_SIMD) void propagate_ke
             // step's words are fixed registers; the block index is wave-uniform.
             // A step returns true when the photon hit a target (state at the hit).
             auto scatter_step = [&](uint32_t wx, uint32_t wy, uint32_t wz) -> bool {
-                const uint32_t ww = ((wx & 0xFFu) << 8) | ((wy & 0xFFu) << 16) | ((wz & 0xFFu) << 24);
                 const float step = -log_unit(u01_oc(wx)) * sca_len;
                 // first listed target hit within this step (photon_prop_cuda.jl:457-468)
                 float dt = 0.0f;
@@ -483,7 +482,12 @@ __global__ __launch_bounds__(kThreads, NTP_MIN_WAVES_PER_SIMD) void propagate_ke
                 if (hit_t < 0) {
                     pos = make_float3(pos.x + dir.x * step, pos.y + dir.y * step, pos.z + dir.z * step);
                     dist += step;
-                    dir = update_direction(dir, u01(wy), u01(wz), u01(ww), m);
+                    // the mixture uniform matters only with a Simplified-Liu share
+                    // (a wave-uniform medium field): with none, u = 1 never selects it
+                    float u_mix = 1.0f;
+                    if (m.sl_fraction > 0.0f)
+                        u_mix = u01(((wx & 0xFFu) << 8) | ((wy & 0xFFu) << 16) | ((wz & 0xFFu) << 24));
+                    dir = update_direction(dir, u01(wy), u01(wz), u_mix, m);
                     return false;
                 }
                 pos = make_float3(pos.x + dir.x * dt, pos.y + dir.y * dt, pos.z + dir.z * dt);
