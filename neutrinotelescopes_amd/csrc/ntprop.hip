@@ -474,8 +474,11 @@ __global__ __launch_bounds__(kThreads, NTP_MIN_WAVES_PER_SIMD) void propagate_ke
             // their low bytes.  The loop is unrolled by four so that every
             // step's words are fixed registers; the block index is wave-uniform.
             // A step returns true when the photon hit a target (state at the hit).
+            // −ln u · L_sca = log2 u · (−ln 2 · L_sca): the hardware log2 of u ∈ (0, 1]
+            // times a per-photon constant (≤ 2 ulp, like −ln u · L_sca itself)
+            const float step_scale = -0.693147182f * sca_len;
             auto scatter_step = [&](uint32_t wx, uint32_t wy, uint32_t wz) -> bool {
-                const float step = -log_unit(u01_oc(wx)) * sca_len;
+                const float step = __builtin_amdgcn_logf(u01_oc(wx)) * step_scale;
                 // first listed target hit within this step (photon_prop_cuda.jl:457-468)
                 float dt = 0.0f;
                 const int hit_t = first_hit(p, pos, dir, step, dt, tests, ctests);
