@@ -337,9 +337,12 @@ __device__ __forceinline__ void commit_counts(int slot, int64_t simulated, uint3
 }
 
 // Write the staged hits of this wavefront to the output columns of `slot`.
-__device__ __forceinline__ void flush_stage(const uint32_t* __restrict__ st, int n, int lane, int slot) {
+__device__ __forceinline__ void flush_stage(const uint32_t* __restrict__ st, int n, int, int slot) {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
+    // the lane index afresh (mbcnt of all lanes): nothing derived from it has to
+    // stay live from kernel start to the rare flushes
+    const int lane = (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
     int64_t* counters;
     const HitCols h = slot_cols(kargs(), slot, counters);
     unsigned long long base = 0;
