@@ -168,6 +168,58 @@ def surrogate_benchmark():
     return {k: r[k] for k in keep if k in r}
 
 
+def flow_kernels_benchmark(steps: int = 2000, llh_reps: int = 10):
+    """Flow kernels on synthetic data: `steps` Adam steps of the conditional
+    RQ-spline flow on batches of 16384 rows (80 % time rows, 20 % count rows,
+    random network weights) and the event likelihood of 1024 cascades on the
+    10-string x 20-module x 16-PMT array (3 276 800 pairs, Poisson(2)
+    observed hits per PMT), `llh_reps` times after 3 warm-up evaluations."""
+    import torch
+
+    from neutrinotelescopes_amd import surrogate as S
+    from neutrinotelescopes_amd.flow import FlowModel, FlowRows
+    rng = np.random.default_rng(0)
+    n = 1 << 20
+    x = rng.uniform(-1, 1, (n, 5)).astype(np.float32)
+    t = rng.normal(10 + 20 * x[:, 0], 3 + 2 * (x[:, 1] + 1)).astype(np.float32)
+    cnt = rng.poisson(np.exp(1 + x[:, 2])).astype(np.float32)
+    flags = np.where(rng.random(n) < 0.8, 1, 2).astype(np.uint8)
+    m = FlowModel(0, seed=1)
+    rows = FlowRows.from_numpy(x, t, cnt, flags)
+    batch = 16384
+    gen = torch.Generator(device=m.dev).manual_seed(0)
+    batches = [rows.slice(torch.randint(0, n, (batch,), generator=gen, device=m.dev)) for _ in range(16)]
+    for k in range(20):
+        m.train_step(batches[k % 16], 1e-3)
+    torch.cuda.synchronize(m.dev)
+    t0 = time.perf_counter()
+    for k in range(steps):
+        m.train_step(batches[k % 16], 1e-3)
+    torch.cuda.synchronize(m.dev)
+    step_ms = (time.perf_counter() - t0) / steps * 1e3
+    modules = S.array_detector()
+    casc = S.random_cascades(1024, rng)
+    counts = rng.poisson(2.0, (1024, len(modules), 16))
+    k = np.repeat(np.arange(counts.size), counts.reshape(-1))
+    hits = np.column_stack([k // (len(modules) * 16), (k // 16) % len(modules), k % 16,
+                            rng.uniform(-10, 300, k.size)])
+    f = S.pair_features(S.Cascades(casc.pos[:4], casc.dir[:4], casc.energy[:4]), modules).reshape(-1, 5)
+    sur = S.Surrogate(m, S.FeatureNorm(f.mean(0), f.std(0)),
+                      S.CountScale(np.linspace(0, 6, 9), np.linspace(-10, -20, 9)))
+    d = S.to_device(casc, modules, S.Observation(counts, hits), str(m.dev))
+    for _ in range(3):
+        S.surrogate_llh(sur, d)
+    torch.cuda.synchronize(m.dev)
+    t0 = time.perf_counter()
+    for _ in range(llh_reps):
+        S.surrogate_llh(sur, d)
+    torch.cuda.synchronize(m.dev)
+    llh_ms = (time.perf_counter() - t0) / llh_reps * 1e3
+    m.close()
+    return {"train_steps": steps, "batch": batch, "train_step_ms": step_ms, "llh_events": 1024,
+            "llh_pairs": int(1024 * len(modules) * 16), "llh_hits": int(d.hit_t.shape[0]), "llh_ms": llh_ms}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,6 +238,8 @@ def main():
                          "FETCH_SIZE/WRITE_SIZE counter scaling)")
     ap.add_argument("--no-culling", action="store_true",
                     help="test every sphere on every step instead of string bounds first")
+    ap.add_argument("--flow-kernels", action="store_true",
+                    help="only time the flow kernels on synthetic data (training steps, event likelihood)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU per process; multi-GPU runs are not "
@@ -198,6 +252,9 @@ def main():
         return
     if args.surrogate_only:
         print(json.dumps({"surrogate": surrogate_benchmark()}), flush=True)
+        return
+    if args.flow_kernels:
+        print(json.dumps({"flow_kernels": flow_kernels_benchmark()}), flush=True)
         return
     if args.copy_calibration:
         a = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
