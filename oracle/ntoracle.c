@@ -686,6 +686,51 @@ int ntor_propagate(const nt_source* sources, int n_sources, const nt_target* tar
     return nh > out->capacity ? NT_ERR_HIT_OVERFLOW : NT_OK;
 }
 
+/* Photons [first, first + count) of one source with RNG source index
+   `source_index` (the counter word 3 of its Philox stream), hits ordered by
+   photon: lets a test check photons whose 64-bit index has a nonzero high
+   word without propagating the 2^32 before them. */
+int ntor_propagate_range(const nt_source* source, uint32_t source_index, uint64_t first, int64_t count,
+                         const nt_target* targets, int n_targets, const nt_medium* medium,
+                         const nt_spectrum* spectrum, uint64_t seed, int n_threads, const nt_hits* out,
+                         int64_t* n_hits) {
+    if (!source || count < 0 || n_targets < 0 || !medium || !spectrum || !out) return NT_ERR_ARG;
+    if (n_threads < 1) n_threads = 1;
+    hitbuf* bufs = (hitbuf*)calloc(n_threads, sizeof(hitbuf));
+#pragma omp parallel num_threads(n_threads)
+    {
+        int tid = 0, nth = 1;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+        nth = omp_get_num_threads();
+#endif
+        int64_t lo = count * tid / nth, hi = count * (tid + 1) / nth;
+        for (int64_t g = lo; g < hi; ++g)
+            propagate_one(source, source_index, first + (uint64_t)g, targets, n_targets, medium, spectrum, seed,
+                          &bufs[tid]);
+    }
+    int64_t nh = 0, stored = 0;
+    for (int t = 0; t < n_threads; ++t) nh += bufs[t].n;
+    for (int t = 0; t < n_threads; ++t) {
+        for (int64_t i = 0; i < bufs[t].n && stored < out->capacity; ++i, ++stored) {
+            const hitrec* h = &bufs[t].v[i];
+            memcpy(out->position + 3 * stored, h->pos, 12);
+            memcpy(out->direction + 3 * stored, h->dir, 12);
+            memcpy(out->initial_direction + 3 * stored, h->idir, 12);
+            out->wavelength[stored] = h->wl;
+            out->time[stored] = h->time;
+            out->dist_travelled[stored] = h->dist;
+            out->module_id[stored] = h->module_id;
+            if (out->source_index) out->source_index[stored] = h->src;
+            if (out->photon_id) out->photon_id[stored] = h->photon;
+        }
+        free(bufs[t].v);
+    }
+    free(bufs);
+    *n_hits = stored;
+    return nh > out->capacity ? NT_ERR_HIT_OVERFLOW : NT_OK;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Post-processing                                                           */
 /* ------------------------------------------------------------------------ */

@@ -74,6 +74,10 @@ int ntor_propagate(const nt_source* sources, int n_sources, const nt_target* tar
                    int n_targets, const nt_medium* medium, const nt_spectrum* spectrum,
                    uint64_t seed, int n_threads, const nt_hits* out, int64_t* n_hits,
                    int64_t* n_simulated);
+int ntor_propagate_range(const nt_source* source, uint32_t source_index, uint64_t first, int64_t count,
+                         const nt_target* targets, int n_targets, const nt_medium* medium,
+                         const nt_spectrum* spectrum, uint64_t seed, int n_threads, const nt_hits* out,
+                         int64_t* n_hits);
 
 /* Post-processing (photon_prop_cuda.jl:850-924, detection.jl:72-163) */
 double ntor_fast_linear_interp(double x, const double* xs, const double* ys, int n);

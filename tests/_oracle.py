@@ -69,6 +69,9 @@ def load():
                                C.POINTER(_abi.nt_medium), C.POINTER(_abi.nt_spectrum), u64, i,
                                C.POINTER(_abi.nt_hits), C.POINTER(C.c_int64),
                                C.POINTER(C.c_int64)]),
+        "ntor_propagate_range": (i, [C.POINTER(_abi.nt_source), u32, u64, C.c_int64, C.POINTER(_abi.nt_target), i,
+                                     C.POINTER(_abi.nt_medium), C.POINTER(_abi.nt_spectrum), u64, i,
+                                     C.POINTER(_abi.nt_hits), C.POINTER(C.c_int64)]),
         "ntor_fast_linear_interp": (d, [d, dp, dp, i]),
         "ntor_check_pmt_hit": (i, [dp, dp, i, d]),
         "ntor_pom_pmt_coordinates": (None, [dp]),
@@ -100,6 +103,30 @@ def philox(ctr, key):
     o = (C.c_uint32 * 4)()
     lib.ntor_philox4x32_10(c, k, o)
     return list(o)
+
+
+def propagate_range(setup, first, count, seed=0, capacity=1 << 20, threads=None) -> PhotonHits:
+    """Oracle propagation of photons [first, first + count) of the setup's
+    single source; hits ordered by photon."""
+    lib = load()
+    assert len(setup.sources) == 1
+    src = setup.sources[0].to_abi()
+    tgts = _abi.as_c_array(_abi.nt_target, [t.to_abi() for t in setup.targets])
+    med = setup.medium.to_abi()
+    spec = setup.spectrum.to_abi()
+    hits = PhotonHits.empty(capacity)
+    h = hits.to_abi()
+    nh = C.c_int64()
+    if threads is None:
+        threads = min(len(os.sched_getaffinity(0)), 16)
+    st = lib.ntor_propagate_range(C.byref(src), 0, C.c_uint64(first), int(count), tgts, len(setup.targets),
+                                  C.byref(med), C.byref(spec), C.c_uint64(seed), int(threads), C.byref(h),
+                                  C.byref(nh))
+    if st not in (0, _abi.NT_ERR_HIT_OVERFLOW):
+        raise RuntimeError(f"oracle propagate_range failed: {st}")
+    out = hits.truncated(nh.value)
+    out.n_simulated = count
+    return out
 
 
 def propagate(setup, seed=0, capacity=1 << 20, threads=None) -> PhotonHits:

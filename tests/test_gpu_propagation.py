@@ -354,3 +354,25 @@ def test_epilogue_matches_host_postprocessing(engine):
     # the reference's PMT filter keeps only hits that land on a PMT
     kept = nt.make_hits_from_photons(ref, s, orient)
     assert len(kept) == int(np.sum(pmt > 0))
+
+
+def test_photon_indices_above_2_32(engine):
+    """One source of 2^32 + 2^20 photons (the reference's photon counts are
+    Int64): the hits of the last 2^20 photons, whose 64-bit index has a
+    nonzero high word, match the oracle photon for photon."""
+    import _oracle
+    first, count = 1 << 32, 1 << 20
+    med = nt.make_cascadia_medium_properties(0.95)
+    spec = nt.CherenkovSpectrum((300.0, 600.0), 30, med)
+    src = nt.PointlikeIsotropicEmitter(np.zeros(3, np.float32), 0.0, first + count)
+    tgt = nt.DetectionSphere(np.array([0, 0, 10.0], np.float32), 0.21, 16, PMT_AREA, 1)
+    setup = nt.PhotonPropSetup([src], [tgt], med, spec, seed=5)
+    print("photons", first + count, flush=True)
+    engine.setup(setup)
+    g = engine.propagate(seed=5, capacity=1 << 21)
+    assert g.n_simulated == first + count and g.n_dropped == 0
+    high = g.select(g.photon_id >= first).sorted()
+    o = _oracle.propagate_range(setup, first, count, seed=5).sorted()
+    print("hits above 2^32:", len(high), "oracle", len(o), "of", len(g), "in total")
+    assert len(o) > 50
+    assert_hits_match(high, o)
