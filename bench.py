@@ -105,9 +105,29 @@ def cpu_baseline(workload: str, target_s: float = 12.0):
     t0 = time.perf_counter()
     _oracle.propagate(s, seed=2, threads=threads, capacity=1 << 22)
     dt = time.perf_counter() - t0
-    return {"value": n_final / dt, "unit": "photons/s", "cores": threads, "kind": "port",
+    rate = n_final / dt
+    # the same workload on one core, a ~4 s sample
+    n1 = max(10000, int(rate / threads * 4.0))
+    s1 = build_workload(workload, n1)
+    t0 = time.perf_counter()
+    _oracle.propagate(s1, seed=3, threads=1, capacity=1 << 22)
+    dt1 = time.perf_counter() - t0
+    return {"value": rate, "unit": "photons/s", "cores": threads, "kind": "port",
             "sample": f"{n_final} photons of the {workload} workload, {dt:.1f} s on {threads} threads "
-                      f"(CPU oracle, g++ -O3, OpenMP)"}
+                      f"(CPU oracle, g++ -O3, OpenMP)",
+            "one_core_value": n1 / dt1, "one_core_sample": f"{n1} photons, {dt1:.1f} s on 1 thread",
+            "cpu_model": _cpu_model()}
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def serving_benchmark(duration_s: float = 8.0, seed: int = 11, round_photons: int = 10_000_000):
