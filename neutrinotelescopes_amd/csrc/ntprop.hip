@@ -4,11 +4,17 @@
 // Kernel: one photon per lane, a persistent grid that strides over the
 // concatenated photon index space of all sources.  Per photon: initial state
 // (photon_prop_cuda.jl:435-446), then at most max_steps scattering steps
-// (photon_prop_cuda.jl:449-495), each testing every target sphere in list
-// order.  Target spheres are wave-uniform data read through scalar loads, the
-// spectrum knots sit in LDS, and hits are compacted per wavefront with a
-// ballot + mbcnt prefix into an LDS staging area, then written to HBM as
-// contiguous column segments with one 64-bit atomic per flush.
+// (photon_prop_cuda.jl:449-495), unrolled by four so that each step's Philox
+// words (three per step, blocks 1, 2, ... of the photon's stream, evaluated
+// with the per-photon/per-wave split of PhiloxRun) are fixed registers.  Each
+// step finds the first listed target sphere the step reaches: vertical
+// strings of targets are culled by a conservative bound first (axis distance,
+// then z range), the exact sphere test runs on the survivors.  Target and
+// string records are wave-uniform data read through scalar loads; the
+// spectrum knots and the FP64 coefficients of the scattering-length fit sit
+// in LDS; hits are compacted per wavefront with a ballot + mbcnt prefix into
+// an LDS staging area, then written to HBM as contiguous column segments with
+// one 64-bit atomic per flush.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
