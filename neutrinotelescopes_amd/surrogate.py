@@ -687,7 +687,11 @@ def fallback_terms(engine, medium, spectrum, casc: Cascades, modules: np.ndarray
     engine.set_targets(make_targets(modules[mods]))
     engine.set_sources(srcs)
     engine.set_epilogue(medium, None, casc.pos[evs[0]])
-    h = engine.propagate(seed=seed, capacity=1 << 22)
+    # host hit columns sized for the usual case (tens of thousands of hits);
+    # an overflow repeats the launch with room for 4M
+    h = engine.propagate(seed=seed, capacity=1 << 18, raise_on_overflow=False)
+    if h.n_dropped:
+        h = engine.propagate(seed=seed, capacity=1 << 22)
     si = h.source_index.astype(np.int64)
     ev = evs[si]
     m = mods[h.module_id.astype(np.int64) - 1]
