@@ -34,7 +34,8 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-FP32_PEAK_TFLOPS = 157.3        # MI355X published FP32 vector peak
+FP32_PEAK_TFLOPS = 157.3        # MI355X published FP32 vector peak (packed FP32, two operations per lane)
+FP32_UNPACKED_TFLOPS = 78.6     # the same vector units issuing one FP32 operation per lane
 PMT_AREA = (75e-3 / 2) ** 2 * math.pi
 
 # Algorithmic work (counting rule: add/mul/cmp/select 1, fma 2, div 1,
@@ -365,6 +366,7 @@ def main():
                    "string_bound_tests_per_scattering_step": bound_tests_per_step / max(steps_total, 1)},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": profiled_traffic(args),
+                     "peak_unpacked": FP32_UNPACKED_TFLOPS, "frac_unpacked": achieved / FP32_UNPACKED_TFLOPS,
                      "kernel": "propagate_kernel", "kernel_ms": k_ms,
                      "flop_per_launch": flop},
     }
