@@ -9,6 +9,10 @@ Workload (one step = one launch of the propagation kernel over the batch):
   MultiPMTDetector modules (r = 0.21 m, 16 PMTs) at 50 m vertical spacing,
   20 m from the cascade axis.  Synthetic inputs, all device-resident before
   the timed region; hits go to device-resident columns.
+  `--workload cluster`: the same cascade inside the serving cluster (a centre
+  string and a hexagon of strings at 50 m, 7 × 20 modules), for the kernel's
+  rate with several strings; `--workload isotropic`: config 1's point source
+  and one sphere.
 
 Prints one JSON line.  `roofline` is for the propagation kernel (FP32 VALU
 bound): algorithmic FLOP per launch from the kernel's own step/test counters
@@ -48,7 +52,10 @@ PMT_AREA = (75e-3 / 2) ** 2 * math.pi
 FLOP_STEP = 61 + 4
 FLOP_TEST = 18
 FLOP_STRING_BOUND = 27   # z-range + segment/axis distance test of one string
-FLOP_INIT = {"isotropic": 49 + 9, "cascade": 49 + 9 + 95 + 20}
+FLOP_INIT = {"isotropic": 49 + 9, "cascade": 49 + 9 + 95 + 20, "cluster": 49 + 9 + 95 + 20}
+WORKLOAD_NAMES = {"cascade": "cherenkov_cascade_1e4GeV_1e7ph_20x16pmt_string",
+                  "cluster": "cherenkov_cascade_1e4GeV_in_7x20x16pmt_cluster",
+                  "isotropic": "isotropic_point_source_1_sphere_10m"}
 
 
 PROFILE = ROOT / "profiles" / "propagate_kernel_cascade_1e7ph.md"
@@ -78,6 +85,15 @@ def build_workload(name: str, photons: int):
         coords = nt.make_pom_pmt_coordinates(np.float32)
         tgts = [nt.MultiPMTDetector(np.array([20.0, 0.0, -475.0 + 50.0 * k], np.float32), 0.21,
                                     PMT_AREA, coords, k + 1) for k in range(20)]
+    elif name == "cluster":
+        # the serving cluster: a centre string and a hexagon of strings at
+        # 50 m, 20 modules each; the cascade inside it, pointing obliquely up
+        from neutrinotelescopes_amd.serving import cluster_targets
+        spec = nt.CherenkovSpectrum((300.0, 800.0), 30, med)
+        lp = nt.LongitudinalParameterisation.from_energy(1e4, med, nt.PEMinus)
+        src = nt.ExtendedCherenkovEmitter(np.array([10.0, 20.0, 5.0], np.float32),
+                                          np.array([0.6, 0.0, 0.8], np.float32), 0.0, photons, lp)
+        tgts = cluster_targets()
     elif name == "isotropic":
         spec = nt.CherenkovSpectrum((300.0, 600.0), 30, med)
         src = nt.PointlikeIsotropicEmitter(np.zeros(3, np.float32), 0.0, photons)
@@ -246,7 +262,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="cascade", choices=["cascade", "isotropic"])
+    ap.add_argument("--workload", default="cascade", choices=["cascade", "cluster", "isotropic"])
     ap.add_argument("--photons", type=int, default=10_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving (p50/p99) measurement")
@@ -357,8 +373,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "cherenkov_cascade_1e4GeV_1e7ph_20x16pmt_string" if args.workload == "cascade"
-                   else "isotropic_point_source_1_sphere_10m", "photons_per_step": args.photons,
+        "config": {"workload": WORKLOAD_NAMES[args.workload], "photons_per_step": args.photons,
                    "targets": n_targets, "hits_per_step": hits_per_step,
                    "string_culling": not args.no_culling,
                    "scattering_steps_per_photon": steps_total / args.photons,

@@ -110,10 +110,16 @@ struct Epilogue {
 // spheres only when the step's segment comes within reach.  Per cluster, two
 // float4 (one 32-byte scalar load):
 //   [2c]     axis x, axis y, lowest centre z, highest centre z
-//   [2c + 1] reach r0, first member, member count (int bits), 0
-// with r0 = r_max + 2^-10·(r_max + 1) + 1e-3 m rounded up to FP32.
-// Members (targets reordered by cluster): float4 (x, y, z, r^2) and their
-// list index, so the first listed target still wins.
+//   [2c + 1] reach r0, first member, member count (int bits), 1/Δz
+// with r0 = r_max + 2^-10·(r_max + 1) + 1e-3 m rounded up to FP32.  Members
+// of a string are sorted by z; 1/Δz > 0 when their centres sit within 5 % of
+// Δz of a uniform ladder z_lo + k·Δz (then a step only visits the members
+// near its z range), 0 otherwise (every member is visited).  The
+// kernel tests every string with the largest r0 (`cluster_reach`), a larger
+// and so still conservative bound, computed once per step.
+// Members (targets reordered by cluster, by z inside one): float4
+// (x, y, z, r^2) and their list index; the hit kept is the one with the
+// lowest list index, so the first listed target still wins.
 struct KParams {
     const DevSource* __restrict__ sources;
     const int64_t* __restrict__ offsets;     // n_sources + 1 (multiples of 64)
@@ -133,6 +139,7 @@ struct KParams {
     int n_sources;
     int n_targets;
     int n_clusters;                          // 0: test every target in list order
+    float cluster_reach;                     // largest cluster reach r0
     uint32_t seed_lo, seed_hi;
 };
 
@@ -152,7 +159,8 @@ __device__ __forceinline__ bool sphere_hit(float3 pos, float3 dir, float step, f
 
 // First listed target hit by the step [pos, pos + step·dir]: returns its list
 // index or -1.  Brute force over the list (n_clusters == 0) or through the
-// cluster bounds.  The bounds are inflated by m with
+// cluster bounds; `tests` counts the sphere tests behind the bounds (the
+// caller counts bound tests and brute-force tests per photon).  The bounds are inflated by m with
 // m = 2^-10·(step + r_max + 1) + 1e-3 m: the FP32 discriminant of the sphere
 // test can accept rays passing up to ~sqrt(5·2^-23)·|p − c| beyond the radius,
 // and |p − c| ≤ step + r for any accepted hit, so a culled target can never be
@@ -160,7 +168,7 @@ __device__ __forceinline__ bool sphere_hit(float3 pos, float3 dir, float step, f
 // geometry, not the reference's arithmetic, and use fused forms; their
 // roundings are ~1e-7 relative, far inside the 2^-10 and 1e-3 m slack.
 __device__ __forceinline__ int first_hit(const KParams& p, float3 pos, float3 dir, float step, float& dt,
-                                         uint32_t& tests, uint32_t& ctests) {
+                                         uint32_t& tests) {
     int best = 0x7fffffff;
     if (p.n_clusters == 0) {
         const const_f4_ptr tgt = (const_f4_ptr)(const void*)p.targets;
@@ -172,7 +180,6 @@ __device__ __forceinline__ int first_hit(const KParams& p, float3 pos, float3 di
                 dt = d;
             }
         }
-        tests += p.n_targets;
         return best == 0x7fffffff ? -1 : best;
     }
     typedef const __attribute__((address_space(4))) uint32_t* const_u32_ptr;
@@ -182,24 +189,45 @@ __device__ __forceinline__ int first_hit(const KParams& p, float3 pos, float3 di
     const float dx = dir.x * step, dy = dir.y * step;
     const float dd = __builtin_fmaf(dx, dx, dy * dy);
     const float inv_dd = dd > 0.0f ? __builtin_amdgcn_rcpf(dd) : 0.0f;
+    const float re = __builtin_fmaf(step, 0x1p-10f, p.cluster_reach);
+    const float re2 = re * re;
     for (int c = 0; c < p.n_clusters; ++c) {
         const f4 c0 = cl[2 * c];
         const f4 c1 = cl[2 * c + 1];
         const int first = __float_as_int(c1.y);
         const int count = __float_as_int(c1.z);
         if (first < 0 || count < 0 || first + count > p.n_targets) continue;  // corrupt table guard
-        ++ctests;
-        const float re = __builtin_fmaf(step, 0x1p-10f, c1.x);
         // distance of the step's xy projection from the string axis first:
         // for strings long compared with a step it is the selective test
         const float wx = c0.x - pos.x, wy = c0.y - pos.y;
         const float t = fminf(fmaxf(__builtin_fmaf(wx, dx, wy * dy) * inv_dd, 0.0f), 1.0f);
         const float ex = __builtin_fmaf(-t, dx, wx), ey = __builtin_fmaf(-t, dy, wy);
-        if (__builtin_fmaf(ex, ex, ey * ey) > re * re) continue;
+        if (__builtin_fmaf(ex, ex, ey * ey) > re2) continue;
         const float z1 = __builtin_fmaf(dir.z, step, pos.z);
         const float zmin = fminf(pos.z, z1), zmax = fmaxf(pos.z, z1);
         if (zmax < c0.z - re || zmin > c0.w + re) continue;
-        for (int k = first; k < first + count; ++k) {
+        int k0 = first, k1 = first + count;
+        if (c1.w > 0.0f) {
+            // members sorted by z at a uniform spacing 1/c1.w: the lanes still
+            // here need only the members whose centres can lie in their
+            // [zmin − re, zmax + re]; the index window is widened by a quarter
+            // spacing each side and the exact per-member test below still runs
+            const float top = (float)(count - 1);
+            const int lo = (int)fminf(fmaxf(floorf(__builtin_fmaf(zmin - re - c0.z, c1.w, -0.25f)), 0.0f), top);
+            const int hi = (int)fminf(fmaxf(floorf(__builtin_fmaf(zmax + re - c0.z, c1.w, 0.25f)), 0.0f), top);
+            // union over the lanes in this branch (wave-uniform)
+            uint64_t act = __ballot(1);
+            int ulo = count, uhi = -1;
+            while (act) {
+                const int l = __builtin_ctzll(act);
+                act &= act - 1;
+                ulo = min(ulo, __builtin_amdgcn_readlane(lo, l));
+                uhi = max(uhi, __builtin_amdgcn_readlane(hi, l));
+            }
+            k0 = first + ulo;
+            k1 = first + uhi + 1;
+        }
+        for (int k = k0; k < k1; ++k) {
             const f4 mv = mem[k];
             if (zmax < mv.z - re || zmin > mv.z + re) continue;
             ++tests;
@@ -490,7 +518,7 @@ __global__ __launch_bounds__(kThreads, NTP_MIN_WAVES_PER_SIMD) void propagate_ke
                 const float step = __builtin_amdgcn_logf(u01_oc(wx)) * step_scale;
                 // first listed target hit within this step (photon_prop_cuda.jl:457-468)
                 float dt = 0.0f;
-                const int hit_t = first_hit(p, pos, dir, step, dt, tests, ctests);
+                const int hit_t = first_hit(p, pos, dir, step, dt, tests);
                 if (hit_t < 0) {
                     pos = make_float3(pos.x + dir.x * step, pos.y + dir.y * step, pos.z + dir.z * step);
                     dist += step;
@@ -524,6 +552,13 @@ __global__ __launch_bounds__(kThreads, NTP_MIN_WAVES_PER_SIMD) void propagate_ke
                 if (scatter_step(b1.z, b1.w, b2.x) || ++k >= max_steps) break;
                 if (scatter_step(b2.y, b2.z, b2.w) || ++k >= max_steps) break;
             }
+            // every step tested all strings (or all targets): k steps without
+            // a hit, one more if the photon hit
+            const uint32_t steps_taken = (uint32_t)k + (hit ? 1u : 0u);
+            if (p.n_clusters)
+                ctests += steps_taken * (uint32_t)p.n_clusters;
+            else
+                tests += steps_taken * (uint32_t)p.n_targets;
         }
         simulated += __popcll(__ballot(active));
 
@@ -714,6 +749,7 @@ struct nt_ctx {
     // cluster index over the targets (vertical strings)
     bool culling = true;
     int n_clusters = 0;
+    float cluster_reach = 0.0f;         // largest reach r0 over the clusters
     float4* d_clusters = nullptr;
     float4* d_members = nullptr;
     uint32_t* d_member_idx = nullptr;
@@ -807,6 +843,7 @@ int32_t launch_plan(nt_ctx* c, uint64_t seed, const LaunchPlan& lp, hipStream_t 
     kp.members = c->d_members;
     kp.member_idx = c->d_member_idx;
     kp.n_clusters = c->culling ? c->n_clusters : 0;
+    kp.cluster_reach = c->cluster_reach;
     kp.knots = c->d_knots;
     kp.counters = lp.counters;
     kp.hits = lp.cols;
@@ -1283,8 +1320,8 @@ int32_t nt_set_targets(nt_ctx* c, const nt_target* t, int32_t n) {
         HIP_TRY(hipMemcpy(c->d_module_ids, ids.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
     }
     // Clusters: targets with identical (x, y) form a vertical string; a
-    // string of one is tested directly.  Members keep list order inside a
-    // cluster, clusters are ordered by their first member.
+    // string of one is tested directly.  Members are sorted by z (then list
+    // index) inside a cluster, clusters are ordered by their first member.
     std::vector<int> cluster_of(n, -1);
     std::vector<std::vector<int>> groups;
     for (int i = 0; i < n; ++i) {
@@ -1299,22 +1336,36 @@ int32_t nt_set_targets(nt_ctx* c, const nt_target* t, int32_t n) {
     }
     std::vector<float4> cl, mem;
     std::vector<uint32_t> midx;
-    for (const auto& grp : groups) {
+    float reach_max = 0.0f;
+    for (auto& grp : groups) {
+        std::stable_sort(grp.begin(), grp.end(), [&](int a, int b) { return tg[a].z < tg[b].z; });
         float zlo = INFINITY, zhi = -INFINITY, rmax = 0.0f;
         for (int i : grp) {
             zlo = std::min(zlo, tg[i].z);
             zhi = std::max(zhi, tg[i].z);
             rmax = std::max(rmax, t[i].radius);
         }
+        // uniform ladder test (FP64): |z_k − (z_lo + k·Δz)| ≤ Δz/20 for all k,
+        // and Δz ≥ 1e-3·(1 + |z|) so that the kernel's FP32 index arithmetic
+        // (errors ≲ 1e-3 of a spacing) stays well inside its quarter-spacing margin
+        float inv_dz = 0.0f;
+        if (grp.size() >= 2) {
+            const double dz = ((double)zhi - (double)zlo) / (double)(grp.size() - 1);
+            bool uniform = dz >= 1e-3 * (1.0 + std::max(std::fabs((double)zlo), std::fabs((double)zhi)));
+            for (size_t k = 0; k < grp.size() && uniform; ++k)
+                uniform = std::fabs((double)tg[grp[k]].z - ((double)zlo + (double)k * dz)) <= 0.05 * dz;
+            if (uniform) inv_dz = (float)(1.0 / dz);
+        }
         const int first = (int)mem.size(), count = (int)grp.size();
         const double reach = (double)rmax + 0x1p-10 * ((double)rmax + 1.0) + 1e-3;
         float r0 = (float)reach;
         if ((double)r0 < reach) r0 = std::nextafter(r0, INFINITY);
+        reach_max = std::max(reach_max, r0);
         float first_bits, count_bits;
         std::memcpy(&first_bits, &first, 4);
         std::memcpy(&count_bits, &count, 4);
         cl.push_back(make_float4(tg[grp[0]].x, tg[grp[0]].y, zlo, zhi));
-        cl.push_back(make_float4(r0, first_bits, count_bits, 0.0f));
+        cl.push_back(make_float4(r0, first_bits, count_bits, inv_dz));
         for (int i : grp) {
             mem.push_back(tg[i]);
             midx.push_back((uint32_t)i);
@@ -1334,6 +1385,7 @@ int32_t nt_set_targets(nt_ctx* c, const nt_target* t, int32_t n) {
     }
     // culling pays only when some cluster holds several targets
     c->n_clusters = (nc < n) ? nc : 0;
+    c->cluster_reach = reach_max;
     c->n_targets = n;
     c->host_targets.assign(t, t + n);
     c->has_targets = true;

@@ -67,13 +67,18 @@ class PhotonHits:
     extra: dict = field(default_factory=dict)
 
     @classmethod
-    def empty(cls, capacity: int) -> "PhotonHits":
+    def empty(cls, capacity: int, epilogue: bool = True) -> "PhotonHits":
+        """Unfilled columns for `capacity` hits: the library writes the first
+        n rows and callers keep only those (zero-filling hundreds of MB for a
+        few thousand hits cost milliseconds per call).  Without `epilogue`
+        the PMT id, weight and residual columns are left out (None)."""
         n = int(capacity)
-        return cls(np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32),
-                   np.zeros((n, 3), np.float32), np.zeros(n, np.float32), np.zeros(n, np.float32),
-                   np.zeros(n, np.float32), np.zeros(n, np.uint16), np.zeros(n, np.uint32),
-                   np.zeros(n, np.uint64), np.zeros(n, np.uint8), np.zeros(n, np.float32),
-                   np.zeros(n, np.float32))
+        epi = (np.empty(n, np.uint8), np.empty(n, np.float32), np.empty(n, np.float32)) if epilogue \
+            else (None, None, None)
+        return cls(np.empty((n, 3), np.float32), np.empty((n, 3), np.float32),
+                   np.empty((n, 3), np.float32), np.empty(n, np.float32), np.empty(n, np.float32),
+                   np.empty(n, np.float32), np.empty(n, np.uint16), np.empty(n, np.uint32),
+                   np.empty(n, np.uint64), *epi)
 
     def __len__(self):
         return len(self.wavelength)
@@ -126,6 +131,7 @@ class PropagationEngine:
         _abi.check(self.lib.nt_create(int(device), C.byref(h)), "nt_create")
         self.handle = h
         self._keep = []
+        self._epilogue = False          # device epilogue enabled (set_epilogue succeeded)
 
     def close(self):
         if self.handle:
@@ -166,6 +172,7 @@ class PropagationEngine:
         with the absorption and PMT-acceptance tables of this package."""
         e = _abi.nt_epilogue()
         e.enabled = int(enabled)
+        self._epilogue = False
         if enabled:
             from .detection import pmt_acceptance_table
             from .medium import absorption_table
@@ -186,6 +193,7 @@ class PropagationEngine:
             for i, v in enumerate(np.asarray(source_position, np.float32).reshape(3)):
                 e.source_position[i] = float(v)
         _abi.check(self.lib.nt_set_epilogue(self.handle, C.byref(e)), "nt_set_epilogue")
+        self._epilogue = bool(enabled)
 
     def launch_shape(self):
         b, t = C.c_int32(), C.c_int32()
@@ -194,7 +202,7 @@ class PropagationEngine:
 
     def propagate(self, seed: int = 0, capacity: int = 1 << 20, raise_on_overflow: bool = True):
         """Synchronous propagation into host columns (nt_propagate)."""
-        hits = PhotonHits.empty(capacity)
+        hits = PhotonHits.empty(capacity, epilogue=self._epilogue)
         h = hits.to_abi()
         nh, ns, nd = C.c_int64(), C.c_int64(), C.c_int64()
         st = self.lib.nt_propagate(self.handle, C.c_uint64(seed & (2**64 - 1)), C.byref(h), C.byref(nh),
@@ -202,6 +210,11 @@ class PropagationEngine:
         if st != _abi.NT_OK and not (st == _abi.NT_ERR_HIT_OVERFLOW and not raise_on_overflow):
             raise _abi.NTError(st, "nt_propagate")
         out = hits.truncated(nh.value)
+        if not self._epilogue:
+            # without the device epilogue these columns read as zeros
+            out.pmt_id = np.zeros(nh.value, np.uint8)
+            out.total_weight = np.zeros(nh.value, np.float32)
+            out.tres = np.zeros(nh.value, np.float32)
         out.n_simulated = ns.value
         out.n_dropped = nd.value
         return out

@@ -72,8 +72,7 @@ class PhotonServer:
 
     def wait(self, rid: int, raise_on_overflow: bool = True):
         cap = self._capacity.pop(rid)
-        hits = PhotonHits.empty(max(cap, 1))
-        hits.pmt_id = hits.total_weight = hits.tres = None
+        hits = PhotonHits.empty(max(cap, 1), epilogue=False)
         h = hits.to_abi()
         nh, ns, nd, lat = C.c_int64(), C.c_int64(), C.c_int64(), C.c_double()
         st = self.lib.nt_server_wait(self.handle, C.c_uint64(rid), C.byref(h), C.byref(nh), C.byref(ns),

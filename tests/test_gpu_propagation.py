@@ -295,7 +295,28 @@ def cluster_layout(n_strings=7, per_string=20, spacing=50.0, radius=0.21):
     return tg
 
 
-@pytest.mark.parametrize("layout", ["string", "cluster"])
+def ladder_layouts(kind):
+    """Strings whose list order is not their z order.  "dense_ladder": two
+    uniform ladders (5 m and 7 m spacing, r = 2 m) listed in shuffled order,
+    so a step spans several members; "irregular": a string with random z
+    spacing and overlapping spheres (r = 3 m, gaps 2-9 m) next to a uniform
+    one, so several members can be hit in one step and the first listed must
+    win."""
+    rng = np.random.default_rng(17)
+    coords = nt.make_pom_pmt_coordinates(np.float32)
+    if kind == "dense_ladder":
+        pts = [(8.0, 0.0, -60.0 + 5.0 * k, 2.0) for k in range(30)]
+        pts += [(-6.0, 9.0, -40.0 + 7.0 * k, 2.0) for k in range(20)]
+    else:
+        z = -50.0 + np.cumsum(rng.uniform(2.0, 9.0, 25))
+        pts = [(7.0, 2.0, float(zk), 3.0) for zk in z]
+        pts += [(-9.0, -4.0, -60.0 + 6.0 * k, 2.0) for k in range(20)]
+    order = rng.permutation(len(pts))
+    return [nt.MultiPMTDetector(np.array(pts[i][:3], np.float32), pts[i][3], PMT_AREA, coords, j + 1)
+            for j, i in enumerate(order)]
+
+
+@pytest.mark.parametrize("layout", ["string", "cluster", "dense_ladder", "irregular"])
 def test_culling_identical_to_brute_force(engine, layout):
     med = nt.make_cascadia_medium_properties(0.95)
     spec = nt.CherenkovSpectrum((300.0, 800.0), 30, med)
@@ -303,9 +324,12 @@ def test_culling_identical_to_brute_force(engine, layout):
     if layout == "string":
         tg = string_targets_r(2.0)
         pos = np.zeros(3, np.float32)
-    else:
+    elif layout == "cluster":
         tg = cluster_layout(radius=1.5)
         pos = np.array([10.0, 5.0, 20.0], np.float32)
+    else:
+        tg = ladder_layouts(layout)
+        pos = np.array([0.0, 1.0, -5.0], np.float32)
     src = nt.ExtendedCherenkovEmitter(pos, np.array([0.6, 0.0, 0.8], np.float32), 0.0, 1_000_000, lp)
     s = nt.PhotonPropSetup([src], tg, med, spec)
     engine.setup(s)
