@@ -181,7 +181,9 @@ int32_t nt_set_medium(nt_ctx* ctx, const nt_medium* medium);
 int32_t nt_set_spectrum(nt_ctx* ctx, const nt_spectrum* spectrum);
 
 /* Replaces `targets` (photon_prop_cuda.jl:740: CuVector(targets)).  Order is
-   kept: the first listed intersecting target wins (photon_prop_cuda.jl:460-468). */
+   kept: the first listed intersecting target wins (photon_prop_cuda.jl:460-468).
+   NT_ERR_ARG for a radius that is not finite and > 0 or a non-finite position
+   (the context's targets are then unchanged). */
 int32_t nt_set_targets(nt_ctx* ctx, const nt_target* targets, int32_t n_targets);
 
 /* Replaces `sources` (one kernel launch per source in photon_prop_cuda.jl:749-753;

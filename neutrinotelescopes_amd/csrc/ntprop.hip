@@ -1300,7 +1300,14 @@ int32_t nt_set_targets(nt_ctx* c, const nt_target* t, int32_t n) {
     std::vector<float4> tg(n);
     std::vector<uint32_t> ids(n);
     for (int i = 0; i < n; ++i) {
-        if (!(t[i].radius > 0.0f)) { snprintf(g_err, sizeof g_err, "target %d: radius must be > 0", i); return NT_ERR_ARG; }
+        if (!(t[i].radius > 0.0f) || !std::isfinite(t[i].radius)) {
+            snprintf(g_err, sizeof g_err, "target %d: radius must be finite and > 0", i);
+            return NT_ERR_ARG;
+        }
+        if (!std::isfinite(t[i].position[0]) || !std::isfinite(t[i].position[1]) || !std::isfinite(t[i].position[2])) {
+            snprintf(g_err, sizeof g_err, "target %d: position must be finite", i);
+            return NT_ERR_ARG;
+        }
         tg[i] = make_float4(t[i].position[0], t[i].position[1], t[i].position[2], t[i].radius * t[i].radius);
         ids[i] = t[i].module_id;
     }

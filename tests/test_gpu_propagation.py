@@ -225,6 +225,10 @@ def test_invalid_arguments(engine):
     src.photons, src.kind = 10, 42
     arr = _abi.as_c_array(_abi.nt_source, [src])
     assert lib.nt_set_sources(engine.handle, arr, 1) == _abi.NT_ERR_ARG
+    # targets: non-finite positions or radii are refused
+    for pos, r in (((0.0, 0.0, float("nan")), 0.2), ((float("inf"), 0.0, 0.0), 0.2), ((0.0, 0.0, 0.0), float("inf"))):
+        t = nt.DetectionSphere(np.array(pos, np.float32), r, 16, PMT_AREA, 1).to_abi()
+        assert lib.nt_set_targets(engine.handle, _abi.as_c_array(_abi.nt_target, [t]), 1) == _abi.NT_ERR_ARG
 
 
 def test_full_size_c1_properties(engine):
