@@ -1326,8 +1326,8 @@ int32_t nt_set_targets(nt_ctx* c, const nt_target* t, int32_t n) {
         HIP_TRY(hipMemcpy(c->d_targets, tg.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(c->d_module_ids, ids.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
     }
-    // Clusters: targets with identical (x, y) form a vertical string; a
-    // string of one is tested directly.  Members are sorted by z (then list
+    // Clusters: targets with identical (x, y) form a vertical string (a
+    // single target is a string of one).  Members are sorted by z (then list
     // index) inside a cluster, clusters are ordered by their first member.
     std::vector<int> cluster_of(n, -1);
     std::vector<std::vector<int>> groups;
@@ -1390,8 +1390,11 @@ int32_t nt_set_targets(nt_ctx* c, const nt_target* t, int32_t n) {
         HIP_TRY(hipMemcpy(c->d_members, mem.data(), sizeof(float4) * mem.size(), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(c->d_member_idx, midx.data(), sizeof(uint32_t) * midx.size(), hipMemcpyHostToDevice));
     }
-    // culling pays only when some cluster holds several targets
-    c->n_clusters = (nc < n) ? nc : 0;
+    // every target set goes through the bounds, single spheres included: a
+    // bound costs about what the sphere test's first half does, and it keeps
+    // the square root off the steps whose infinite line meets the sphere but
+    // whose segment does not (one sphere at 10 m: +2-3 % photons/s)
+    c->n_clusters = nc;
     c->cluster_reach = reach_max;
     c->n_targets = n;
     c->host_targets.assign(t, t + n);
