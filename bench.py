@@ -147,7 +147,8 @@ def _cpu_model() -> str:
     return "unknown"
 
 
-def serving_benchmark(duration_s: float = 8.0, seed: int = 11, round_photons: int = 10_000_000):
+def serving_benchmark(duration_s: float = 8.0, seed: int = 11, round_photons: int = 10_000_000,
+                      max_inflight_rounds: int = 2):
     """Serving configuration: Poisson stream of cascade requests (position
     uniform in a 300 m cylinder, isotropic direction, E log-uniform 1e2-1e6 GeV,
     1e4·E/100 GeV photons) against a 7-string × 20-module cluster, arrival
@@ -162,9 +163,11 @@ def serving_benchmark(duration_s: float = 8.0, seed: int = 11, round_photons: in
     eng.set_targets(cluster_targets())
     rng = np.random.default_rng(seed)
     mean_photons = 1e4 * (1e6 - 1e2) / math.log(1e4) / 100.0
-    with PhotonServer(eng, device_budget=32 << 30, host_budget=8 << 30, round_photons=round_photons) as srv:
-        # capacity: a burst submitted at once
-        burst = cascade_requests(64, med, rng)
+    with PhotonServer(eng, device_budget=32 << 30, host_budget=8 << 30, round_photons=round_photons,
+                      max_inflight_rounds=max_inflight_rounds) as srv:
+        # capacity: a burst submitted at once (256 requests, ~3e9 photons, so
+        # that the partly filled first and last rounds weigh little)
+        burst = cascade_requests(256, med, rng)
         t0 = time.perf_counter()
         ids = [srv.submit(r.sources, r.seed, r.capacity) for r in burst]
         for i in ids:
@@ -179,7 +182,7 @@ def serving_benchmark(duration_s: float = 8.0, seed: int = 11, round_photons: in
     return {"config": "poisson_cascades_1e2-1e6GeV_7x20_cluster", "requests": res["requests"],
             "arrival_rate_hz": rate, "p50_ms": res["p50_ms"], "p99_ms": res["p99_ms"],
             "photons_per_s": res["photons_per_s"], "capacity_photons_per_s": cap_rate,
-            "rounds": st["rounds"], "round_photons": round_photons,
+            "rounds": st["rounds"], "round_photons": round_photons, "max_inflight_rounds": max_inflight_rounds,
             "device_busy_fraction": st["busy_ms"] / 1e3 / res["wall_s"]}
 
 
