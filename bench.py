@@ -11,8 +11,8 @@ Workload (one step = one launch of the propagation kernel over the batch):
   the timed region; hits go to device-resident columns.
   `--workload cluster`: the same cascade inside the serving cluster (a centre
   string and a hexagon of strings at 50 m, 7 × 20 modules), for the kernel's
-  rate with several strings; `--workload isotropic`: config 1's point source
-  and one sphere.
+  rate with several strings; `--workload isotropic`: an isotropic point source
+  and one sphere at 10 m.
 
 Prints one JSON line.  `roofline` is for the propagation kernel (FP32 VALU
 bound): algorithmic FLOP per launch from the kernel's own step/test counters

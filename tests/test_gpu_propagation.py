@@ -63,8 +63,8 @@ def run_gpu(engine, setup, seed=0, capacity=1 << 20):
     return h
 
 
-def c1_ref_twin(n, g=0.95, r=0.21, **medium_ext):
-    """Config 1 in reference mode: isotropic point source, one DetectionSphere at
+def iso_sphere_setup(n, g=0.95, r=0.21, **medium_ext):
+    """Reference mode: isotropic point source, one DetectionSphere at
     10 m, Cascadia water, Frank-Tamm 300-600 nm table of 30 knots."""
     med = nt.make_cascadia_medium_properties(g, **medium_ext)
     spec = nt.CherenkovSpectrum((300.0, 600.0), 30, med)
@@ -85,7 +85,7 @@ def string_targets_r(radius, n=20, spacing=50.0, x=20.0):
                                 coords, k + 1) for k in range(n)]
 
 
-def c2_cascade(n, energy=1e4, radius=0.21):
+def cascade_string_setup(n, energy=1e4, radius=0.21):
     med = nt.make_cascadia_medium_properties(0.95)
     spec = nt.CherenkovSpectrum((300.0, 800.0), 30, med)
     lp = nt.LongitudinalParameterisation.from_energy(energy, med, nt.PEMinus)
@@ -93,8 +93,8 @@ def c2_cascade(n, energy=1e4, radius=0.21):
     return nt.PhotonPropSetup([src], string_targets_r(radius), med, spec)
 
 
-def test_c1_ref_twin_per_photon(engine):
-    s = c1_ref_twin(1_000_000, r=1.0)
+def test_isotropic_sphere_per_photon(engine):
+    s = iso_sphere_setup(1_000_000, r=1.0)
     g = run_gpu(engine, s, seed=5)
     o = O.propagate(s, seed=5)
     assert g.n_simulated == o.n_simulated == 1_000_000
@@ -102,8 +102,8 @@ def test_c1_ref_twin_per_photon(engine):
     assert_hits_match(g, o)
 
 
-def test_c1_statistics_independent_seeds(engine):
-    s = c1_ref_twin(2_000_000, r=1.0)
+def test_isotropic_sphere_statistics_independent_seeds(engine):
+    s = iso_sphere_setup(2_000_000, r=1.0)
     g = run_gpu(engine, s, seed=11)
     o = O.propagate(s, seed=12)
     ng, no = len(g), len(o)
@@ -113,19 +113,19 @@ def test_c1_statistics_independent_seeds(engine):
     assert stats.ks_2samp(g.dist_travelled, o.dist_travelled).pvalue > 0.01
 
 
-def test_c2_cascade_string_per_photon(engine):
+def test_cascade_string_per_photon(engine):
     # modules enlarged to r = 1 m so that 1e6 photons give ~10^3 hits to match
-    s = c2_cascade(1_000_000, radius=1.0)
+    s = cascade_string_setup(1_000_000, radius=1.0)
     g = run_gpu(engine, s, seed=3)
     o = O.propagate(s, seed=3)
     assert len(o) > 300
     assert_hits_match(g, o)
 
 
-def test_c2_cascade_module_counts_independent_seeds(engine):
+def test_cascade_string_module_counts_independent_seeds(engine):
     # the configuration itself (r = 0.21 m, 1e7 photons): per-module counts
     # of independent GPU and oracle runs agree within 3σ Poisson
-    s = c2_cascade(10_000_000)
+    s = cascade_string_setup(10_000_000)
     g = run_gpu(engine, s, seed=4)
     o = O.propagate(s, seed=5)
     assert len(o) > 100
@@ -200,7 +200,7 @@ def test_no_targets_and_zero_photons(engine):
 
 
 def test_overflow_keeps_prefix_and_counts(engine):
-    s = c1_ref_twin(400_000, r=2.0)
+    s = iso_sphere_setup(400_000, r=2.0)
     full = run_gpu(engine, s, seed=8)
     engine.setup(s)
     part = engine.propagate(seed=8, capacity=100, raise_on_overflow=False)
@@ -231,10 +231,10 @@ def test_invalid_arguments(engine):
         assert lib.nt_set_targets(engine.handle, _abi.as_c_array(_abi.nt_target, [t]), 1) == _abi.NT_ERR_ARG
 
 
-def test_full_size_c1_properties(engine):
-    """1e8 photons (the parity size of config 1): size-independent properties."""
+def test_isotropic_sphere_1e8_properties(engine):
+    """Isotropic source, one sphere at 10 m, 1e8 photons: size-independent properties."""
     n = 100_000_000
-    s = c1_ref_twin(n)
+    s = iso_sphere_setup(n)
     g = run_gpu(engine, s, seed=1, capacity=1 << 22)
     assert g.n_simulated == n and g.n_dropped == 0
     # every hit lies on the sphere surface, inside the table's wavelengths,
@@ -249,7 +249,7 @@ def test_full_size_c1_properties(engine):
     assert np.all(g.dist_travelled >= 10.0 - 0.21 - 1e-4)
     assert len(np.unique(g.photon_id)) == len(g)
     # hit count agrees with the oracle's rate measured on 2e6 photons (3σ)
-    o = O.propagate(c1_ref_twin(2_000_000), seed=99)
+    o = O.propagate(iso_sphere_setup(2_000_000), seed=99)
     rate = len(o) / 2e6
     expect = rate * n
     sigma = math.sqrt(expect + (n / 2e6) ** 2 * len(o))
@@ -261,7 +261,7 @@ def test_full_size_c1_properties(engine):
 
 def test_device_launch_with_torch_buffers(engine):
     torch = pytest.importorskip("torch")
-    s = c1_ref_twin(500_000, r=1.0)
+    s = iso_sphere_setup(500_000, r=1.0)
     host = run_gpu(engine, s, seed=4)
     cap = 1 << 16
     dev = torch.device("cuda:0")
@@ -287,7 +287,7 @@ def test_device_launch_with_torch_buffers(engine):
 
 
 def cluster_layout(n_strings=7, per_string=20, spacing=50.0, radius=0.21):
-    """A centre string plus a hexagon at 50 m, 20 modules each (config 3 layout)."""
+    """A centre string plus a hexagon at 50 m, 20 modules each (the serving cluster)."""
     coords = nt.make_pom_pmt_coordinates(np.float32)
     xy = [(0.0, 0.0)] + [(50.0 * math.cos(k * math.pi / 3), 50.0 * math.sin(k * math.pi / 3))
                          for k in range(n_strings - 1)]
