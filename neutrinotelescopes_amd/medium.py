@@ -117,7 +117,7 @@ def refractive_index(wavelength, medium: WaterProperties):
     x2 = x * x
     # fma(x, a2, a01) + fma(x2, a3, x2*x*a4), fma evaluated exactly then rounded
     t1 = (x.astype(np.float64) * np.float64(a2) + np.float64(a01)).astype(f32)
-    t2 = (x2.astype(np.float64) * np.float64(a3) + np.float64(x2 * x * a4)).astype(f32)
+    t2 = (x2.astype(np.float64) * np.float64(a3) + np.asarray(x2 * x * a4, dtype=np.float64)).astype(f32)
     return t1 + t2
 
 
