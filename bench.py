@@ -147,7 +147,7 @@ def _cpu_model() -> str:
     return "unknown"
 
 
-def serving_benchmark(duration_s: float = 8.0, seed: int = 11, round_photons: int = 10_000_000,
+def serving_benchmark(duration_s: float = 15.0, seed: int = 11, round_photons: int = 10_000_000,
                       max_inflight_rounds: int = 2):
     """Serving configuration: Poisson stream of cascade requests (position
     uniform in a 300 m cylinder, isotropic direction, E log-uniform 1e2-1e6 GeV,
