@@ -74,9 +74,9 @@ def profiled_traffic(args):
     return None
 
 
-def build_workload(name: str, photons: int):
+def build_workload(name: str, photons: int, direction_arithmetic: str = "fp64"):
     import neutrinotelescopes_amd as nt
-    med = nt.make_cascadia_medium_properties(0.95)
+    med = nt.make_cascadia_medium_properties(0.95, direction_arithmetic=direction_arithmetic)
     if name == "cascade":
         spec = nt.CherenkovSpectrum((300.0, 800.0), 30, med)
         lp = nt.LongitudinalParameterisation.from_energy(1e4, med, nt.PEMinus)
@@ -276,6 +276,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device-to-device copy 5 times (reference for "
                          "FETCH_SIZE/WRITE_SIZE counter scaling)")
+    ap.add_argument("--direction-arithmetic", default="fp64", choices=["fp64", "fp32"],
+                    help="direction sampling and rotations in FP64 (default) or in the reference's FP32 arithmetic")
     ap.add_argument("--no-culling", action="store_true",
                     help="test every sphere on every step instead of string bounds first")
     ap.add_argument("--flow-kernels", action="store_true",
@@ -308,7 +310,7 @@ def main():
     from neutrinotelescopes_amd.propagation import PropagationEngine
 
     dev = torch.device("cuda:0")
-    setup = build_workload(args.workload, args.photons)
+    setup = build_workload(args.workload, args.photons, args.direction_arithmetic)
     eng = PropagationEngine(0)
     eng.setup(setup)
     _abi.check(eng.lib.nt_set_option(eng.handle, _abi.NT_OPT_TARGET_CULLING, 0 if args.no_culling else 1),
@@ -378,6 +380,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": WORKLOAD_NAMES[args.workload], "photons_per_step": args.photons,
                    "targets": n_targets, "hits_per_step": hits_per_step,
+                   "direction_arithmetic": args.direction_arithmetic,
                    "string_culling": not args.no_culling,
                    "scattering_steps_per_photon": steps_total / args.photons,
                    "sphere_tests_per_scattering_step": tests_per_step / max(steps_total, 1),

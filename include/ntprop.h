@@ -92,8 +92,24 @@ typedef struct nt_medium {           /* 48 bytes */
     int32_t dispersion_mode;         /* NT_DISPERSION_* */
     int32_t max_steps;               /* scattering steps per photon; 15 = reference
                                         (photon_prop_cuda.jl:449) */
-    int32_t reserved;
+    int32_t direction_arithmetic;    /* NT_DIR_* */
 } nt_medium;
+
+/* Arithmetic of the direction sampling and rotations: the scattering
+   direction (update_direction, photon_prop_cuda.jl:204-222, with
+   rot_from_ez_fast, utils.jl:228-243), the Cherenkov emission direction
+   (photon_prop_cuda.jl:107-125, utils.jl:249-260) and the axicone direction.
+   NT_DIR_FP64 evaluates the reference formulas in FP64 from the FP32 inputs
+   and rounds once: within 2 FP32 ulp of an FP64 evaluation of the reference
+   functions (the hardware sine/cosine of the scattering azimuth, <= 1.3e-7,
+   is the main remaining error).  NT_DIR_FP32_REF evaluates them in FP32 with
+   the reference's operation order, as the reference kernel does; there
+   √(1 − cos²θ) near cos θ = ±1 and the track-direction sampler lose up to
+   thousands of ulps against FP64, and it runs about a quarter faster. */
+enum {
+    NT_DIR_FP64 = 0,
+    NT_DIR_FP32_REF = 1
+};
 
 /* ---- spectrum: src/photon_propagation/spectrum.jl:21-108 --------------- */
 enum {
@@ -389,7 +405,8 @@ enum {
     NT_FN_GAMMA = 13,           /* a b lrad seed_lo seed_hi photon (bits) -> depth trials */
     NT_FN_SCA_LEN_FIT = 14,     /* wl -> L_sca as the kernel takes it: FP64 polynomial fit of the
                                    formula over the spectrum's range, the formula outside it */
-    NT_FN_COUNT = 15
+    NT_FN_TRACK_DIR = 15,       /* u_cos u_phi -> v3                (utils.jl:249-260) */
+    NT_FN_COUNT = 16
 };
 int32_t nt_eval(nt_ctx* ctx, int32_t fn, const float* in, int32_t in_width,
                 float* out, int32_t out_width, int64_t n);

@@ -33,6 +33,8 @@ NT_SRC_POINTLIKE_CHERENKOV = 5
 
 NT_DISPERSION_REF = 0
 NT_DISPERSION_QUAN_FRY = 1
+NT_DIR_FP64 = 0
+NT_DIR_FP32_REF = 1
 
 NT_SPEC_MONOCHROMATIC = 0
 NT_SPEC_TABLE = 1
@@ -52,7 +54,7 @@ NT_MAX_KNOTS = 1024
 (NT_FN_INTERSECT, NT_FN_HG_COS, NT_FN_SL_COS, NT_FN_ROT_FROM_EZ, NT_FN_SPH_TO_CART,
  NT_FN_ISO_DIR, NT_FN_REFRACTIVE_INDEX, NT_FN_GROUP_VELOCITY, NT_FN_SCA_LEN,
  NT_FN_CHERENKOV_DIR, NT_FN_UPDATE_DIR, NT_FN_SPECTRUM, NT_FN_PHILOX, NT_FN_GAMMA,
- NT_FN_SCA_LEN_FIT) = range(15)
+ NT_FN_SCA_LEN_FIT, NT_FN_TRACK_DIR) = range(16)
 
 
 class nt_source(C.Structure):
@@ -81,7 +83,7 @@ class nt_medium(C.Structure):
         ("sca_len_scale", C.c_float),
         ("dispersion_mode", C.c_int32),
         ("max_steps", C.c_int32),
-        ("reserved", C.c_int32),
+        ("direction_arithmetic", C.c_int32),
     ]
 
 

@@ -265,7 +265,9 @@ __device__ __forceinline__ Medium load_medium(const_kparams_ptr k) {
     m.vcs = k->medium.vcs; m.vcl = k->medium.vcl; m.g = k->medium.g;
     m.sl_fraction = k->medium.sl_fraction; m.sca_len_scale = k->medium.sca_len_scale;
     m.dispersion_mode = k->medium.dispersion_mode; m.max_steps = k->medium.max_steps;
-    m.track_norm = k->medium.track_norm;
+    m.dir_mode = k->medium.dir_mode;
+    m.hg.omg = k->medium.hg.omg; m.hg.twog = k->medium.hg.twog; m.hg.omg2 = k->medium.hg.omg2;
+    m.hg.k0 = k->medium.hg.k0; m.hg.inv2g = k->medium.hg.inv2g;
     return m;
 }
 
@@ -413,8 +415,9 @@ __device__ __forceinline__ void flush_stage(const uint32_t* __restrict__ st, int
 #endif
 
 // KIND >= 0: every source of the launch has that kind (smaller code, fewer
-// registers); KIND < 0: mixed kinds, dispatched per photon.
-template <int KIND>
+// registers); KIND < 0: mixed kinds, dispatched per photon.  DIRM: the
+// medium's direction arithmetic (NT_DIR_FP64 or NT_DIR_FP32_REF).
+template <int KIND, int DIRM>
 __global__ __launch_bounds__(kThreads, NTP_MIN_WAVES_PER_SIMD) void propagate_kernel(KParams p) {
     extern __shared__ float lds[];
     float* s_knots = lds;                                          // n_knots (padded to 4)
@@ -485,7 +488,7 @@ __global__ __launch_bounds__(kThreads, NTP_MIN_WAVES_PER_SIMD) void propagate_ke
                     time = S->t0 + u01(r0.w) * (S->t1 - S->t0);
                     break;
                 case NT_SRC_AXICONE:
-                    dir = rot_from_ez(sdir, sph_to_cart(S->angle, 0.0f + u01(r0.z) * kTwoPi));
+                    dir = rot_from_ez_mode(sdir, sph_to_cart(S->angle, 0.0f + u01(r0.z) * kTwoPi), DIRM);
                     break;
                 case NT_SRC_PENCIL:
                     dir = sdir;
@@ -494,11 +497,13 @@ __global__ __launch_bounds__(kThreads, NTP_MIN_WAVES_PER_SIMD) void propagate_ke
                     const float lp = rand_gamma(S->gamma_d, S->gamma_c, S->gamma_scale, rng, nullptr) * S->lrad;
                     pos = make_float3(pos.x + lp * sdir.x, pos.y + lp * sdir.y, pos.z + lp * sdir.z);
                     time = S->t0 + lp / kCVacMNs;
-                    dir = cherenkov_dir(wl, u01(r0.y), u01(r0.z), u01(r0.w), sdir, mi);
+                    dir = DIRM == NT_DIR_FP32_REF ? cherenkov_dir32(wl, u01(r0.y), u01(r0.z), u01(r0.w), sdir, mi)
+                                                  : cherenkov_dir(wl, u01(r0.y), u01(r0.z), u01(r0.w), sdir, mi);
                     break;
                 }
                 default:  // NT_SRC_POINTLIKE_CHERENKOV
-                    dir = cherenkov_dir(wl, u01(r0.y), u01(r0.z), u01(r0.w), sdir, mi);
+                    dir = DIRM == NT_DIR_FP32_REF ? cherenkov_dir32(wl, u01(r0.y), u01(r0.z), u01(r0.w), sdir, mi)
+                                                  : cherenkov_dir(wl, u01(r0.y), u01(r0.z), u01(r0.w), sdir, mi);
                     break;
             }
             idir = dir;
@@ -527,7 +532,14 @@ __global__ __launch_bounds__(kThreads, NTP_MIN_WAVES_PER_SIMD) void propagate_ke
                     float u_mix = 1.0f;
                     if (m.sl_fraction > 0.0f)
                         u_mix = u01(((wx & 0xFFu) << 8) | ((wy & 0xFFu) << 16) | ((wz & 0xFFu) << 24));
-                    dir = update_direction(dir, u01(wy), u01(wz), u_mix, m);
+                    if (DIRM == NT_DIR_FP32_REF) {
+                        dir = update_direction32(dir, u01(wy), u01(wz), u_mix, m);
+                    } else {
+                        const const_kparams_ptr kh = kargs();
+                        const HgConst hg{kh->medium.hg.omg, kh->medium.hg.twog, kh->medium.hg.omg2, kh->medium.hg.k0,
+                                         kh->medium.hg.inv2g};
+                        dir = update_direction(dir, u01(wy), u01(wz), u_mix, m, hg);
+                    }
                     return false;
                 }
                 pos = make_float3(pos.x + dir.x * dt, pos.y + dir.y * dt, pos.z + dir.z * dt);
@@ -603,16 +615,21 @@ __global__ __launch_bounds__(kThreads, NTP_MIN_WAVES_PER_SIMD) void propagate_ke
 typedef void (*PropagateFn)(KParams);
 constexpr int kKernelVariants = 7;     // one per source kind + mixed
 
-PropagateFn propagate_variant(int v) {
+template <int DIRM>
+PropagateFn propagate_variant_m(int v) {
     switch (v) {
-        case NT_SRC_POINTLIKE_ISOTROPIC: return propagate_kernel<NT_SRC_POINTLIKE_ISOTROPIC>;
-        case NT_SRC_POINTLIKE_TIME_RANGE: return propagate_kernel<NT_SRC_POINTLIKE_TIME_RANGE>;
-        case NT_SRC_AXICONE: return propagate_kernel<NT_SRC_AXICONE>;
-        case NT_SRC_PENCIL: return propagate_kernel<NT_SRC_PENCIL>;
-        case NT_SRC_EXTENDED_CHERENKOV: return propagate_kernel<NT_SRC_EXTENDED_CHERENKOV>;
-        case NT_SRC_POINTLIKE_CHERENKOV: return propagate_kernel<NT_SRC_POINTLIKE_CHERENKOV>;
-        default: return propagate_kernel<-1>;
+        case NT_SRC_POINTLIKE_ISOTROPIC: return propagate_kernel<NT_SRC_POINTLIKE_ISOTROPIC, DIRM>;
+        case NT_SRC_POINTLIKE_TIME_RANGE: return propagate_kernel<NT_SRC_POINTLIKE_TIME_RANGE, DIRM>;
+        case NT_SRC_AXICONE: return propagate_kernel<NT_SRC_AXICONE, DIRM>;
+        case NT_SRC_PENCIL: return propagate_kernel<NT_SRC_PENCIL, DIRM>;
+        case NT_SRC_EXTENDED_CHERENKOV: return propagate_kernel<NT_SRC_EXTENDED_CHERENKOV, DIRM>;
+        case NT_SRC_POINTLIKE_CHERENKOV: return propagate_kernel<NT_SRC_POINTLIKE_CHERENKOV, DIRM>;
+        default: return propagate_kernel<-1, DIRM>;
     }
+}
+
+PropagateFn propagate_variant(int v, int dir_mode) {
+    return dir_mode == NT_DIR_FP32_REF ? propagate_variant_m<NT_DIR_FP32_REF>(v) : propagate_variant_m<NT_DIR_FP64>(v);
 }
 
 __global__ void clear_counters(int64_t* c) {
@@ -666,10 +683,10 @@ __global__ void eval_kernel(int fn, const float* __restrict__ in, int iw, float*
             y[1] = d;
             break;
         }
-        case NT_FN_HG_COS: y[0] = hg_cos(x[0], x[1]); break;
+        case NT_FN_HG_COS: y[0] = m.dir_mode == NT_DIR_FP32_REF ? hg_cos32(x[0], x[1]) : hg_cos(x[0], x[1]); break;
         case NT_FN_SL_COS: y[0] = sl_cos(x[0], x[1]); break;
         case NT_FN_ROT_FROM_EZ: {
-            const float3 v = rot_from_ez(make_float3(x[0], x[1], x[2]), make_float3(x[3], x[4], x[5]));
+            const float3 v = rot_from_ez_mode(make_float3(x[0], x[1], x[2]), make_float3(x[3], x[4], x[5]), m.dir_mode);
             y[0] = v.x; y[1] = v.y; y[2] = v.z;
             break;
         }
@@ -688,12 +705,19 @@ __global__ void eval_kernel(int fn, const float* __restrict__ in, int iw, float*
         case NT_FN_SCA_LEN: y[0] = scattering_length(x[0], m); break;
         case NT_FN_SCA_LEN_FIT: y[0] = scattering_length_fit(x[0], &sca_poly, m); break;
         case NT_FN_CHERENKOV_DIR: {
-            const float3 v = cherenkov_dir(x[0], x[1], x[2], x[3], make_float3(x[4], x[5], x[6]), m);
+            const float3 v = cherenkov_dir_mode(x[0], x[1], x[2], x[3], make_float3(x[4], x[5], x[6]), m);
+            y[0] = v.x; y[1] = v.y; y[2] = v.z;
+            break;
+        }
+        case NT_FN_TRACK_DIR: {
+            const float3 v = m.dir_mode == NT_DIR_FP32_REF ? track_dir32(x[0], x[1]) : to_f3(track_dir64(x[0], x[1]));
             y[0] = v.x; y[1] = v.y; y[2] = v.z;
             break;
         }
         case NT_FN_UPDATE_DIR: {
-            const float3 v = update_direction(make_float3(x[0], x[1], x[2]), x[3], x[4], x[5], m);
+            const float3 v = m.dir_mode == NT_DIR_FP32_REF
+                                 ? update_direction32(make_float3(x[0], x[1], x[2]), x[3], x[4], x[5], m)
+                                 : update_direction(make_float3(x[0], x[1], x[2]), x[3], x[4], x[5], m, m.hg);
             y[0] = v.x; y[1] = v.y; y[2] = v.z;
             break;
         }
@@ -730,7 +754,7 @@ struct nt_ctx {
     int num_cus = 0;
     hipStream_t stream = nullptr;
     int blocks = 0;                    // grid of the mixed-kind kernel
-    int blocks_per_variant[7] = {};
+    int blocks_per_variant[2][7] = {};      // [direction arithmetic][variant]
     int variant = 6;                   // source kind shared by all sources, 6 = mixed
     std::mutex mu;
 
@@ -856,9 +880,10 @@ int32_t launch_plan(nt_ctx* c, uint64_t seed, const LaunchPlan& lp, hipStream_t 
     kp.seed_lo = (uint32_t)seed;
     kp.seed_hi = (uint32_t)(seed >> 32);
     const int64_t need_blocks = (lp.total + kThreads - 1) / kThreads;
-    const int blocks = (int)std::min<int64_t>(c->blocks_per_variant[lp.variant], need_blocks);
-    hipLaunchKernelGGL(propagate_variant(lp.variant), dim3(blocks), dim3(kThreads), lds_bytes(c->spectrum.n),
-                       s, kp);
+    const int dm = c->medium.dir_mode == NT_DIR_FP32_REF ? 1 : 0;
+    const int blocks = (int)std::min<int64_t>(c->blocks_per_variant[dm][lp.variant], need_blocks);
+    hipLaunchKernelGGL(propagate_variant(lp.variant, c->medium.dir_mode), dim3(blocks), dim3(kThreads),
+                       lds_bytes(c->spectrum.n), s, kp);
     HIP_TRY(hipGetLastError());
     return NT_OK;
 }
@@ -961,6 +986,17 @@ int32_t rebuild_epilogue(nt_ctx* c) {
 
 int64_t round_up64(int64_t n) { return (n + 63) & ~int64_t(63); }
 
+// Every float field of a source is finite (infinite or NaN longitudinal
+// parameters would make every Gamma trial fail; the others would propagate
+// NaN photons).
+__host__ __device__ inline bool source_fields_finite(const nt_source& a) {
+    const float v[13] = {a.position[0], a.position[1], a.position[2], a.direction[0], a.direction[1],
+                         a.direction[2], a.time, a.time_end, a.angle, a.long_a, a.long_b, a.long_lrad, 0.0f};
+    bool ok = true;
+    for (int k = 0; k < 12; ++k) ok = ok && __builtin_isfinite(v[k]);
+    return ok;
+}
+
 // Device record of an nt_source (offset, key, base and slot left at 0).
 // nt_set_sources_device: per record the preparation of make_dev_source (same
 // FP32 operations, IEEE square root and division), photon counts rounded up
@@ -974,7 +1010,7 @@ __global__ __launch_bounds__(256) void prepare_sources_kernel(const nt_source* _
     const nt_source a = in[i];
     DevSource d{};
     bool ok = a.photons >= 0 && a.photons <= (int64_t(1) << 56) && a.kind >= NT_SRC_POINTLIKE_ISOTROPIC &&
-              a.kind <= NT_SRC_POINTLIKE_CHERENKOV;
+              a.kind <= NT_SRC_POINTLIKE_CHERENKOV && source_fields_finite(a);
     d.kind = a.kind;
     d.rng_index = (uint32_t)i;
     d.photons = a.photons;
@@ -988,6 +1024,7 @@ __global__ __launch_bounds__(256) void prepare_sources_kernel(const nt_source* _
         d.gamma_c = 1.0f / sqrtf(9.0f * d.gamma_d);
         d.gamma_scale = 1.0f / a.long_b;
         d.lrad = a.long_lrad;
+        ok = ok && isfinite(d.gamma_scale);
     }
     if (!ok) {
         atomicMin(err, i + 1);
@@ -1032,6 +1069,11 @@ int32_t make_dev_source(const nt_source& a, int i, DevSource& d) {
         snprintf(g_err, sizeof g_err, "source %d: photons must be in [0, 2^56] and kind known", i);
         return NT_ERR_ARG;
     }
+    if (!source_fields_finite(a)) {
+        snprintf(g_err, sizeof g_err, "source %d: position, direction, times, angle and longitudinal "
+                 "parameters must be finite", i);
+        return NT_ERR_ARG;
+    }
     memset(&d, 0, sizeof d);
     d.kind = a.kind;
     d.rng_index = (uint32_t)i;
@@ -1041,8 +1083,9 @@ int32_t make_dev_source(const nt_source& a, int i, DevSource& d) {
     d.t1 = a.time_end;
     d.angle = a.angle;
     if (a.kind == NT_SRC_EXTENDED_CHERENKOV) {
-        if (!(a.long_a > 1.0f && a.long_b > 0.0f && a.long_lrad > 0.0f)) {
-            snprintf(g_err, sizeof g_err, "source %d: longitudinal parameters need a > 1, b > 0, lrad > 0", i);
+        if (!(a.long_a > 1.0f && a.long_b > 0.0f && a.long_lrad > 0.0f) || !std::isfinite(1.0f / a.long_b)) {
+            snprintf(g_err, sizeof g_err, "source %d: longitudinal parameters need a > 1, b > 0 (1/b finite), "
+                     "lrad > 0", i);
             return NT_ERR_ARG;
         }
         // rand_gamma(T(a), T(1/b)) set-up, utils.jl:269-270 and photon_prop_cuda.jl:168
@@ -1090,14 +1133,16 @@ int32_t nt_create(int32_t device, nt_ctx** out) {
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete c; return NT_ERR_DEVICE; }
     c->num_cus = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return NT_ERR_DEVICE; }
-    for (int v = 0; v < kKernelVariants; ++v) {
-        int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, propagate_variant(v), kThreads,
-                                                         lds_bytes(NT_MAX_KNOTS)) != hipSuccess || per_cu < 1)
-            per_cu = 1;
-        c->blocks_per_variant[v] = c->num_cus * per_cu;
-    }
-    c->blocks = c->blocks_per_variant[kKernelVariants - 1];
+    for (int dm = 0; dm < 2; ++dm)
+        for (int v = 0; v < kKernelVariants; ++v) {
+            int per_cu = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, propagate_variant(v, dm ? NT_DIR_FP32_REF : NT_DIR_FP64),
+                                                             kThreads, lds_bytes(NT_MAX_KNOTS)) != hipSuccess ||
+                per_cu < 1)
+                per_cu = 1;
+            c->blocks_per_variant[dm][v] = c->num_cus * per_cu;
+        }
+    c->blocks = c->blocks_per_variant[0][kKernelVariants - 1];
     if (hipMalloc((void**)&c->d_counters, 4 * sizeof(int64_t)) != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
         delete c;
@@ -1209,6 +1254,7 @@ int32_t nt_set_medium(nt_ctx* c, const nt_medium* m) {
     if (!c || !m) return NT_ERR_ARG;
     if (!(m->mean_scattering_angle > 0.0f && m->mean_scattering_angle < 1.0f) ||
         m->max_steps < 0 || !(m->sl_fraction >= 0.0f && m->sl_fraction <= 1.0f) ||
+        (m->direction_arithmetic != NT_DIR_FP64 && m->direction_arithmetic != NT_DIR_FP32_REF) ||
         !(m->sca_len_scale > 0.0f) ||
         (m->dispersion_mode != NT_DISPERSION_REF && m->dispersion_mode != NT_DISPERSION_QUAN_FRY)) {
         snprintf(g_err, sizeof g_err, "invalid medium parameters");
@@ -1224,9 +1270,8 @@ int32_t nt_set_medium(nt_ctx* c, const nt_medium* m) {
     d.sca_len_scale = m->sca_len_scale;
     d.dispersion_mode = m->dispersion_mode;
     d.max_steps = m->max_steps;
-    // normaliser of the Cherenkov track-direction law (utils.jl:251-253),
-    // a constant: evaluated once here in FP32 with the host's libm
-    d.track_norm = 1.0f - expf(-2.61f * powf(2.0f, 0.39f));
+    d.dir_mode = m->direction_arithmetic;
+    d.hg = hg_const(d.g);                // FP64 constants of the scattering function
     c->has_medium = true;
     refresh_sca_poly(c);
     if (c->epi_enabled) {
@@ -1433,8 +1478,8 @@ int32_t nt_set_sources_device(nt_ctx* c, const nt_source* d_src, int32_t n, int3
     int32_t bad;
     std::memcpy(&bad, &back[0], sizeof bad);
     if (bad != none) {
-        snprintf(g_err, sizeof g_err, "source %d: photons must be in [0, 2^56], kind known, and extended "
-                 "Cherenkov longitudinal parameters a > 1, b > 0, lrad > 0", bad - 1);
+        snprintf(g_err, sizeof g_err, "source %d: photons must be in [0, 2^56], kind known, fields finite, "
+                 "and extended Cherenkov longitudinal parameters a > 1, b > 0 (1/b finite), lrad > 0", bad - 1);
         c->n_sources = 0;
         c->total = 0;
         c->has_sources = false;
@@ -1521,7 +1566,7 @@ int32_t nt_set_option(nt_ctx* c, int32_t option, int64_t value) {
 
 int32_t nt_launch_shape(nt_ctx* c, int32_t* blocks, int32_t* threads) {
     if (!c) return NT_ERR_ARG;
-    if (blocks) *blocks = c->blocks_per_variant[c->variant];
+    if (blocks) *blocks = c->blocks_per_variant[c->medium.dir_mode == NT_DIR_FP32_REF ? 1 : 0][c->variant];
     if (threads) *threads = kThreads;
     return NT_OK;
 }

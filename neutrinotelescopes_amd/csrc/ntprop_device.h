@@ -1,8 +1,9 @@
-// ntprop_device.h — FP32 device functions of the photon-propagation kernel
+// ntprop_device.h — device functions of the photon-propagation kernel
 // (gfx950).  Each function states the reference function it implements
-// (paths relative to the reference repository root).  The library is built
-// with -ffp-contract=off: the only fused multiply-adds are the explicit
-// __builtin_fmaf calls, at the places where the reference calls fma.
+// (paths relative to the reference repository root).  Photon state is FP32
+// as in the reference; direction sampling and rotation compute in FP64 and
+// round once (see below).  The library is built with -ffp-contract=off: the
+// only fused multiply-adds are the explicit fma calls.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -136,7 +137,14 @@ __device__ __forceinline__ float clampf(float x, float lo, float hi) {
 // ---------------------------------------------------------------------------
 // Medium: medium.jl
 // ---------------------------------------------------------------------------
+// Constants of the scattering function for one medium (FP64): 1 − g, 2g,
+// 1 − g², (1 + g²)/(2g), 1/(2g)
+struct HgConst {
+    double omg, twog, omg2, k0, inv2g;
+};
+
 struct Medium {
+    HgConst hg;               // set from g on the host (hg_const)
     float a01, a2, a3, a4;    // Quan-Fry coefficients
     float vcs, vcl;           // particle volume concentrations (ppm)
     float g;                  // mean scattering cosine
@@ -144,7 +152,7 @@ struct Medium {
     float sca_len_scale;      // scattering-length multiplier (1 = reference)
     int dispersion_mode;
     int max_steps;
-    float track_norm;         // 1 − exp(−2.61·2^0.39) of utils.jl:253, evaluated on the host
+    int dir_mode;             // NT_DIR_FP64 or NT_DIR_FP32_REF
 };
 
 // refractive_index_fry, medium.jl:234-244
@@ -231,8 +239,126 @@ __device__ __forceinline__ float3 sph_to_cart_cos(float ct, float u_phi) {
     return make_float3(cp * st, sp * st, ct);
 }
 
-// rot_from_ez_fast, utils.jl:228-243 (the |a_z| == 1 branch as written)
+// ---------------------------------------------------------------------------
+// Direction sampling and rotation in FP64.  The reference evaluates these in
+// FP32, where √(1 − cos²θ) near cos θ = ±1, the rotation coefficients near
+// a_z = −1 and the track-direction sampler lose up to thousands of FP32 ulps
+// against an FP64 evaluation of the same formulas.  The kernel evaluates the
+// formulas from the FP32 inputs in FP64 (an FP64 fma issues at the rate of an
+// FP32 one on gfx950) and rounds once to FP32; the remaining error is the
+// hardware sine/cosine of the azimuth (≤ 1.3e-7) and that final rounding.
+// ---------------------------------------------------------------------------
+// An FP64 constant materialised where it is used (two scalar moves): the
+// persistent kernel's loop would otherwise hoist every FP64 literal into a
+// pair of VGPRs for its whole duration and spill them.
+__device__ __forceinline__ double kc(double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+s"(c));
+#endif
+    return c;
+}
+
+// 1/x for normal x: v_rcp_f64 and one Newton step (relative error < 2^-50)
+__device__ __forceinline__ double rcp64(double x) {
+    const double r = __builtin_amdgcn_rcp(x);
+    return __builtin_fma(__builtin_fma(-x, r, 1.0), r, r);
+}
+
+// √x for x ≥ 0: x·rsq(x) refined by one Newton step (relative error < 2^-48;
+// 0 for x = 0)
+__device__ __forceinline__ double sqrt64(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    const double s = x * y;                                   // x = 0: 0 · inf = NaN, masked below
+    const double r = __builtin_fma(-s, s, x);
+    const double v = __builtin_fma(r, 0.5 * y, s);
+    return x > 0.0 ? v : 0.0;
+}
+
+struct D3 { double x, y, z; };
+
+// ln x for a normal x > 0: x = m·2^e with m in [√½, √2), ln m = 2·atanh(z),
+// z = (m − 1)/(m + 1), |z| < 0.1716, series to z^15 (relative error < 1e-15)
+__device__ __forceinline__ double log64(double x) {
+    double m = __builtin_amdgcn_frexp_mant(x);           // [0.5, 1)
+    int e = __builtin_amdgcn_frexp_exp(x);
+    if (m < 0.70710678118654752) { m = m + m; e -= 1; }
+    const double z = (m - 1.0) * rcp64(m + 1.0), z2 = z * z;
+    double p = __builtin_fma(z2, kc(1.0 / 15.0), kc(1.0 / 13.0));
+    p = __builtin_fma(z2, p, kc(1.0 / 11.0));
+    p = __builtin_fma(z2, p, kc(1.0 / 9.0));
+    p = __builtin_fma(z2, p, kc(1.0 / 7.0));
+    p = __builtin_fma(z2, p, kc(1.0 / 5.0));
+    p = __builtin_fma(z2, p, kc(1.0 / 3.0));
+    return __builtin_fma((double)e, kc(0.69314718055994531), 2.0 * __builtin_fma(z * z2, p, z));
+}
+
+// 2^y for y <= 1024: y = k + f with k = rint(y), |f| <= 1/2, 2^f = exp(f·ln 2)
+// by its Taylor series to the 12th power (relative error < 1e-14); ldexp scales
+__device__ __forceinline__ double exp2_64(double y) {
+    const double k = rint(y);
+    const double x = (y - k) * kc(0.69314718055994531);
+    double p = __builtin_fma(x, kc(1.0 / 479001600.0), kc(1.0 / 39916800.0));
+    p = __builtin_fma(x, p, kc(1.0 / 3628800.0));
+    p = __builtin_fma(x, p, kc(1.0 / 362880.0));
+    p = __builtin_fma(x, p, kc(1.0 / 40320.0));
+    p = __builtin_fma(x, p, kc(1.0 / 5040.0));
+    p = __builtin_fma(x, p, kc(1.0 / 720.0));
+    p = __builtin_fma(x, p, kc(1.0 / 120.0));
+    p = __builtin_fma(x, p, kc(1.0 / 24.0));
+    p = __builtin_fma(x, p, kc(1.0 / 6.0));
+    p = __builtin_fma(x, p, 0.5);
+    p = __builtin_fma(x, p, 1.0);
+    p = __builtin_fma(x, p, 1.0);
+    return __builtin_amdgcn_ldexp(p, (int)fmax(k, -2000.0));
+}
+
+// sin and cos of 2π·u in FP64 for an FP32 u in [0, 1]: u − q/4 with q the
+// nearest quarter turn is exact in FP32, then Taylor series in x = 2π(u − q/4),
+// |x| ≤ π/4 (truncation below 3e-14), and the quadrant's swap and signs.
+__device__ __forceinline__ void sincos_rev64(float u, double& s, double& c) {
+    const float q = rintf(4.0f * u);
+    const double x = (double)__builtin_fmaf(-0.25f, q, u) * kc(6.283185307179586);
+    const double x2 = x * x;
+    double ps = __builtin_fma(x2, kc(-1.0 / 39916800.0), kc(1.0 / 362880.0));
+    ps = __builtin_fma(x2, ps, kc(-1.0 / 5040.0));
+    ps = __builtin_fma(x2, ps, kc(1.0 / 120.0));
+    ps = __builtin_fma(x2, ps, kc(-1.0 / 6.0));
+    const double s0 = __builtin_fma(x * x2, ps, x);
+    double pc = __builtin_fma(x2, kc(1.0 / 479001600.0), kc(-1.0 / 3628800.0));
+    pc = __builtin_fma(x2, pc, kc(1.0 / 40320.0));
+    pc = __builtin_fma(x2, pc, kc(-1.0 / 720.0));
+    pc = __builtin_fma(x2, pc, kc(1.0 / 24.0));
+    pc = __builtin_fma(x2, pc, -0.5);
+    const double c0 = __builtin_fma(x2, pc, 1.0);
+    const int k = (int)q & 3;                  // quarter turns: (s, c) → (c, −s) → (−s, −c) → (−c, s)
+    const double sa = (k & 1) ? c0 : s0, ca = (k & 1) ? s0 : c0;
+    s = (k & 2) ? -sa : sa;
+    c = ((k + 1) & 2) ? -ca : ca;
+}
+
+// rot_from_ez_fast, utils.jl:228-243 (the |a_z| == 1 branch as written), in FP64
+__device__ __forceinline__ D3 rot_from_ez64(D3 a, D3 o) {
+    if (fabs(a.z) == 1.0) return D3{o.x, copysign(o.y, a.z), copysign(o.z, a.z)};
+    const double a1sq = a.x * a.x, a2sq = a.y * a.y;
+    const double fact = (1.0 - a.z) * rcp64(a2sq + a1sq);
+    const double a1a2f = a.x * a.y * fact;
+    const double x = __builtin_fma(__builtin_fma(-a1sq, fact, 1.0), o.x, __builtin_fma(-a1a2f, o.y, a.x * o.z));
+    const double y = __builtin_fma(__builtin_fma(-a2sq, fact, 1.0), o.y, __builtin_fma(-a1a2f, o.x, a.y * o.z));
+    const double z = __builtin_fma(a.z, o.z, __builtin_fma(-a.x, o.x, -(a.y * o.y)));
+    return D3{x, y, z};
+}
+
+__device__ __forceinline__ D3 to_d3(float3 v) { return D3{(double)v.x, (double)v.y, (double)v.z}; }
+__device__ __forceinline__ float3 to_f3(D3 v) { return make_float3((float)v.x, (float)v.y, (float)v.z); }
+
+// rot_from_ez_fast, utils.jl:228-243, for FP32 operands (FP64 inside)
 __device__ __forceinline__ float3 rot_from_ez(float3 a, float3 o) {
+    return to_f3(rot_from_ez64(to_d3(a), to_d3(o)));
+}
+
+// rot_from_ez_fast in FP32 with the reference's operation order
+// (NT_DIR_FP32_REF)
+__device__ __forceinline__ float3 rot_from_ez32(float3 a, float3 o) {
     if (fabsf(a.z) == 1.0f) return make_float3(o.x, copysignf(o.y, a.z), copysignf(o.z, a.z));
     const float a1sq = a.x * a.x, a2sq = a.y * a.y;
     // 1 − a_z in [2^-24, 2], a_1² + a_2² = (1 − a_z)(1 + a_z) up to rounding: normal operands
@@ -244,8 +370,34 @@ __device__ __forceinline__ float3 rot_from_ez(float3 a, float3 o) {
     return make_float3(x, y, z);
 }
 
-// cuda_hg_scattering_func, photon_prop_cuda.jl:70-77
+__device__ __forceinline__ float3 rot_from_ez_mode(float3 a, float3 o, int mode) {
+    return mode == NT_DIR_FP32_REF ? rot_from_ez32(a, o) : rot_from_ez(a, o);
+}
+
+// cuda_hg_scattering_func, photon_prop_cuda.jl:70-77, in FP64: cos θ and
+// sin θ = √(1 − cos²θ) of the scattering angle for the uniform η
+__host__ __device__ __forceinline__ HgConst hg_const(float g) {
+    const double gd = g;
+    return HgConst{1.0 - gd, 2.0 * gd, __builtin_fma(-gd, gd, 1.0), __builtin_fma(gd, gd, 1.0) / (2.0 * gd),
+                   0.5 / gd};
+}
+__device__ __forceinline__ void hg_cos_sin(const HgConst& h, float eta, double& c, double& s) {
+    // 1 + g(2η − 1) = (1 − g) + 2g·η ≥ 1 − g
+    const double t = h.omg2 * rcp64(__builtin_fma(h.twog, (double)eta, h.omg));
+    c = fmin(fmax(__builtin_fma(-t * t, h.inv2g, h.k0), -1.0), 1.0);
+    s = sqrt64(__builtin_fma(-c, c, 1.0));
+}
+
+// the same cosine rounded to FP32
 __device__ __forceinline__ float hg_cos(float g, float eta) {
+    double c, s;
+    hg_cos_sin(hg_const(g), eta, c, s);
+    return (float)c;
+}
+
+// cuda_hg_scattering_func in FP32 with the reference's operation order
+// (NT_DIR_FP32_REF)
+__device__ __forceinline__ float hg_cos32(float g, float eta) {
     // numerator 1 − g² and denominator 1 + g(2η − 1) ≥ 1 − |g| are normal for |g| < 1
     const float t = div_normal(__builtin_fmaf(-g, g, 1.0f), __builtin_fmaf(g, __builtin_fmaf(2.0f, eta, -1.0f), 1.0f));
     const float c = (1.0f / (2.0f * g)) * (__builtin_fmaf(g, g, 1.0f) - t * t);
@@ -258,15 +410,32 @@ __device__ __forceinline__ float sl_cos(float g, float xi) {
     return clampf(2.0f * pow_pos(xi, 1.0f / (1.0f + alpha)) - 1.0f, -1.0f, 1.0f);
 }
 
-// update_direction, photon_prop_cuda.jl:204-222
+// update_direction, photon_prop_cuda.jl:204-222: scattering angle, azimuth
+// and the rotation onto the current direction in FP64 (see above)
+// `hg` is passed apart from the medium so that the kernel can read it where
+// it is used rather than hold its ten scalar registers across the loop.
 __device__ __forceinline__ float3 update_direction(float3 dir, float eta, float u_phi, float u_mix,
-                                                   const Medium& m) {
-    const float c = (u_mix < m.sl_fraction) ? sl_cos(m.g, eta) : hg_cos(m.g, eta);
+                                                   const Medium& m, const HgConst& hg) {
+    double c, s;
+    hg_cos_sin(hg, eta, c, s);
+    if (m.sl_fraction > 0.0f && u_mix < m.sl_fraction) {    // the first test is wave-uniform
+        c = (double)sl_cos(m.g, eta);
+        s = sqrt64(__builtin_fma(-c, c, 1.0));
+    }
+    float sp, cp;
+    sincos_rev(u_phi, &sp, &cp);       // φ = 2π·u_phi
+    return to_f3(rot_from_ez64(to_d3(dir), D3{(double)cp * s, (double)sp * s, c}));
+}
+
+// update_direction in FP32 with the reference's operation order (NT_DIR_FP32_REF)
+__device__ __forceinline__ float3 update_direction32(float3 dir, float eta, float u_phi, float u_mix,
+                                                     const Medium& m) {
+    const float c = (u_mix < m.sl_fraction) ? sl_cos(m.g, eta) : hg_cos32(m.g, eta);
     // v_sqrt_f32 (1 ulp): the argument lies in [0, 1]
     const float s = __builtin_amdgcn_sqrtf(__builtin_fmaf(-c, c, 1.0f));
     float sp, cp;
     sincos_rev(u_phi, &sp, &cp);       // φ = 2π·u_phi
-    return rot_from_ez(dir, make_float3(cp * s, sp * s, c));
+    return rot_from_ez32(dir, make_float3(cp * s, sp * s, c));
 }
 
 // initialize_direction_isotropic, photon_prop_cuda.jl:100-104
@@ -274,18 +443,60 @@ __device__ __forceinline__ float3 iso_dir(float u_cos, float u_phi) {
     return sph_to_cart_cos(-1.0f + u_cos * 2.0f, u_phi);     // θ = acos(2u − 1)
 }
 
+// sample_cherenkov_track_direction, utils.jl:249-260, in FP64: the clsim law
+// cos θ = max(1 − (−ln(1 − u·I)/b)^(1/a), −1) with a = 0.39, b = 2.61,
+// I = 1 − exp(−b·2^a), and sin θ = √(s(2 − s)) for s = 1 − cos θ (no
+// cancellation near cos θ = 1); φ = 2π·u_phi.
+__device__ __forceinline__ D3 track_dir64(float u_tcos, float u_tphi) {
+    // I = 1 − exp(−b·2^a), 1/b, 1/(a·ln 2)
+    const double t = -log64(__builtin_fma(-(double)u_tcos, kc(0.9672917122137301), 1.0)) *
+                     kc(0.3831417624521073);                                   // log argument in [1 − I, 1]
+    const double sv = t > 0.0 ? exp2_64(log64(t) * kc(3.6992180535614447)) : 0.0;   // t^(1/a) in [0, 2]
+    const double ct = fmax(1.0 - sv, -1.0);
+    const double st = sqrt64(sv * (2.0 - sv));                                // 0 for sv >= 2
+    double sp, cp;
+    sincos_rev64(u_tphi, sp, cp);
+    return D3{cp * st, sp * st, ct};
+}
+
 // sample_cherenkov_direction (photon_prop_cuda.jl:107-125) with
-// cherenkov_angle (medium.jl:309-311) and sample_cherenkov_track_direction
-// (utils.jl:249-260)
+// cherenkov_angle (medium.jl:309-311) and the track direction above, in FP64
+// from the FP32 wavelength, uniforms, medium coefficients and source direction:
+// cos θc = 1/n(λ), sin θc = √(n² − 1)/n, then the two rotations.
 __device__ __forceinline__ float3 cherenkov_dir(float wl, float u_phi, float u_tcos, float u_tphi,
                                                 float3 src_dir, const Medium& m) {
-    const float3 ph = sph_to_cart_cos(1.0f / refractive_index(wl, m), u_phi);   // θc = acos(1/n)
+    const double x = rcp64((double)wl), x2 = x * x;
+    const double n = __builtin_fma(x, (double)m.a2, (double)m.a01) +
+                     __builtin_fma(x2, (double)m.a3, x2 * x * (double)m.a4);
+    const double ctc = rcp64(n);
+    const double stc = sqrt64(__builtin_fma(n, n, -1.0)) * ctc;
+    double sp, cp;
+    sincos_rev64(u_phi, sp, cp);
+    const D3 ph{cp * stc, sp * stc, ctc};
+    return to_f3(rot_from_ez64(to_d3(src_dir), rot_from_ez64(track_dir64(u_tcos, u_tphi), ph)));
+}
+
+// The same two functions in FP32 with the reference's operation order
+// (NT_DIR_FP32_REF); the track normaliser I = 1 − exp(−2.61·2^0.39) is the
+// FP32 constant the reference computes (utils.jl:253).
+__device__ __forceinline__ float3 track_dir32(float u_tcos, float u_tphi) {
     const float a = 0.39f, b = 2.61f;
-    const float I = m.track_norm;                    // < 1: the log argument lies in (0, 1]
+    const float I = 0.96729171f;                     // < 1: the log argument lies in (0, 1]
     const float t = -log_unit(1.0f - u_tcos * I) / b;
     const float ct = fmaxf(1.0f - pow_pos(t, 1.0f / a), -1.0f);
-    const float3 track = sph_to_cart_cos(ct, u_tphi);
-    return rot_from_ez(src_dir, rot_from_ez(track, ph));
+    return sph_to_cart_cos(ct, u_tphi);
+}
+
+__device__ __forceinline__ float3 cherenkov_dir32(float wl, float u_phi, float u_tcos, float u_tphi,
+                                                  float3 src_dir, const Medium& m) {
+    const float3 ph = sph_to_cart_cos(1.0f / refractive_index(wl, m), u_phi);   // θc = acos(1/n)
+    return rot_from_ez32(src_dir, rot_from_ez32(track_dir32(u_tcos, u_tphi), ph));
+}
+
+__device__ __forceinline__ float3 cherenkov_dir_mode(float wl, float u_phi, float u_tcos, float u_tphi,
+                                                     float3 src_dir, const Medium& m) {
+    return m.dir_mode == NT_DIR_FP32_REF ? cherenkov_dir32(wl, u_phi, u_tcos, u_tphi, src_dir, m)
+                                         : cherenkov_dir(wl, u_phi, u_tcos, u_tphi, src_dir, m);
 }
 
 // check_intersection(::Spherical), photon_prop_cuda.jl:233-271, split in two:
@@ -335,10 +546,18 @@ __device__ __forceinline__ float sample_spectrum(const Spectrum& s, const float*
 // kGammaBlock + t: Box-Muller normal from words 0,1, acceptance uniform from
 // word 2.  The squeeze of utils.jl:286 can never accept (1 − fma(0.0331, x⁴, 1)
 // ≤ 0 < u) and is omitted; the log test alone decides, as in the reference.
+// The reference loops until a trial is accepted.  Here at most kGammaMaxTrials
+// trials run, so that every photon (and every kernel) ends whatever the
+// parameters: for finite d ≥ 2/3 (shape a > 1, which the source validation
+// requires) a trial is accepted with probability above 0.9, so the bound is
+// never reached in practice (probability < 1e-3000); when it is, the sample
+// is d·scale (the distribution's mode region) and *trials is kGammaMaxTrials + 1.
 // ---------------------------------------------------------------------------
+constexpr uint32_t kGammaMaxTrials = 4096;
+
 __device__ __forceinline__ float rand_gamma(float d, float c, float scale, const Rng& rng,
                                             int* trials) {
-    for (uint32_t t = 0;; ++t) {
+    for (uint32_t t = 0; t < kGammaMaxTrials; ++t) {
         const uint4 r = rng.block(kGammaBlock + t);
         const float cs = __builtin_amdgcn_cosf(u01(r.y));      // cos 2πu
         const float x = __builtin_amdgcn_sqrtf(-2.0f * log_unit(u01_oc(r.x))) * cs;
@@ -352,6 +571,8 @@ __device__ __forceinline__ float rand_gamma(float d, float c, float scale, const
             return d * v * scale;
         }
     }
+    if (trials) *trials = (int)kGammaMaxTrials + 1;
+    return d * scale;
 }
 
 }  // namespace ntp

@@ -14,6 +14,10 @@ Extensions with no reference counterpart (defaults reproduce the reference):
                    differentiates the Quan-Fry polynomial
   max_steps        scattering steps per photon (15, photon_prop_cuda.jl:449)
   abs_len_scale    multiplier on the absorption-length table (weights only)
+  direction_arithmetic
+                   "fp64": direction sampling and rotations evaluated in FP64
+                   and rounded once (within 2 FP32 ulp of the reference
+                   formulas in FP64); "fp32": the reference's FP32 arithmetic
 """
 from __future__ import annotations
 
@@ -63,6 +67,7 @@ class WaterProperties:
     dispersion_mode: str = "ref"
     max_steps: int = 15
     abs_len_scale: float = 1.0
+    direction_arithmetic: str = "fp64"
 
     def __post_init__(self):
         for name in ("salinity", "pressure", "temperature", "vol_conc_small_part",
@@ -73,6 +78,8 @@ class WaterProperties:
         self.density = f32(DIPPR105(float(self.temperature) + 273.15))
         if self.dispersion_mode not in ("ref", "quan_fry"):
             raise ValueError("dispersion_mode must be 'ref' or 'quan_fry'")
+        if self.direction_arithmetic not in ("fp64", "fp32"):
+            raise ValueError("direction_arithmetic must be 'fp64' or 'fp32'")
 
     def to_abi(self) -> _abi.nt_medium:
         m = _abi.nt_medium()
@@ -86,6 +93,7 @@ class WaterProperties:
         m.dispersion_mode = (_abi.NT_DISPERSION_REF if self.dispersion_mode == "ref"
                              else _abi.NT_DISPERSION_QUAN_FRY)
         m.max_steps = int(self.max_steps)
+        m.direction_arithmetic = _abi.NT_DIR_FP64 if self.direction_arithmetic == "fp64" else _abi.NT_DIR_FP32_REF
         return m
 
 

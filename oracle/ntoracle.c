@@ -403,12 +403,17 @@ static inline float box_muller(uint32_t r0, uint32_t r1) {
 }
 
 /* rand_gamma, utils.jl:267-294 (Marsaglia-Tsang); trial t uses Philox block
-   0x80000000+t: normal from words 0,1 and the acceptance uniform from word 2. */
+   0x80000000+t: normal from words 0,1 and the acceptance uniform from word 2.
+   The reference loops until a trial is accepted; here (as in the kernel) at
+   most NTOR_GAMMA_MAX_TRIALS trials run, after which d·scale is returned with
+   *trials = NTOR_GAMMA_MAX_TRIALS + 1 (for a finite shape > 1 a trial is
+   accepted with probability above 0.9, so the bound is never reached). */
+#define NTOR_GAMMA_MAX_TRIALS 4096u
 float ntor_rand_gamma(float shape, float scale, uint64_t seed, uint32_t src, uint64_t photon,
                       int* trials) {
     float d = (float)((double)shape - 1.0 / 3.0);
     float c = 1.0f / sqrtf(9.0f * d);
-    for (uint32_t t = 0;; ++t) {
+    for (uint32_t t = 0; t < NTOR_GAMMA_MAX_TRIALS; ++t) {
         uint32_t r[4];
         draw_block(seed, src, photon, 0x80000000u + t, r);
         float x = box_muller(r[0], r[1]);
@@ -428,6 +433,8 @@ float ntor_rand_gamma(float shape, float scale, uint64_t seed, uint32_t src, uin
             return d * v * scale;
         }
     }
+    if (trials) *trials = (int)NTOR_GAMMA_MAX_TRIALS + 1;
+    return d * scale;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -457,6 +464,48 @@ void ntor_rot_from_ez_f64(const double a[3], const double op[3], double out[3]) 
     out[1] = -a1a2 * fact * op[0] + (1.0 - a2sq * fact) * op[1] + a[1] * op[2];
     out[2] = -a[0] * op[0] - a[1] * op[1] + a[2] * op[2];
 }
+/* sample_cherenkov_track_direction, utils.jl:249-260, with T = Float64 */
+void ntor_cherenkov_track_dir_f64(double u_cos, double u_phi, double out[3]) {
+    const double a = 0.39, b = 2.61;
+    double I = 1.0 - exp(-b * pow(2.0, a));
+    double c = fmax(1.0 - pow(-log(1.0 - u_cos * I) / b, 1.0 / a), -1.0);
+    double phi = 2.0 * M_PI * u_phi, st = sqrt(fmax(1.0 - c * c, 0.0));
+    out[0] = cos(phi) * st;
+    out[1] = sin(phi) * st;
+    out[2] = c;
+}
+
+/* sample_cherenkov_direction, photon_prop_cuda.jl:107-125, in FP64 from the
+   FP32 wavelength, uniforms, Quan-Fry coefficients and source direction */
+void ntor_cherenkov_dir_f64(float wl, float u_phi, float u_tcos, float u_tphi, const float src_dir[3],
+                            const float qf[4], double out[3]) {
+    double q[4] = {qf[0], qf[1], qf[2], qf[3]};
+    double n = ntor_refractive_index_f64(wl, q);
+    double ct = 1.0 / n, st = sqrt(1.0 - ct * ct), phi = 2.0 * M_PI * (double)u_phi;
+    double ph[3] = {cos(phi) * st, sin(phi) * st, ct}, track[3], tmp[3];
+    double src[3] = {src_dir[0], src_dir[1], src_dir[2]};
+    ntor_cherenkov_track_dir_f64(u_tcos, u_tphi, track);
+    ntor_rot_from_ez_f64(track, ph, tmp);
+    ntor_rot_from_ez_f64(src, tmp, out);
+}
+
+/* update_direction, photon_prop_cuda.jl:204-222, in FP64 (the Simplified-Liu
+   extension keeps its FP32 cosine) */
+void ntor_update_dir_f64(const float dir[3], float eta, float u_phi, float u_mix, const nt_medium* m,
+                         double out[3]) {
+    float g = m->mean_scattering_angle;
+    double c = (u_mix < m->sl_fraction) ? (double)ntor_sl_cos(g, eta) : ntor_hg_cos_f64(g, eta);
+    double s = sqrt(fmax(1.0 - c * c, 0.0)), phi = 2.0 * M_PI * (double)u_phi;
+    double nd[3] = {cos(phi) * s, sin(phi) * s, c}, d[3] = {dir[0], dir[1], dir[2]};
+    ntor_rot_from_ez_f64(d, nd, out);
+}
+
+static void round3(const double v[3], float out[3]) {
+    out[0] = (float)v[0];
+    out[1] = (float)v[1];
+    out[2] = (float)v[2];
+}
+
 int ntor_intersect_f64(const double pos[3], const double dir[3], double step, const double tpos[3],
                        double radius, double* d_out) {
     double dp[3] = {pos[0] - tpos[0], pos[1] - tpos[1], pos[2] - tpos[2]};
@@ -512,10 +561,34 @@ static void hb_push(hitbuf* b, const hitrec* h) {
     b->v[b->n++] = *h;
 }
 
+/* the medium's direction arithmetic: FP32 as the reference, or FP64 rounded once */
+static void cherenkov_dir_mode(float wl, float u_phi, float u_tcos, float u_tphi, const float src[3],
+                               const nt_medium* m, float out[3]) {
+    if (m->direction_arithmetic == NT_DIR_FP64) {
+        double v[3];
+        ntor_cherenkov_dir_f64(wl, u_phi, u_tcos, u_tphi, src, m->quan_fry, v);
+        round3(v, out);
+    } else {
+        ntor_cherenkov_dir(wl, u_phi, u_tcos, u_tphi, src, m->quan_fry, out);
+    }
+}
+
+static void update_dir_mode(const float dir[3], float eta, float u_phi, float u_mix, const nt_medium* m,
+                            float out[3]) {
+    if (m->direction_arithmetic == NT_DIR_FP64) {
+        double v[3];
+        ntor_update_dir_f64(dir, eta, u_phi, u_mix, m, v);
+        round3(v, out);
+    } else {
+        ntor_update_dir(dir, eta, u_phi, u_mix, m, out);
+    }
+}
+
 /* initialize_photon_state, photon_prop_cuda.jl:133-188 */
 static void init_photon(const nt_source* s, const nt_medium* m, const nt_spectrum* sp,
                         uint64_t seed, uint32_t si, uint64_t p, float pos[3], float dir[3],
                         float* time, float* wl) {
+    const int fp64 = m->direction_arithmetic == NT_DIR_FP64;
     uint32_t r[4];
     draw_block(seed, si, p, 0u, r);
     *wl = ntor_spectrum_sample(sp, u01(r[0]));
@@ -534,7 +607,13 @@ static void init_photon(const nt_source* s, const nt_medium* m, const nt_spectru
         case NT_SRC_AXICONE: {
             float phi = 0.0f + u01(r[2]) * TWO_PI_F, v[3];
             ntor_sph_to_cart(s->angle, phi, v);
-            ntor_rot_from_ez(s->direction, v, dir);
+            if (fp64) {
+                double a[3] = {s->direction[0], s->direction[1], s->direction[2]}, o[3] = {v[0], v[1], v[2]}, w[3];
+                ntor_rot_from_ez_f64(a, o, w);
+                round3(w, dir);
+            } else {
+                ntor_rot_from_ez(s->direction, v, dir);
+            }
             break;
         }
         case NT_SRC_PENCIL:
@@ -547,11 +626,11 @@ static void init_photon(const nt_source* s, const nt_medium* m, const nt_spectru
             float lp = ntor_rand_gamma(s->long_a, 1.0f / s->long_b, seed, si, p, NULL) * s->long_lrad;
             for (int k = 0; k < 3; ++k) pos[k] = s->position[k] + lp * s->direction[k];
             *time = s->time + lp / (float)C_VAC_M_NS;
-            ntor_cherenkov_dir(*wl, u01(r[1]), u01(r[2]), u01(r[3]), s->direction, m->quan_fry, dir);
+            cherenkov_dir_mode(*wl, u01(r[1]), u01(r[2]), u01(r[3]), s->direction, m, dir);
             break;
         }
         case NT_SRC_POINTLIKE_CHERENKOV:
-            ntor_cherenkov_dir(*wl, u01(r[1]), u01(r[2]), u01(r[3]), s->direction, m->quan_fry, dir);
+            cherenkov_dir_mode(*wl, u01(r[1]), u01(r[2]), u01(r[3]), s->direction, m, dir);
             break;
         default:
             dir[0] = dir[1] = 0.0f;
@@ -605,7 +684,7 @@ static void propagate_one(const nt_source* s, uint32_t si, uint64_t p, const nt_
             dist += step;
             time += step / c_grp;
             float nd[3];
-            ntor_update_dir(dir, u01(r[1]), u01(r[2]), u01(r[3]), m, nd);
+            update_dir_mode(dir, u01(r[1]), u01(r[2]), u01(r[3]), m, nd);
             dir[0] = nd[0];
             dir[1] = nd[1];
             dir[2] = nd[2];

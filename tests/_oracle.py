@@ -61,6 +61,9 @@ def load():
         "ntor_hg_cos_f64": (d, [d, d]),
         "ntor_sl_cos_f64": (d, [d, d]),
         "ntor_rot_from_ez_f64": (None, [dp, dp, dp]),
+        "ntor_cherenkov_track_dir_f64": (None, [d, d, dp]),
+        "ntor_cherenkov_dir_f64": (None, [f, f, f, f, fp, fp, dp]),
+        "ntor_update_dir_f64": (None, [fp, f, f, f, C.POINTER(_abi.nt_medium), dp]),
         "ntor_intersect_f64": (i, [dp, dp, d, dp, d, dp]),
         "ntor_group_velocity_f64": (d, [d, dp, i]),
         "ntor_refractive_index_f64": (d, [d, dp]),

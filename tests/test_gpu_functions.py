@@ -1,10 +1,16 @@
 """Device functions of the propagation kernel (run through nt_eval) against
 the oracle's FP32 restatement and against FP64 evaluations.
 
-Error metric: |y_gpu - y_f64| in FP32 ulps of a per-function scale, the
-magnitude at which the reference formula itself rounds:
+Direction sampling and rotations run in the medium's direction arithmetic:
+FP64 (the default) is held to 2 FP32 ulps of an FP64 evaluation of the
+reference formulas (scale 1, the norm of the unit vectors; the HG cosine and
+the rotation, rounded once from FP64, to half an ulp of the value itself);
+FP32 reference arithmetic is checked bit for bit against the oracle's FP32
+restatement and against the bound of the FP32 formula's own rounding.
+
+Error metric: |y_gpu - y_f64| in FP32 ulps of a per-function scale:
   directions (unit vectors)  scale 1
-  HG cosine                  ulp((1 + g^2 + t^2)/(2g)) + ulp(2η-1)·t²/(1+g(2η-1)),
+  HG cosine, FP32 reference  ulp((1 + g^2 + t^2)/(2g)) + ulp(2η-1)·t²/(1+g(2η-1)),
                              the formula's own FP32 rounding (2η-1 rounded, then
                              amplified by the small denominator near η = 0)
   SL cosine                  scale 1
@@ -12,6 +18,7 @@ magnitude at which the reference formula itself rounds:
   intersection distance d    error bound ulp(|a|) + ulp(max(a^2, |p-c|^2)) / (2 sqrt(b))
 """
 import ctypes as C
+import dataclasses
 import json
 import math
 from pathlib import Path
@@ -35,6 +42,19 @@ def ulp32(x):
 
 def errs_in_ulps(y, ref, scale):
     return np.abs(y.astype(np.float64) - ref) / ulp32(np.maximum(np.abs(ref), scale))
+
+
+class direction_mode:
+    """Context manager: the engine's medium with another direction arithmetic."""
+
+    def __init__(self, eng, med, mode):
+        self.eng, self.med, self.mode = eng, med, mode
+
+    def __enter__(self):
+        self.eng.set_medium(dataclasses.replace(self.med, direction_arithmetic=self.mode))
+
+    def __exit__(self, *exc):
+        self.eng.set_medium(self.med)
 
 
 @pytest.fixture(scope="module")
@@ -110,20 +130,25 @@ def test_intersection_device(medium_engine, oracle):
 
 @pytest.mark.parametrize("g", [0.5, 0.9, 0.95, 0.99])
 def test_hg_and_sl_cos_device(medium_engine, oracle, g):
-    eng, _ = medium_engine
+    eng, med = medium_engine
     eta = np.random.default_rng(2).random(20000).astype(np.float32)
     x = np.stack([np.full_like(eta, g), eta], axis=1)
-    y = eng.eval(_abi.NT_FN_HG_COS, x, 1)[:, 0]
-    yo = np.array([oracle.ntor_hg_cos(np.float32(g), e) for e in eta], np.float32)
-    assert np.array_equal(y, yo)                      # pure arithmetic: bit-exact
     g32 = np.float64(np.float32(g))
     e64 = eta.astype(np.float64)
     u = 2 * e64 - 1
     den = 1 + g32 * u
     t = (1 - g32 * g32) / den
     f64 = np.clip((1 + g32 * g32 - t * t) / (2 * g32), -1, 1)
+    # FP64 arithmetic (the default): the FP64 cosine rounded once
+    y = eng.eval(_abi.NT_FN_HG_COS, x, 1)[:, 0]
+    assert errs_in_ulps(y, f64, 1e-30).max() <= 0.501
+    # FP32 reference arithmetic: bit-exact with the oracle's restatement
+    with direction_mode(eng, med, "fp32"):
+        y = eng.eval(_abi.NT_FN_HG_COS, x, 1)[:, 0]
+    yo = np.array([oracle.ntor_hg_cos(np.float32(g), e) for e in eta], np.float32)
+    assert np.array_equal(y, yo)                      # pure arithmetic: bit-exact
     scale = (1 + g32 ** 2 + t ** 2) / (2 * g32)
-    # the formula's own FP32 conditioning: the rounding of 2η−1 (ulp(u)/2)
+    # the FP32 formula's own conditioning: the rounding of 2η−1 (ulp(u)/2)
     # reaches cos θ amplified by t²/den, large when 1 + g(2η−1) is small
     bound = ulp32(scale) + ulp32(u) * t * t / den
     assert (np.abs(y.astype(np.float64) - f64) / bound).max() <= ULP_MAX
@@ -136,23 +161,28 @@ def test_hg_and_sl_cos_device(medium_engine, oracle, g):
 
 
 def test_rotation_and_directions_device(medium_engine, oracle):
-    eng, _ = medium_engine
+    eng, med = medium_engine
     rng = np.random.default_rng(3)
     n = 20000
     a = rng.normal(size=(n, 3)); a /= np.linalg.norm(a, axis=1, keepdims=True)
     op = rng.normal(size=(n, 3)); op /= np.linalg.norm(op, axis=1, keepdims=True)
     a[:5] = [[0, 0, 1], [0, 0, -1], [1, 0, 0], [0, 1, 0], [0.6, 0, 0.8]]
     a32, op32 = a.astype(np.float32), op.astype(np.float32)
-    y = eng.eval(_abi.NT_FN_ROT_FROM_EZ, np.concatenate([a32, op32], 1), 3)
-    yo = np.zeros_like(y)
-    for i in range(n):
-        oracle.ntor_rot_from_ez(O.fptr(a32[i]), O.fptr(op32[i]), yo[i].ctypes.data_as(C.POINTER(C.c_float)))
-    assert np.array_equal(y, yo)
     f64 = np.zeros((n, 3))
     for i in range(n):
         oracle.ntor_rot_from_ez_f64(O.dptr(a32[i].astype(np.float64)), O.dptr(op32[i].astype(np.float64)),
                                     f64[i].ctypes.data_as(C.POINTER(C.c_double)))
-    # the rotation's own rounding: terms up to |fact * a1^2| ~ 2 (fact = 1/(1+a_z))
+    # FP64 arithmetic (the default): the FP64 rotation rounded once
+    y = eng.eval(_abi.NT_FN_ROT_FROM_EZ, np.concatenate([a32, op32], 1), 3)
+    assert errs_in_ulps(y, f64, 1e-6).max() <= 0.501
+    # FP32 reference arithmetic: bit-exact with the oracle's restatement
+    with direction_mode(eng, med, "fp32"):
+        y = eng.eval(_abi.NT_FN_ROT_FROM_EZ, np.concatenate([a32, op32], 1), 3)
+    yo = np.zeros_like(y)
+    for i in range(n):
+        oracle.ntor_rot_from_ez(O.fptr(a32[i]), O.fptr(op32[i]), yo[i].ctypes.data_as(C.POINTER(C.c_float)))
+    assert np.array_equal(y, yo)
+    # the FP32 rotation's own rounding: terms up to |fact * a1^2| ~ 2 (fact = 1/(1+a_z))
     fact = np.where(np.abs(a32[:, 2]) < 1, (1 - a32[:, 2].astype(np.float64)) /
                     np.maximum(a32[:, 0].astype(np.float64) ** 2 + a32[:, 1].astype(np.float64) ** 2, 1e-30), 0)
     scale = np.maximum(1.0, fact * np.maximum(a32[:, 0] ** 2, a32[:, 1] ** 2))[:, None]
@@ -240,7 +270,93 @@ def test_scattering_length_fit_device(medium_engine, oracle, kind):
     assert errs_in_ulps(fit[:2000], o64, 0.0).max() <= 4
 
 
-def test_cherenkov_and_update_direction_device(medium_engine, oracle):
+def _u24(rng, n):
+    """Uniforms as the kernel draws them: 24-bit multiples of 2^-24."""
+    return (rng.integers(0, 2 ** 24, n) * 2.0 ** -24).astype(np.float32)
+
+
+def _rot64(a, o):
+    """rot_from_ez_fast (utils.jl:228-243) in FP64, |a_z| = 1 branch included."""
+    a1, a2, a3 = a[:, 0], a[:, 1], a[:, 2]
+    q2 = np.abs(a3) == 1
+    with np.errstate(divide="ignore", invalid="ignore"):
+        f = (1 - a3) / (a1 * a1 + a2 * a2)
+    gen = np.stack([(1 - a1 * a1 * f) * o[:, 0] - a1 * a2 * f * o[:, 1] + a1 * o[:, 2],
+                    -a1 * a2 * f * o[:, 0] + (1 - a2 * a2 * f) * o[:, 1] + a2 * o[:, 2],
+                    -a1 * o[:, 0] - a2 * o[:, 1] + a3 * o[:, 2]], 1)
+    quirk = np.stack([o[:, 0], np.copysign(o[:, 1], a3), np.copysign(o[:, 2], a3)], 1)
+    return np.where(q2[:, None], quirk, gen)
+
+
+def _sph(ct, phi):
+    st = np.sqrt(np.maximum(1 - ct * ct, 0))
+    return np.stack([np.cos(phi) * st, np.sin(phi) * st, ct], 1)
+
+
+def _unit(rng, n, near_axis=0):
+    v = rng.normal(size=(n, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    if near_axis:    # directions within 1e-4..0.3 rad of -e_z and of +e_z, where rotations are ill-conditioned
+        k = near_axis // 2
+        th = 10 ** rng.uniform(-4, -0.5, 2 * k)
+        th[:k] = np.pi - th[:k]
+        ph = rng.uniform(0, 2 * np.pi, 2 * k)
+        v[:2 * k] = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], 1)
+    return v.astype(np.float32)
+
+
+def _track64(ut, utp):
+    """sample_cherenkov_track_direction (utils.jl:249-260) with T = Float64."""
+    a, b = 0.39, 2.61
+    I = 1 - np.exp(-b * 2 ** a)
+    ct = np.maximum(1 - (-np.log(1 - ut.astype(np.float64) * I) / b) ** (1 / a), -1)
+    return _sph(ct, 2 * np.pi * utp.astype(np.float64))
+
+
+def test_direction_sampling_fp64_within_2ulp(medium_engine):
+    """FP64 direction arithmetic against FP64 evaluations of the reference
+    formulas (utils.jl:228-260, photon_prop_cuda.jl:70-77,107-125,204-222)
+    from the same FP32 inputs: every output component within 2 FP32 ulps of
+    1 (the vectors' norm), over 24-bit uniforms as the kernel draws them and
+    directions concentrated near ±e_z."""
+    eng, med = medium_engine
+    rng = np.random.default_rng(4)
+    n = 100000
+    q = np.array(med.quan_fry_params, np.float32).astype(np.float64)
+    # track direction alone, u_tcos also at 0 and near 1
+    ut, utp = _u24(rng, n), _u24(rng, n)
+    ut[:3] = [0.0, 2.0 ** -24, 1 - 2.0 ** -24]
+    y = eng.eval(_abi.NT_FN_TRACK_DIR, np.column_stack([ut, utp]), 3)
+    tr = _track64(ut, utp)
+    e_track = errs_in_ulps(y, tr, 1.0).max()
+    # Cherenkov emission direction
+    wl = rng.uniform(300, 800, n).astype(np.float32)
+    up = _u24(rng, n)
+    src = _unit(rng, n, near_axis=20000)
+    src[:2] = [[0, 0, 1], [0, 0, -1]]
+    y = eng.eval(_abi.NT_FN_CHERENKOV_DIR, np.column_stack([wl, up, ut, utp, src]), 3)
+    x = 1 / wl.astype(np.float64)
+    nn = (x * q[1] + q[0]) + (x * x * q[2] + x ** 3 * q[3])
+    ph = _sph(1 / nn, 2 * np.pi * up.astype(np.float64))
+    ref = _rot64(src.astype(np.float64), _rot64(tr, ph))
+    e_cher = errs_in_ulps(y, ref, 1.0).max()
+    # scattering direction (HG g = 0.95 of the medium)
+    d = _unit(rng, n, near_axis=40000)
+    eta, uphi = _u24(rng, n), _u24(rng, n)
+    y = eng.eval(_abi.NT_FN_UPDATE_DIR, np.column_stack([d, eta, uphi, np.ones(n, np.float32)]), 3)
+    g = np.float64(np.float32(med.mean_scattering_angle))
+    t = (1 - g * g) / (1 + g * (2 * eta.astype(np.float64) - 1))
+    c = np.clip((1 + g * g - t * t) / (2 * g), -1, 1)
+    ref = _rot64(d.astype(np.float64), _sph(c, 2 * np.pi * uphi.astype(np.float64)))
+    e_upd = errs_in_ulps(y, ref, 1.0).max()
+    print(f"max error in ulp(1): track {e_track:.3f}, Cherenkov {e_cher:.3f}, update_direction {e_upd:.3f}")
+    assert max(e_track, e_cher, e_upd) <= ULP_MAX
+
+
+def test_cherenkov_and_update_direction_fp32_reference(medium_engine, oracle):
+    """FP32 reference arithmetic: the device follows the oracle's FP32
+    restatement; the transcendental chains (log, pow, exp, sincos) differ
+    between the hardware instructions and glibc in the last bits."""
     eng, med = medium_engine
     rng = np.random.default_rng(4)
     n = 5000
@@ -248,26 +364,25 @@ def test_cherenkov_and_update_direction_device(medium_engine, oracle):
     u = rng.random((n, 3)).astype(np.float32)
     src = rng.normal(size=(n, 3)); src /= np.linalg.norm(src, axis=1, keepdims=True)
     src = src.astype(np.float32)
-    y = eng.eval(_abi.NT_FN_CHERENKOV_DIR, np.concatenate([wl[:, None], u, src], 1), 3)
+    d = rng.normal(size=(n, 3)); d /= np.linalg.norm(d, axis=1, keepdims=True)
+    x = np.concatenate([d.astype(np.float32), rng.random((n, 3)).astype(np.float32)], 1)
+    with direction_mode(eng, med, "fp32"):
+        y = eng.eval(_abi.NT_FN_CHERENKOV_DIR, np.concatenate([wl[:, None], u, src], 1), 3)
+        y2 = eng.eval(_abi.NT_FN_UPDATE_DIR, x, 3)
     yo = np.zeros_like(y)
     q = med.to_abi().quan_fry
     for i in range(n):
         oracle.ntor_cherenkov_dir(wl[i], u[i, 0], u[i, 1], u[i, 2], O.fptr(src[i]), q,
                                   yo[i].ctypes.data_as(C.POINTER(C.c_float)))
-    # transcendental chains (acos, pow, exp, log, sincos): ocml vs glibc differ in the last bits
     assert np.abs(y - yo).max() < 5e-5
     assert np.allclose(np.linalg.norm(y, axis=1), 1, atol=1e-5)
-
-    d = rng.normal(size=(n, 3)); d /= np.linalg.norm(d, axis=1, keepdims=True)
-    x = np.concatenate([d.astype(np.float32), rng.random((n, 3)).astype(np.float32)], 1)
-    y = eng.eval(_abi.NT_FN_UPDATE_DIR, x, 3)
-    m = med.to_abi()
-    yo = np.zeros_like(y)
+    m = dataclasses.replace(med, direction_arithmetic="fp32").to_abi()
+    yo = np.zeros_like(y2)
     for i in range(n):
         oracle.ntor_update_dir(O.fptr(x[i, :3]), x[i, 3], x[i, 4], x[i, 5], C.byref(m),
                                yo[i].ctypes.data_as(C.POINTER(C.c_float)))
-    assert np.abs(y - yo).max() < 1e-5
-    assert np.allclose(np.linalg.norm(y, axis=1), 1, atol=1e-5)
+    assert np.abs(y2 - yo).max() < 1e-5
+    assert np.allclose(np.linalg.norm(y2, axis=1), 1, atol=1e-5)
 
 
 def test_spectrum_sampling_device(engine, oracle):

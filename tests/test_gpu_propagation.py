@@ -9,6 +9,7 @@ at least 99.5 % of each side's hits must match and matched hits agree within
 reference's Monte Carlo would be: per-module counts within 3σ Poisson and
 two-sample KS tests on arrival time and wavelength at p > 0.01.
 """
+import dataclasses
 import math
 
 import numpy as np
@@ -231,6 +232,107 @@ def test_invalid_arguments(engine):
         assert lib.nt_set_targets(engine.handle, _abi.as_c_array(_abi.nt_target, [t]), 1) == _abi.NT_ERR_ARG
 
 
+def test_sources_reject_non_finite_fields(engine):
+    """Infinite or NaN source fields are refused by both source paths and by
+    the server (an infinite longitudinal parameter would make every Γ trial
+    fail), so no kernel is launched with them."""
+    torch = pytest.importorskip("torch")
+    lib = engine.lib
+    engine.set_medium(nt.make_cascadia_medium_properties(0.95))
+    engine.set_spectrum(nt.Monochromatic(400.0))
+    engine.set_targets([nt.DetectionSphere(np.array([0, 0, 5.0], np.float32), 0.5, 16, PMT_AREA, 1)])
+    lp = nt.LongitudinalParameterisation(6.0, 0.63, 0.36)
+    base = nt.ExtendedCherenkovEmitter(np.zeros(3, np.float32), np.array([1, 0, 0], np.float32), 0.0, 100, lp)
+    good = base.to_abi()
+    arr = _abi.as_c_array(_abi.nt_source, [good])
+    assert lib.nt_set_sources(engine.handle, arr, 1) == _abi.NT_OK
+    cases = [("long_a", float("inf")), ("long_b", float("nan")), ("long_lrad", float("inf")),
+             ("long_b", 1e-45), ("time", float("nan")), ("time_end", float("inf")), ("angle", float("nan"))]
+    cases += [("position", i) for i in range(3)] + [("direction", i) for i in range(3)]
+    for field, v in cases:
+        src = base.to_abi()
+        if field in ("position", "direction"):
+            getattr(src, field)[v] = float("inf")
+        else:
+            setattr(src, field, v)
+        arr = _abi.as_c_array(_abi.nt_source, [good, src])
+        assert lib.nt_set_sources(engine.handle, arr, 2) == _abi.NT_ERR_ARG, field
+        raw = np.frombuffer(bytes(arr), np.uint8)
+        d = torch.from_numpy(raw.copy()).to("cuda:0")
+        st = lib.nt_set_sources_device(engine.handle, d.data_ptr(), 2, -1, None)
+        assert st == _abi.NT_ERR_ARG, field
+        assert "source 1" in lib.nt_status_string(st).decode()
+    engine.set_sources([base])                            # the context still works
+    assert engine.propagate(seed=1).n_simulated == 100
+
+
+def test_single_target_column_swap(engine):
+    """Q4 (photon_prop_cuda.jl:590-592, taken for one target at :806-808): with
+    one target propagate_photons returns the initial direction in `direction`
+    and the final direction in `initial_direction`; with a second (never hit)
+    target the same photons come back with the columns as named."""
+    s1 = iso_sphere_setup(300_000, r=1.0)
+    far = nt.DetectionSphere(np.array([0, 0, 1e4], np.float32), 0.1, 16, PMT_AREA, 2)
+    s2 = nt.PhotonPropSetup(s1.sources, s1.targets + [far], s1.medium, s1.spectrum)
+    one = nt.propagate_photons(s1).sorted()
+    two = nt.propagate_photons(s2).sorted()
+    assert len(one) > 100 and np.array_equal(one.photon_id, two.photon_id)
+    assert set(two.module_id) == {1}
+    assert np.array_equal(one.direction, two.initial_direction)
+    assert np.array_equal(one.initial_direction, two.direction)
+    assert not np.array_equal(two.direction, two.initial_direction)       # scattered photons
+    plain = nt.propagate_photons(s1, single_target_swap=False).sorted()
+    assert np.array_equal(plain.direction, two.direction)
+
+
+@pytest.mark.parametrize("arith", ["fp32", "fp64"])
+def test_scattered_history_brute_force_device(engine, arith):
+    """The kernel against a brute-force FP64 history of one photon (its Philox
+    words, two scatterings, a small sphere centred on the third leg), as the
+    oracle is pinned in test_oracle.py."""
+    from test_oracle import _brute_force_history
+    seed, photon = 0x1234_5678_9ABC, 77
+    med = nt.make_cascadia_medium_properties(0.95, sca_len_scale=0.1, direction_arithmetic=arith)
+    wl = np.float32(400.0)
+    L = float(nt.scattering_length(wl, med))
+    cg = float(nt.group_velocity(wl, med))
+    legs = _brute_force_history(seed, photon, L, float(np.float32(0.95)), 3)
+    p3, d3, s3 = legs[2]
+    r = 0.02
+    tgt = nt.DetectionSphere((p3 + d3 * (s3 / 2)).astype(np.float32), r, 1, 1.0, 5)
+    s = nt.PhotonPropSetup([nt.PointlikeIsotropicEmitter(np.zeros(3), 0.0, photon + 1)], [tgt], med,
+                           nt.Monochromatic(400.0))
+    h = run_gpu(engine, s, seed=seed)
+    k = np.flatnonzero(h.photon_id == photon)
+    assert len(k) == 1
+    i = int(k[0])
+    dist = legs[0][2] + legs[1][2] + (s3 / 2 - r)
+    tol = 2e-4 if arith == "fp32" else 2e-5
+    assert float(h.dist_travelled[i]) == pytest.approx(dist, rel=tol)
+    assert np.allclose(h.position[i], p3 + d3 * (s3 / 2 - r), atol=tol * (1 + dist))
+    assert float(h.time[i]) == pytest.approx(dist / cg, rel=tol)
+    assert np.allclose(h.direction[i], d3, atol=tol)
+
+
+def test_fp32_reference_arithmetic_per_photon(engine):
+    """The FP32 reference direction arithmetic against the oracle's FP32
+    restatement, photon for photon (cascade on the 20-module string and an
+    axicone source)."""
+    s = cascade_string_setup(2_000_000, radius=1.0)
+    s = nt.PhotonPropSetup(s.sources, s.targets, dataclasses.replace(s.medium, direction_arithmetic="fp32"),
+                           s.spectrum)
+    g = run_gpu(engine, s, seed=6)
+    o = O.propagate(s, seed=6)
+    assert len(o) > 200
+    assert_hits_match(g, o)
+    med = nt.make_cascadia_medium_properties(0.9, direction_arithmetic="fp32")
+    src = nt.AxiconeEmitter(np.array([0.5, -0.3, 0.2], np.float32), np.array([0.0, 0.6, 0.8], np.float32),
+                            1.0, 300_000, 0.7)
+    tg = [nt.DetectionSphere(np.array([0, 6.0, 8.0], np.float32), 1.0, 16, PMT_AREA, 7)]
+    s = nt.PhotonPropSetup([src], tg, med, nt.CherenkovSpectrum((300.0, 800.0), 30, med))
+    assert_hits_match(run_gpu(engine, s, seed=9), O.propagate(s, seed=9))
+
+
 def test_isotropic_sphere_1e8_properties(engine):
     """Isotropic source, one sphere at 10 m, 1e8 photons: size-independent properties."""
     n = 100_000_000
@@ -280,7 +382,9 @@ def test_device_launch_with_torch_buffers(engine):
     _abi.check(engine.lib.nt_launch(engine.handle, 4, h, counters.data_ptr(), stream.cuda_stream), "nt_launch")
     torch.cuda.synchronize()
     c = counters.cpu().numpy()
-    assert c[0] == len(host) and c[1] == 500_000 and c[2] == 500_000 * 15 or c[2] > 0
+    assert c[0] == len(host)
+    assert c[1] == 500_000
+    assert c[2] > 0 or c[3] > 0
     n = int(c[0])
     pid = cols["photon_id"][:n].cpu().numpy().astype(np.uint64)
     assert np.array_equal(np.sort(pid), np.sort(host.photon_id))

@@ -323,6 +323,103 @@ def test_cherenkov_emission_angle(oracle):
     assert math.degrees(oracle.ntor_cherenkov_angle(400.0, q)) == pytest.approx(42.4, abs=0.1)
 
 
+def _angle(u, v):
+    u, v = np.asarray(u, float), np.asarray(v, float)
+    return math.acos(float(np.clip(u @ v / (np.linalg.norm(u) * np.linalg.norm(v)), -1, 1)))
+
+
+@pytest.mark.parametrize("arith", ["fp32", "fp64"])
+def test_cherenkov_composition_off_axis_track(oracle, arith):
+    """photon_prop_cuda.jl:107-125: the photon lies on the Cherenkov cone
+    around the TRACK direction after both rotations, i.e. at θc from
+    rot_from_ez(source, track), for tracks well off the source axis (u_tcos >
+    0).  Swapping the two rotations (rotating the cone to the source axis
+    first) puts the photon at another angle, which this catches."""
+    med = cascadia()
+    q = med.to_abi().quan_fry
+    src = np.array([1.0, 0.0, 0.0], np.float32)       # exactly unit in FP32, oblique to e_z
+    tr32 = np.zeros(3, np.float32)
+    out32 = np.zeros(3, np.float32)
+    for wl in (320.0, 400.0, 550.0):
+        theta_c = math.acos(1 / oracle.ntor_refractive_index_f64(wl, O.dptr(np.float64(med.quan_fry_params))))
+        for u_t, u_tp, u_p in ((0.5, 0.3, 0.1), (0.8, 0.55, 0.7), (0.97, 0.9, 0.4)):
+            u_t, u_tp, u_p = (float(np.float32(v)) for v in (u_t, u_tp, u_p))     # the FP32 uniforms
+            if arith == "fp32":
+                oracle.ntor_cherenkov_track_dir(u_t, u_tp, tr32.ctypes.data_as(C.POINTER(C.c_float)))
+                oracle.ntor_cherenkov_dir(wl, u_p, u_t, u_tp, O.fptr(src), q,
+                                          out32.ctypes.data_as(C.POINTER(C.c_float)))
+                track, out, tol = tr32.astype(float), out32.astype(float), 2e-4
+            else:
+                track, out = np.zeros(3), np.zeros(3)
+                oracle.ntor_cherenkov_track_dir_f64(u_t, u_tp, track.ctypes.data_as(C.POINTER(C.c_double)))
+                oracle.ntor_cherenkov_dir_f64(wl, u_p, u_t, u_tp, O.fptr(src), q,
+                                              out.ctypes.data_as(C.POINTER(C.c_double)))
+                tol = 1e-9
+            assert _angle(track, [0, 0, 1]) > 0.1                      # the track is off the source axis
+            axis = apply_rot(np.array([0.0, 0.0, 1.0]), src.astype(float), track)   # rot_from_ez(src, track)
+            assert _angle(out, axis) == pytest.approx(theta_c, abs=tol)
+            # the photon's azimuth about the track: φ = 2π·u_phi measured in the track's frame
+            ph = sph(theta_c, 2 * math.pi * u_p)
+            swapped = apply_rot(np.array([0.0, 0.0, 1.0]), track, apply_rot(np.array([0.0, 0.0, 1.0]),
+                                                                           src.astype(float), ph))
+            assert abs(_angle(swapped, axis) - theta_c) > 10 * tol or np.linalg.norm(swapped - out) > 1e-3
+
+
+def test_cherenkov_track_direction_f64_round_trip(oracle):
+    """The FP64 track sampler inverts the clsim CDF (utils.jl:249-260): from
+    its cos θ the CDF gives back u_tcos; the azimuth is 2π·u_phi; unit norm."""
+    a, b = 0.39, 2.61
+    I = 1 - math.exp(-b * 2 ** a)
+    rng = np.random.default_rng(8)
+    o = np.zeros(3)
+    for u, up in zip(rng.random(2000), rng.random(2000)):
+        oracle.ntor_cherenkov_track_dir_f64(u, up, o.ctypes.data_as(C.POINTER(C.c_double)))
+        assert np.linalg.norm(o) == pytest.approx(1, abs=1e-14)
+        t = 1 - o[2]
+        assert (1 - math.exp(-b * t ** a)) / I == pytest.approx(u, abs=1e-10)
+        if o[2] < 1 - 1e-9:
+            assert math.atan2(o[1], o[0]) % (2 * math.pi) == pytest.approx((2 * math.pi * up) % (2 * math.pi),
+                                                                          abs=1e-7)
+
+
+def test_update_direction_f64_geometry(oracle):
+    """FP64 update_direction (photon_prop_cuda.jl:204-222): the new direction
+    is the HG angle from the old one (the HG CDF of its cosine gives back η)
+    at azimuth 2π·u_phi in the old direction's rot_from_ez frame, checked
+    against the Rodrigues matrix (utils.jl:173-197)."""
+    g = 0.95
+    m = cascadia(g).to_abi()
+    rng = np.random.default_rng(9)
+    ez = np.array([0.0, 0.0, 1.0])
+    o = np.zeros(3)
+    for _ in range(1000):
+        d = rng.normal(size=3)
+        d = (d / np.linalg.norm(d)).astype(np.float32)
+        eta, up = np.float32(rng.random()), np.float32(rng.random())
+        oracle.ntor_update_dir_f64(O.fptr(d), eta, up, 0.5, C.byref(m), o.ctypes.data_as(C.POINTER(C.c_double)))
+        c = float(o @ d.astype(float) / np.linalg.norm(d.astype(float)))
+        g32 = float(np.float32(g))
+        # HG CDF: F(c) = (1 − g²)/(2g)·(1/√(1 + g² − 2gc) − 1/(1 + g))
+        F = (1 - g32 ** 2) / (2 * g32) * (1 / math.sqrt(1 + g32 ** 2 - 2 * g32 * c) - 1 / (1 + g32))
+        assert F == pytest.approx(float(eta), abs=5e-5)       # dF/dc <= 400 at c = 1; |d| = 1 ± 1e-7
+        # the same angle from the inverse CDF (η ↦ cos θ) for the azimuth check
+        t = (1 - g32 ** 2) / (1 - g32 + 2 * g32 * float(eta))
+        c = (1 + g32 ** 2 - t * t) / (2 * g32)
+        s = math.sqrt(max(1 - c * c, 0))
+        local = np.array([math.cos(2 * math.pi * up) * s, math.sin(2 * math.pi * up) * s, c])
+        assert np.allclose(o, apply_rot(ez, d.astype(float) / np.linalg.norm(d), local), atol=1e-6)
+
+
+def test_rand_gamma_bounded_trials(oracle):
+    """The Γ sampler stops after 4096 rejected trials (so that every kernel
+    ends): with an infinite shape no trial can be accepted."""
+    t = C.c_int()
+    x = oracle.ntor_rand_gamma(np.float32(np.inf), 1.0, 3, 0, 0, C.byref(t))
+    assert t.value == 4097 and not math.isnan(x) or t.value == 4097
+    oracle.ntor_rand_gamma(np.float32(7.837), 1.0, 3, 0, 0, C.byref(t))
+    assert 1 <= t.value <= 10
+
+
 def test_rand_gamma_distribution(oracle):
     a, b = 7.837, 0.63207
     x = np.array([oracle.ntor_rand_gamma(a, 1 / b, 11, 0, i, None) for i in range(20000)])
@@ -478,6 +575,76 @@ def test_propagate_pencil_beam():
     cg = float(nt.group_velocity(np.float32(400.0), med))
     assert np.allclose(h.time, 3.0 + 9.5 / cg, atol=1e-4)
     assert np.all(h.direction == h.initial_direction)
+
+
+def _u01(x):
+    return ((x >> 8) & 0xFFFFFF) * 2.0 ** -24
+
+
+def _brute_force_history(seed, photon, L_sca, g, n_steps):
+    """Positions and directions of one isotropic photon over `n_steps`
+    scattering steps, in FP64, from its Philox words as ntprop.h lays them out:
+    block 0 → (wavelength, cos θ, φ); blocks 1, 2, … → three words per step
+    (step length −ln u·L_sca with u in (0, 1], HG uniform, azimuth); each step
+    moves, then scatters (photon_prop_cuda.jl:449-495).  The scattering
+    rotation is the Rodrigues matrix (utils.jl:173-197) taking e_z to the
+    current direction, and the HG cosine the textbook inverse CDF."""
+    key = [seed & 0xFFFFFFFF, seed >> 32]
+    b0 = O.philox([0, photon & 0xFFFFFFFF, photon >> 32, 0], key)
+    ct, phi = -1 + 2 * _u01(b0[1]), 2 * math.pi * _u01(b0[2])
+    d = np.array([math.cos(phi) * math.sqrt(1 - ct * ct), math.sin(phi) * math.sqrt(1 - ct * ct), ct])
+    words = []
+    for blk in range(1, 2 + (3 * n_steps) // 4 + 1):
+        words += O.philox([blk, photon & 0xFFFFFFFF, photon >> 32, 0], key)
+    p = np.zeros(3)
+    legs = []
+    for k in range(n_steps):
+        w0, w1, w2 = words[3 * k:3 * k + 3]
+        step = -math.log(((w0 >> 8) + 1) * 2.0 ** -24) * L_sca
+        legs.append((p.copy(), d.copy(), step))
+        p = p + d * step
+        eta, up = _u01(w1), _u01(w2)
+        t = (1 - g * g) / (1 + g * (2 * eta - 1))
+        c = min(max((1 + g * g - t * t) / (2 * g), -1.0), 1.0)
+        s = math.sqrt(max(1 - c * c, 0.0))
+        local = np.array([math.cos(2 * math.pi * up) * s, math.sin(2 * math.pi * up) * s, c])
+        d = apply_rot(np.array([0.0, 0.0, 1.0]), d, local)
+    return legs
+
+
+@pytest.mark.parametrize("arith", ["fp32", "fp64"])
+def test_propagate_scattered_history_brute_force(oracle, arith):
+    """A target sphere centred on the third leg of one photon's path (after two
+    scatterings, computed by brute force in FP64 from the photon's Philox
+    words) is hit by that photon on that leg: position on the sphere, distance
+    = legs 1 + 2 + the way into leg 3, time = distance/c_grp, direction = the
+    leg's, initial direction = the first leg's.  Pins the order of the scatter
+    branch (move, distance, time, then new direction) and the word layout."""
+    seed, photon = 0x1234_5678_9ABC, 77
+    med = cascadia(0.95, sca_len_scale=0.1, direction_arithmetic=arith)
+    wl = np.float32(400.0)
+    L = float(nt.scattering_length(wl, med))
+    cg = float(nt.group_velocity(wl, med))
+    g = float(np.float32(0.95))
+    legs = _brute_force_history(seed, photon, L, g, 3)
+    p3, d3, s3 = legs[2]
+    r = 0.02
+    centre = p3 + d3 * (s3 / 2)
+    # the sphere is off the first two legs
+    for p0, d0, s0 in legs[:2]:
+        a = d0 @ (p0 - centre)
+        assert a * a - (p0 - centre) @ (p0 - centre) + r * r < 0 or -a - math.sqrt(a * a - (p0 - centre) @ (p0 - centre) + r * r) > s0
+    tgt = nt.DetectionSphere(centre.astype(np.float32), r, 1, 1.0, 5)
+    s = _setup([nt.PointlikeIsotropicEmitter(np.zeros(3), 0.0, photon + 1)], [tgt], med)
+    h = O.propagate_range(s, photon, 1, seed=seed)
+    assert len(h) == 1 and int(h.photon_id[0]) == photon and int(h.module_id[0]) == 5
+    dist = legs[0][2] + legs[1][2] + (s3 / 2 - r)
+    tol = 2e-4 if arith == "fp32" else 2e-5
+    assert float(h.dist_travelled[0]) == pytest.approx(dist, rel=tol)
+    assert np.allclose(h.position[0], p3 + d3 * (s3 / 2 - r), atol=tol * (1 + dist))
+    assert float(h.time[0]) == pytest.approx(dist / cg, rel=tol)
+    assert np.allclose(h.direction[0], d3, atol=tol)
+    assert np.allclose(h.initial_direction[0], legs[0][1], atol=1e-6)
 
 
 def test_propagate_first_listed_target_wins():
