@@ -222,6 +222,22 @@ int32_t nt_set_sources_device(nt_ctx* ctx, const nt_source* d_sources, int32_t n
 int32_t nt_propagate(nt_ctx* ctx, uint64_t seed, const nt_hits* out_host,
                      int64_t* n_hits, int64_t* n_simulated, int64_t* n_dropped);
 
+/* nt_propagate without any device allocation.  No counterpart in the
+   reference: for callers that cannot allocate device memory (code under HIP
+   graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+   device hit columns and counters live in a file-scope device array of
+   NT_NOALLOC_MAX_HITS hits (9.3 MB, held by every GPU the library runs on
+   until the process ends) instead of buffers the context allocates; the
+   caller passes host columns as for nt_propagate.  The context's medium,
+   spectrum, targets and sources are set beforehand as usual (those calls do
+   allocate).  Returns NT_ERR_ARG for a capacity above NT_NOALLOC_MAX_HITS.
+   Every call on one GPU shares that GPU's array and it keeps the last call's
+   contents: calls through one context are serialised by the context, but
+   calls from two contexts on the same GPU that overlap in time race. */
+#define NT_NOALLOC_MAX_HITS (1 << 17)
+int32_t nt_propagate_noalloc(nt_ctx* ctx, uint64_t seed, const nt_hits* out_host,
+                             int64_t* n_hits, int64_t* n_simulated, int64_t* n_dropped);
+
 /* Asynchronous device form: hits go to caller-owned DEVICE columns, counters to
    `d_counters` (device int64[4]: total hits found, photons simulated, sphere
    tests, string-bound tests), which the call clears first.  `stream` is a hipStream_t

@@ -50,6 +50,7 @@ NT_TARGET_MULTI_PMT = 1
 NT_MAX_PMTS = 16
 NT_MAX_TARGETS = 4096
 NT_MAX_KNOTS = 1024
+NT_NOALLOC_MAX_HITS = 1 << 17
 
 (NT_FN_INTERSECT, NT_FN_HG_COS, NT_FN_SL_COS, NT_FN_ROT_FROM_EZ, NT_FN_SPH_TO_CART,
  NT_FN_ISO_DIR, NT_FN_REFRACTIVE_INDEX, NT_FN_GROUP_VELOCITY, NT_FN_SCA_LEN,
@@ -197,6 +198,8 @@ SIGNATURES = {
     "nt_set_sources": (C.c_int32, [C.c_void_p, C.POINTER(nt_source), C.c_int32]),
     "nt_set_sources_device": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]),
     "nt_propagate": (C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(nt_hits), C.POINTER(C.c_int64),
+                                 C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "nt_propagate_noalloc": (C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(nt_hits), C.POINTER(C.c_int64),
                                  C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "nt_launch": (C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(nt_hits), C.c_void_p, C.c_void_p]),
     "nt_set_epilogue": (C.c_int32, [C.c_void_p, C.POINTER(nt_epilogue)]),

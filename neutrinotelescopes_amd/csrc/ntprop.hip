@@ -786,6 +786,7 @@ struct nt_ctx {
     int64_t total = 0;
 
     int64_t* d_counters = nullptr;
+    char* scratch = nullptr;            // this device's address of the file-scope scratch array
     // hit epilogue
     std::vector<nt_target> host_targets;
     bool epi_enabled = false;
@@ -828,6 +829,56 @@ int32_t grow(T** p, int64_t need, int64_t* cap_items) {
     int64_t n = std::max<int64_t>(need, 1);
     HIP_TRY(hipMalloc((void**)p, sizeof(T) * n));
     *cap_items = n;
+    return NT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Scratch memory of the allocation-free entry point (nt_propagate_noalloc):
+// one file-scope device array holding the counters and the hit columns of
+// up to NT_NOALLOC_MAX_HITS hits.  Each GPU holds its own copy until the
+// process ends; host code uses only the device address hipGetSymbolAddress
+// returns for it (kept per device), never the array's name.
+// ---------------------------------------------------------------------------
+constexpr int64_t kNoAllocHits = NT_NOALLOC_MAX_HITS;
+struct NoAllocLayout {                 // byte offsets (multiples of 256 for kNoAllocHits = 2^17)
+    static constexpr size_t counters = 0;
+    static constexpr size_t pos = 256;
+    static constexpr size_t dir = pos + 12 * kNoAllocHits;
+    static constexpr size_t idir = dir + 12 * kNoAllocHits;
+    static constexpr size_t wl = idir + 12 * kNoAllocHits;
+    static constexpr size_t time = wl + 4 * kNoAllocHits;
+    static constexpr size_t dist = time + 4 * kNoAllocHits;
+    static constexpr size_t mid = dist + 4 * kNoAllocHits;
+    static constexpr size_t sidx = mid + 2 * kNoAllocHits;
+    static constexpr size_t pid = sidx + 4 * kNoAllocHits;
+    static constexpr size_t pmt = pid + 8 * kNoAllocHits;
+    static constexpr size_t weight = pmt + kNoAllocHits;
+    static constexpr size_t tres = weight + 4 * kNoAllocHits;
+    static constexpr size_t bytes = tres + 4 * kNoAllocHits;
+};
+static_assert(NoAllocLayout::pid % 8 == 0 && NoAllocLayout::weight % 4 == 0 && NoAllocLayout::bytes % 256 == 0,
+              "scratch column alignment");
+
+__device__ __align__(256) unsigned char scratch[NoAllocLayout::bytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_addr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (which must be current),
+// looked up once per device.
+int32_t scratch_address(int device, char** out) {
+    if (device < 0 || device >= kMaxDevices) {
+        snprintf(g_err, sizeof g_err, "device %d: at most %d devices", device, kMaxDevices);
+        return NT_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_addr[device]) {
+        void* p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_addr[device] = static_cast<char*>(p);
+    }
+    *out = g_scratch_addr[device];
     return NT_OK;
 }
 
@@ -1063,6 +1114,44 @@ __global__ __launch_bounds__(1024) void scan_offsets_kernel(int64_t* offs, int n
     if (t == 0) total[0] = part[1023];
 }
 
+// Propagate into the device columns `h` with counters `d_cnt` on the
+// context's stream, then copy the stored hits and counts to the caller's host
+// columns (the synchronous API's second half).
+int32_t launch_and_copy_back(nt_ctx* c, uint64_t seed, const HitCols& h, int64_t* d_cnt, const nt_hits* out,
+                             int64_t* n_hits, int64_t* n_sim, int64_t* n_dropped) {
+    const int64_t cap = out->capacity;
+    int32_t st = launch(c, seed, h, d_cnt, c->stream);
+    if (st) return st;
+    int64_t cnt[4] = {0, 0, 0, 0};
+    HIP_TRY(hipMemcpyAsync(cnt, d_cnt, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const int64_t stored = std::min(cnt[0], cap);
+    if (stored > 0) {
+        HIP_TRY(hipMemcpyAsync(out->position, h.pos, 12 * stored, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(out->direction, h.dir, 12 * stored, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(out->initial_direction, h.idir, 12 * stored, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(out->wavelength, h.wl, 4 * stored, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(out->time, h.time, 4 * stored, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(out->dist_travelled, h.dist, 4 * stored, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(out->module_id, h.module_id, 2 * stored, hipMemcpyDeviceToHost, c->stream));
+        if (out->source_index)
+            HIP_TRY(hipMemcpyAsync(out->source_index, h.source_index, 4 * stored, hipMemcpyDeviceToHost, c->stream));
+        if (out->photon_id)
+            HIP_TRY(hipMemcpyAsync(out->photon_id, h.photon_id, 8 * stored, hipMemcpyDeviceToHost, c->stream));
+        if (out->pmt_id && c->epi.enabled)
+            HIP_TRY(hipMemcpyAsync(out->pmt_id, h.pmt_id, stored, hipMemcpyDeviceToHost, c->stream));
+        if (out->total_weight && c->epi.enabled)
+            HIP_TRY(hipMemcpyAsync(out->total_weight, h.total_weight, 4 * stored, hipMemcpyDeviceToHost, c->stream));
+        if (out->tres && c->epi.enabled)
+            HIP_TRY(hipMemcpyAsync(out->tres, h.tres, 4 * stored, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    *n_hits = stored;
+    *n_sim = cnt[1];
+    if (n_dropped) *n_dropped = cnt[0] - stored;
+    return cnt[0] > cap ? NT_ERR_HIT_OVERFLOW : NT_OK;
+}
+
 int32_t make_dev_source(const nt_source& a, int i, DevSource& d) {
     if (a.photons < 0 || a.photons > (int64_t(1) << 56) || a.kind < NT_SRC_POINTLIKE_ISOTROPIC ||
         a.kind > NT_SRC_POINTLIKE_CHERENKOV) {
@@ -1143,6 +1232,11 @@ int32_t nt_create(int32_t device, nt_ctx** out) {
             c->blocks_per_variant[dm][v] = c->num_cus * per_cu;
         }
     c->blocks = c->blocks_per_variant[0][kKernelVariants - 1];
+    if (scratch_address(device, &c->scratch) != NT_OK) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return NT_ERR_DEVICE;
+    }
     if (hipMalloc((void**)&c->d_counters, 4 * sizeof(int64_t)) != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
         delete c;
@@ -1614,36 +1708,32 @@ int32_t nt_propagate(nt_ctx* c, uint64_t seed, const nt_hits* out, int64_t* n_hi
     }
     HitCols h = c->d_hits;
     h.capacity = cap;
-    int32_t st = launch(c, seed, h, c->d_counters, c->stream);
-    if (st) return st;
-    int64_t cnt[4] = {0, 0, 0, 0};
-    HIP_TRY(hipMemcpyAsync(cnt, c->d_counters, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    const int64_t stored = std::min(cnt[0], cap);
-    if (stored > 0) {
-        HIP_TRY(hipMemcpyAsync(out->position, h.pos, 12 * stored, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipMemcpyAsync(out->direction, h.dir, 12 * stored, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipMemcpyAsync(out->initial_direction, h.idir, 12 * stored, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipMemcpyAsync(out->wavelength, h.wl, 4 * stored, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipMemcpyAsync(out->time, h.time, 4 * stored, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipMemcpyAsync(out->dist_travelled, h.dist, 4 * stored, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipMemcpyAsync(out->module_id, h.module_id, 2 * stored, hipMemcpyDeviceToHost, c->stream));
-        if (out->source_index)
-            HIP_TRY(hipMemcpyAsync(out->source_index, h.source_index, 4 * stored, hipMemcpyDeviceToHost, c->stream));
-        if (out->photon_id)
-            HIP_TRY(hipMemcpyAsync(out->photon_id, h.photon_id, 8 * stored, hipMemcpyDeviceToHost, c->stream));
-        if (out->pmt_id && c->epi.enabled)
-            HIP_TRY(hipMemcpyAsync(out->pmt_id, h.pmt_id, stored, hipMemcpyDeviceToHost, c->stream));
-        if (out->total_weight && c->epi.enabled)
-            HIP_TRY(hipMemcpyAsync(out->total_weight, h.total_weight, 4 * stored, hipMemcpyDeviceToHost, c->stream));
-        if (out->tres && c->epi.enabled)
-            HIP_TRY(hipMemcpyAsync(out->tres, h.tres, 4 * stored, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
+    return launch_and_copy_back(c, seed, h, c->d_counters, out, n_hits, n_sim, n_dropped);
+}
+
+int32_t nt_propagate_noalloc(nt_ctx* c, uint64_t seed, const nt_hits* out, int64_t* n_hits, int64_t* n_sim,
+                             int64_t* n_dropped) {
+    if (!c || !out || out->capacity < 0 || !n_hits || !n_sim) return NT_ERR_ARG;
+    if (out->capacity > 0 && (!out->position || !out->direction || !out->initial_direction ||
+                              !out->wavelength || !out->time || !out->dist_travelled || !out->module_id))
+        return NT_ERR_ARG;
+    if (out->capacity > kNoAllocHits) {
+        snprintf(g_err, sizeof g_err, "nt_propagate_noalloc: capacity %lld above NT_NOALLOC_MAX_HITS (%lld)",
+                 (long long)out->capacity, (long long)kNoAllocHits);
+        return NT_ERR_ARG;
     }
-    *n_hits = stored;
-    *n_sim = cnt[1];
-    if (n_dropped) *n_dropped = cnt[0] - stored;
-    return cnt[0] > cap ? NT_ERR_HIT_OVERFLOW : NT_OK;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    using L = NoAllocLayout;
+    char* b = c->scratch;
+    HitCols h{};
+    h.pos = (float*)(b + L::pos); h.dir = (float*)(b + L::dir); h.idir = (float*)(b + L::idir);
+    h.wl = (float*)(b + L::wl); h.time = (float*)(b + L::time); h.dist = (float*)(b + L::dist);
+    h.module_id = (uint16_t*)(b + L::mid); h.source_index = (uint32_t*)(b + L::sidx);
+    h.photon_id = (uint64_t*)(b + L::pid); h.pmt_id = (uint8_t*)(b + L::pmt);
+    h.total_weight = (float*)(b + L::weight); h.tres = (float*)(b + L::tres);
+    h.capacity = out->capacity;
+    return launch_and_copy_back(c, seed, h, (int64_t*)(b + L::counters), out, n_hits, n_sim, n_dropped);
 }
 
 int32_t nt_eval(nt_ctx* c, int32_t fn, const float* in, int32_t iw, float* out, int32_t ow, int64_t n) {

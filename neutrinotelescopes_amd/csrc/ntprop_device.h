@@ -336,25 +336,45 @@ __device__ __forceinline__ void sincos_rev64(float u, double& s, double& c) {
     c = ((k + 1) & 2) ? -ca : ca;
 }
 
-// rot_from_ez_fast, utils.jl:228-243 (the |a_z| == 1 branch as written), in FP64
+// rot_from_ez_fast, utils.jl:228-243, in FP64, general branch.  The
+// reference's x = (1 − a₁²k)o₁ − a₁a₂k·o₂ + a₁o₃ (and y, z alike) with
+// k = (1 − a₃)/(a₁² + a₂²) is evaluated in the algebraically identical form
+// x = o₁ + a₁(o₃ − k·p), y = o₂ + a₂(o₃ − k·p), z = a₃o₃ − p, p = a₁o₁ + a₂o₂
+// (no unit-norm assumption), which takes eleven operations instead of
+// seventeen; in FP64 either form is exact to far below an FP32 ulp.
+__device__ __forceinline__ D3 rot_from_ez64_general(D3 a, D3 o) {
+    const double S = __builtin_fma(a.x, a.x, a.y * a.y);
+    const double m = 1.0 - a.z;
+    const double r0 = __builtin_amdgcn_rcp(S);
+    const double k0 = m * r0;
+    const double k = __builtin_fma(r0, __builtin_fma(-S, k0, m), k0);     // m/S, one Newton step
+    const double p = __builtin_fma(a.x, o.x, a.y * o.y);
+    const double q = __builtin_fma(-k, p, o.z);
+    return D3{__builtin_fma(a.x, q, o.x), __builtin_fma(a.y, q, o.y), __builtin_fma(a.z, o.z, -p)};
+}
+
+// the |a_z| == 1 branch as written (utils.jl:230-232)
 __device__ __forceinline__ D3 rot_from_ez64(D3 a, D3 o) {
     if (fabs(a.z) == 1.0) return D3{o.x, copysign(o.y, a.z), copysign(o.z, a.z)};
-    const double a1sq = a.x * a.x, a2sq = a.y * a.y;
-    const double fact = (1.0 - a.z) * rcp64(a2sq + a1sq);
-    const double a1a2f = a.x * a.y * fact;
-    const double x = __builtin_fma(__builtin_fma(-a1sq, fact, 1.0), o.x, __builtin_fma(-a1a2f, o.y, a.x * o.z));
-    const double y = __builtin_fma(__builtin_fma(-a2sq, fact, 1.0), o.y, __builtin_fma(-a1a2f, o.x, a.y * o.z));
-    const double z = __builtin_fma(a.z, o.z, __builtin_fma(-a.x, o.x, -(a.y * o.y)));
-    return D3{x, y, z};
+    return rot_from_ez64_general(a, o);
 }
 
 __device__ __forceinline__ D3 to_d3(float3 v) { return D3{(double)v.x, (double)v.y, (double)v.z}; }
 __device__ __forceinline__ float3 to_f3(D3 v) { return make_float3((float)v.x, (float)v.y, (float)v.z); }
 
-// rot_from_ez_fast, utils.jl:228-243, for FP32 operands (FP64 inside)
-__device__ __forceinline__ float3 rot_from_ez(float3 a, float3 o) {
-    return to_f3(rot_from_ez64(to_d3(a), to_d3(o)));
+// rot_from_ez_fast for an FP32 axis: the general branch in FP64, rounded
+// once, and the |a_z| == 1 branch as FP32 selects (no divergent branch)
+__device__ __forceinline__ float3 rot_from_ez_f(float3 a, D3 o) {
+    float3 v = to_f3(rot_from_ez64_general(to_d3(a), o));
+    const bool vertical = fabsf(a.z) == 1.0f;
+    v.x = vertical ? (float)o.x : v.x;
+    v.y = vertical ? copysignf((float)o.y, a.z) : v.y;
+    v.z = vertical ? copysignf((float)o.z, a.z) : v.z;
+    return v;
 }
+
+// rot_from_ez_fast, utils.jl:228-243, for FP32 operands (FP64 inside)
+__device__ __forceinline__ float3 rot_from_ez(float3 a, float3 o) { return rot_from_ez_f(a, to_d3(o)); }
 
 // rot_from_ez_fast in FP32 with the reference's operation order
 // (NT_DIR_FP32_REF)
@@ -381,11 +401,30 @@ __host__ __device__ __forceinline__ HgConst hg_const(float g) {
     return HgConst{1.0 - gd, 2.0 * gd, __builtin_fma(-gd, gd, 1.0), __builtin_fma(gd, gd, 1.0) / (2.0 * gd),
                    0.5 / gd};
 }
+__device__ __forceinline__ double hg_cos64(const HgConst& h, float eta) {
+    // t = (1 − g²)/((1 − g) + 2g·η), the denominator ≥ 1 − g; quotient with
+    // one Newton correction of the hardware reciprocal (2^-24 → 2^-48)
+    const double den = __builtin_fma(h.twog, (double)eta, h.omg);
+    const double r0 = __builtin_amdgcn_rcp(den);
+    const double t0 = h.omg2 * r0;
+    const double t = __builtin_fma(r0, __builtin_fma(-den, t0, h.omg2), t0);
+    // the reference clamps to [−1, 1]; the FP64 value exceeds it by at most
+    // an FP64 rounding, which the sine below and the FP32 rounding absorb
+    return __builtin_fma(-t * t, h.inv2g, h.k0);
+}
+
+// √(1 − c²) for |c| ≤ 1 + 2^-52: rsq and one Newton step; the argument is
+// kept ≥ 1e-300 so that c = ±1 gives 1e-150 (0 after FP32 rounding)
+__device__ __forceinline__ double sin_from_cos64(double c) {
+    const double x = fmax(__builtin_fma(-c, c, 1.0), kc(1e-300));
+    const double y = __builtin_amdgcn_rsq(x);
+    const double s0 = x * y;
+    return __builtin_fma(__builtin_fma(-s0, s0, x), 0.5 * y, s0);
+}
+
 __device__ __forceinline__ void hg_cos_sin(const HgConst& h, float eta, double& c, double& s) {
-    // 1 + g(2η − 1) = (1 − g) + 2g·η ≥ 1 − g
-    const double t = h.omg2 * rcp64(__builtin_fma(h.twog, (double)eta, h.omg));
-    c = fmin(fmax(__builtin_fma(-t * t, h.inv2g, h.k0), -1.0), 1.0);
-    s = sqrt64(__builtin_fma(-c, c, 1.0));
+    c = hg_cos64(h, eta);
+    s = sin_from_cos64(c);
 }
 
 // the same cosine rounded to FP32
@@ -416,15 +455,13 @@ __device__ __forceinline__ float sl_cos(float g, float xi) {
 // it is used rather than hold its ten scalar registers across the loop.
 __device__ __forceinline__ float3 update_direction(float3 dir, float eta, float u_phi, float u_mix,
                                                    const Medium& m, const HgConst& hg) {
-    double c, s;
-    hg_cos_sin(hg, eta, c, s);
-    if (m.sl_fraction > 0.0f && u_mix < m.sl_fraction) {    // the first test is wave-uniform
+    double c = hg_cos64(hg, eta);
+    if (m.sl_fraction > 0.0f && u_mix < m.sl_fraction)      // the first test is wave-uniform
         c = (double)sl_cos(m.g, eta);
-        s = sqrt64(__builtin_fma(-c, c, 1.0));
-    }
+    const double s = sin_from_cos64(c);
     float sp, cp;
     sincos_rev(u_phi, &sp, &cp);       // φ = 2π·u_phi
-    return to_f3(rot_from_ez64(to_d3(dir), D3{(double)cp * s, (double)sp * s, c}));
+    return rot_from_ez_f(dir, D3{(double)cp * s, (double)sp * s, c});
 }
 
 // update_direction in FP32 with the reference's operation order (NT_DIR_FP32_REF)
