@@ -38,8 +38,7 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-FP32_PEAK_TFLOPS = 157.3        # MI355X published FP32 vector peak (packed FP32, two operations per lane)
-FP32_UNPACKED_TFLOPS = 78.6     # the same vector units issuing one FP32 operation per lane
+FP32_PEAK_TFLOPS = 157.3        # MI355X published FP32 vector peak
 PMT_AREA = (75e-3 / 2) ** 2 * math.pi
 
 # Algorithmic work (counting rule: add/mul/cmp/select 1, fma 2, div 1,
@@ -52,7 +51,8 @@ PMT_AREA = (75e-3 / 2) ** 2 * math.pi
 FLOP_STEP = 61 + 4
 FLOP_TEST = 18
 FLOP_STRING_BOUND = 27   # z-range + segment/axis distance test of one string
-FLOP_INIT = {"isotropic": 49 + 9, "cascade": 49 + 9 + 95 + 20, "cluster": 49 + 9 + 95 + 20}
+FLOP_INIT = {"isotropic": 49 + 9, "isotropic_r021": 49 + 9, "cascade": 49 + 9 + 95 + 20,
+             "cluster": 49 + 9 + 95 + 20}
 WORKLOAD_NAMES = {"cascade": "cherenkov_cascade_1e4GeV_1e7ph_20x16pmt_string",
                   "cluster": "cherenkov_cascade_1e4GeV_in_7x20x16pmt_cluster",
                   "isotropic": "isotropic_point_source_1_sphere_10m"}
@@ -94,7 +94,7 @@ def build_workload(name: str, photons: int, direction_arithmetic: str = "fp64"):
         src = nt.ExtendedCherenkovEmitter(np.array([10.0, 20.0, 5.0], np.float32),
                                           np.array([0.6, 0.0, 0.8], np.float32), 0.0, photons, lp)
         tgts = cluster_targets()
-    elif name == "isotropic":
+    elif name in ("isotropic", "isotropic_r021"):
         spec = nt.CherenkovSpectrum((300.0, 600.0), 30, med)
         src = nt.PointlikeIsotropicEmitter(np.zeros(3, np.float32), 0.0, photons)
         tgts = [nt.DetectionSphere(np.array([0, 0, 10.0], np.float32), 0.21, 16, PMT_AREA, 1)]
@@ -103,16 +103,57 @@ def build_workload(name: str, photons: int, direction_arithmetic: str = "fp64"):
     return nt.PhotonPropSetup([src], tgts, med, spec)
 
 
+def host_cpu_info() -> dict:
+    """The host CPUs this process may use: nproc, its affinity mask, the
+    cgroup quota and OMP_NUM_THREADS (the box grants one GPU a share of the
+    host), and the CPU model from lscpu."""
+    info = {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        quota, period = Path("/sys/fs/cgroup/cpu.max").read_text().split()[:2]
+        if quota != "max":
+            info["cgroup_cpus"] = int(quota) / int(period)
+    except (OSError, ValueError):
+        pass
+    model = None
+    try:
+        import subprocess
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, ValueError):
+        pass
+    info["cpu_model"] = model or _cpu_model()
+    share = info["affinity_cpus"]
+    if info["omp_num_threads"]:
+        share = min(share, int(info["omp_num_threads"]))
+    if "cgroup_cpus" in info:
+        share = min(share, max(1, int(info["cgroup_cpus"])))
+    info["cpu_share"] = share
+    return info
+
+
 def cpu_baseline(workload: str, target_s: float = 12.0):
-    """Oracle photons/s on the host cores for a bounded sample."""
+    """Oracle photons/s on the host cores for a bounded sample: the C
+    restatement of the reference kernel built here with -march=native (the
+    shipped x86-64-v3 build if the compiler is missing), OpenMP over all CPUs
+    of this process's share, and on one core."""
+    import tempfile
     sys.path.insert(0, str(ROOT / "tests"))
     import _oracle
-    threads = min(len(os.sched_getaffinity(0)), 16)
+    info = host_cpu_info()
+    try:
+        lib = _oracle.load_native(tempfile.mkdtemp(prefix="ntoracle_"))
+        build = "gcc -O3 -march=native -fopenmp -ffp-contract=off"
+    except Exception:                       # no compiler: the shipped build
+        lib = _oracle.load()
+        build = "gcc -O3 -march=x86-64-v3 -fopenmp -ffp-contract=off (shipped)"
+    threads = info["cpu_share"]
     n = 20000
     for _ in range(6):
         s = build_workload(workload, n)
         t0 = time.perf_counter()
-        _oracle.propagate(s, seed=1, threads=threads, capacity=1 << 20)
+        _oracle.propagate(s, seed=1, threads=threads, capacity=1 << 20, lib=lib)
         dt = time.perf_counter() - t0
         if dt > 1.0:
             break
@@ -120,31 +161,119 @@ def cpu_baseline(workload: str, target_s: float = 12.0):
     n_final = int(n * max(1.0, target_s / max(dt, 1e-6)))
     s = build_workload(workload, n_final)
     t0 = time.perf_counter()
-    _oracle.propagate(s, seed=2, threads=threads, capacity=1 << 22)
+    _oracle.propagate(s, seed=2, threads=threads, capacity=1 << 22, lib=lib)
     dt = time.perf_counter() - t0
     rate = n_final / dt
     # the same workload on one core, a ~4 s sample
     n1 = max(10000, int(rate / threads * 4.0))
     s1 = build_workload(workload, n1)
     t0 = time.perf_counter()
-    _oracle.propagate(s1, seed=3, threads=1, capacity=1 << 22)
+    _oracle.propagate(s1, seed=3, threads=1, capacity=1 << 22, lib=lib)
     dt1 = time.perf_counter() - t0
     return {"value": rate, "unit": "photons/s", "cores": threads, "kind": "port",
-            "sample": f"{n_final} photons of the {workload} workload, {dt:.1f} s on {threads} threads "
-                      f"(CPU oracle, g++ -O3, OpenMP)",
+            "sample": f"{n_final} photons of the {workload} workload (FP64 direction arithmetic), {dt:.1f} s on "
+                      f"{threads} threads (CPU oracle, {build}, OpenMP)",
             "one_core_value": n1 / dt1, "one_core_sample": f"{n1} photons, {dt1:.1f} s on 1 thread",
-            "cpu_model": _cpu_model()}
+            "host": info}
 
 
-def _cpu_model() -> str:
+def measure_kernel(workload: str, photons: int, arithmetic: str = "fp64", steps: int = 20, warmup: int = 3,
+                   culling: bool = True, epilogue: bool = False) -> dict:
+    """One workload on nt_launch: `warmup` untimed launches, then `steps`
+    timed launches (HIP events on the launch stream bracket each one; the
+    wall clock brackets all of them), inputs and hit columns resident in HBM.
+    FLOP per launch from the kernel's own step and test counters."""
+    import torch
+    from neutrinotelescopes_amd import _abi
+    from neutrinotelescopes_amd.propagation import PropagationEngine
+    dev = torch.device("cuda:0")
+    setup = build_workload(workload, photons, arithmetic)
+    eng = PropagationEngine(0)
     try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
-    except OSError:
-        pass
-    return "unknown"
+        eng.setup(setup)
+        if epilogue:
+            eng.set_epilogue(setup.medium, source_position=setup.sources[0].position)
+        _abi.check(eng.lib.nt_set_option(eng.handle, _abi.NT_OPT_TARGET_CULLING, int(culling)), "nt_set_option")
+        cap = 1 << 22
+        cols = dict(position=torch.empty(cap, 3, device=dev), direction=torch.empty(cap, 3, device=dev),
+                    initial_direction=torch.empty(cap, 3, device=dev), wavelength=torch.empty(cap, device=dev),
+                    time=torch.empty(cap, device=dev), dist_travelled=torch.empty(cap, device=dev),
+                    module_id=torch.empty(cap, dtype=torch.int16, device=dev),
+                    source_index=torch.empty(cap, dtype=torch.int32, device=dev),
+                    photon_id=torch.empty(cap, dtype=torch.int64, device=dev))
+        if epilogue:
+            cols.update(pmt_id=torch.empty(cap, dtype=torch.uint8, device=dev),
+                        total_weight=torch.empty(cap, device=dev), tres=torch.empty(cap, device=dev))
+        counters = torch.zeros(4, dtype=torch.int64, device=dev)
+        h = _abi.nt_hits()
+        for k, v in cols.items():
+            setattr(h, k, v.data_ptr())
+        h.capacity = cap
+        stream = torch.cuda.current_stream()
+
+        def step(seed):
+            _abi.check(eng.lib.nt_launch(eng.handle, C.c_uint64(seed), C.byref(h),
+                                         C.c_void_p(counters.data_ptr()), C.c_void_p(stream.cuda_stream)),
+                       "nt_launch")
+
+        for i in range(warmup):
+            step(1000 + i)
+        torch.cuda.synchronize()
+        c = counters.cpu().numpy()
+        hits_per_step, tests_per_step, bound_tests_per_step = int(c[0]), int(c[2]), int(c[3])
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            ev[i][0].record(stream)
+            step(i)
+            ev[i][1].record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kernel_ms = [a.elapsed_time(b) for a, b in ev]
+        c = counters.cpu().numpy()
+        assert int(c[1]) == photons, c
+    finally:
+        eng.close()
+    n_targets = len(setup.targets)
+    n_strings = len({(float(t.position[0]), float(t.position[1])) for t in setup.targets})
+    # every step tests every string bound (culling on) or every sphere (off)
+    steps_total = (bound_tests_per_step / n_strings if bound_tests_per_step
+                   else tests_per_step / max(n_targets, 1))
+    flop = (photons * FLOP_INIT[workload] + steps_total * FLOP_STEP
+            + tests_per_step * FLOP_TEST + bound_tests_per_step * FLOP_STRING_BOUND)
+    k_ms = float(np.mean(kernel_ms))
+    achieved = flop / (k_ms * 1e-3) / 1e12
+    return {"value": photons * steps / wall, "ms_per_step": wall / steps * 1e3, "kernel_ms": k_ms,
+            "config": {"photons_per_step": photons, "targets": n_targets, "hits_per_step": hits_per_step,
+                       "direction_arithmetic": arithmetic, "string_culling": culling, "hit_epilogue": epilogue,
+                       "scattering_steps_per_photon": steps_total / photons,
+                       "sphere_tests_per_scattering_step": tests_per_step / max(steps_total, 1),
+                       "string_bound_tests_per_scattering_step": bound_tests_per_step / max(steps_total, 1)},
+            "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": achieved / FP32_PEAK_TFLOPS, "kernel": "propagate_kernel", "kernel_ms": k_ms,
+                         "flop_per_launch": flop}}
+
+
+def other_legs(args) -> dict:
+    """Beside the headline: the same cascade in the reference's FP32 direction
+    arithmetic, the cascade with the device hit epilogue (PMT id, weights,
+    time residual: the acceptance the C2 configuration names) run after each
+    launch, and the plain configuration's throughput size (isotropic source,
+    one r = 0.21 m sphere at 10 m, 1e9 photons per launch)."""
+    legs = {}
+    r = measure_kernel("cascade", args.photons, "fp32", args.steps, args.warmup)
+    legs["cascade_fp32_reference_arithmetic"] = {"value": r["value"], "kernel_ms": r["kernel_ms"],
+                                                 "frac": r["roofline"]["frac"]}
+    r = measure_kernel("cascade", args.photons, args.direction_arithmetic, args.steps, args.warmup, epilogue=True)
+    legs["cascade_with_hit_epilogue"] = {"value": r["value"], "ms_per_step": r["ms_per_step"],
+                                         "hits_per_step": r["config"]["hits_per_step"]}
+    for arith in (args.direction_arithmetic, "fp32"):
+        r = measure_kernel("isotropic_r021", 1_000_000_000, arith, 3, 1)
+        legs[f"isotropic_1e9ph_r021_{arith}"] = {"value": r["value"], "kernel_ms": r["kernel_ms"],
+                                                "frac": r["roofline"]["frac"],
+                                                "hits_per_step": r["config"]["hits_per_step"]}
+    return legs
 
 
 def serving_benchmark(duration_s: float = 15.0, seed: int = 11, round_photons: int = 10_000_000,
@@ -268,6 +397,8 @@ def main():
     ap.add_argument("--workload", default="cascade", choices=["cascade", "cluster", "isotropic"])
     ap.add_argument("--photons", type=int, default=10_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="skip the FP32-arithmetic, hit-epilogue and 1e9-photon isotropic legs")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving (p50/p99) measurement")
     ap.add_argument("--serving-only", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true",
@@ -306,65 +437,9 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": a.numel() * 4, "dispatches": 5}))
         return
-    from neutrinotelescopes_amd import _abi
-    from neutrinotelescopes_amd.propagation import PropagationEngine
-
-    dev = torch.device("cuda:0")
-    setup = build_workload(args.workload, args.photons, args.direction_arithmetic)
-    eng = PropagationEngine(0)
-    eng.setup(setup)
-    _abi.check(eng.lib.nt_set_option(eng.handle, _abi.NT_OPT_TARGET_CULLING, 0 if args.no_culling else 1),
-               "nt_set_option")
-    cap = 1 << 22
-    cols = dict(position=torch.empty(cap, 3, device=dev), direction=torch.empty(cap, 3, device=dev),
-                initial_direction=torch.empty(cap, 3, device=dev), wavelength=torch.empty(cap, device=dev),
-                time=torch.empty(cap, device=dev), dist_travelled=torch.empty(cap, device=dev),
-                module_id=torch.empty(cap, dtype=torch.int16, device=dev),
-                source_index=torch.empty(cap, dtype=torch.int32, device=dev),
-                photon_id=torch.empty(cap, dtype=torch.int64, device=dev))
-    counters = torch.zeros(4, dtype=torch.int64, device=dev)
-    h = _abi.nt_hits()
-    for k, v in cols.items():
-        setattr(h, k, v.data_ptr())
-    h.capacity = cap
-    stream = torch.cuda.current_stream()
-
-    def step(seed):
-        _abi.check(eng.lib.nt_launch(eng.handle, C.c_uint64(seed), C.byref(h), C.c_void_p(counters.data_ptr()),
-                                     C.c_void_p(stream.cuda_stream)), "nt_launch")
-
-    for i in range(args.warmup):
-        step(1000 + i)
-    torch.cuda.synchronize()
-    c = counters.cpu().numpy()
-    hits_per_step = int(c[0])
-    tests_per_step = int(c[2])
-    bound_tests_per_step = int(c[3])
-
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        ev[i][0].record(stream)
-        step(i)
-        ev[i][1].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kernel_ms = [a.elapsed_time(b) for a, b in ev]
-    c = counters.cpu().numpy()
-    assert int(c[1]) == args.photons, c
-
-    n_targets = len(setup.targets)
-    n_strings = len({(float(t.position[0]), float(t.position[1])) for t in setup.targets})
-    # every step tests every string bound (culling on) or every sphere (off)
-    steps_total = (bound_tests_per_step / n_strings if bound_tests_per_step
-                   else tests_per_step / max(n_targets, 1))
-    flop = (args.photons * FLOP_INIT[args.workload] + steps_total * FLOP_STEP
-            + tests_per_step * FLOP_TEST + bound_tests_per_step * FLOP_STRING_BOUND)
-    k_ms = float(np.mean(kernel_ms))
-    achieved = flop / (k_ms * 1e-3) / 1e12
-    value = args.photons * args.steps / wall
-
+    leg = measure_kernel(args.workload, args.photons, args.direction_arithmetic, args.steps, args.warmup,
+                         culling=not args.no_culling)
+    value = leg["value"]
     out = {
         "metric": "photons propagated/s (1 GPU)",
         "value": value,
@@ -372,26 +447,17 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": leg["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32+f64" if args.direction_arithmetic == "fp64" else "f32",
         "data": "synthetic",
-        "config": {"workload": WORKLOAD_NAMES[args.workload], "photons_per_step": args.photons,
-                   "targets": n_targets, "hits_per_step": hits_per_step,
-                   "direction_arithmetic": args.direction_arithmetic,
-                   "string_culling": not args.no_culling,
-                   "scattering_steps_per_photon": steps_total / args.photons,
-                   "sphere_tests_per_scattering_step": tests_per_step / max(steps_total, 1),
-                   "string_bound_tests_per_scattering_step": bound_tests_per_step / max(steps_total, 1)},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": profiled_traffic(args),
-                     "peak_unpacked": FP32_UNPACKED_TFLOPS, "frac_unpacked": achieved / FP32_UNPACKED_TFLOPS,
-                     "kernel": "propagate_kernel", "kernel_ms": k_ms,
-                     "flop_per_launch": flop},
+        "config": dict(leg["config"], workload=WORKLOAD_NAMES[args.workload]),
+        "roofline": dict(leg["roofline"], traffic=profiled_traffic(args)),
     }
-    eng.close()
+    if args.workload == "cascade" and not args.no_legs:
+        out["legs"] = other_legs(args)
     if not args.no_serving and args.workload == "cascade":
         out["serving"] = serving_benchmark()
     if not args.no_surrogate and args.workload == "cascade":

@@ -27,7 +27,20 @@ def load():
     if not ORACLE_SO.exists() or os.access(ROOT / "oracle", os.W_OK):
         # rebuild when stale (header or source newer); a no-op otherwise
         subprocess.run(["make", "-s", "-C", str(ROOT / "oracle")], check=True, capture_output=True)
-    lib = C.CDLL(str(ORACLE_SO))
+    _lib = _bind(C.CDLL(str(ORACLE_SO)))
+    return _lib
+
+
+def load_native(out_dir):
+    """The oracle built for this host (-march=native) into `out_dir`: the CPU
+    baseline of the benchmark.  Returns the bound library."""
+    so = Path(out_dir) / "liboracle_native.so"
+    subprocess.run(["make", "-s", "-C", str(ROOT / "oracle"), "native", f"NATIVE_OUT={so}"], check=True,
+                   capture_output=True)
+    return _bind(C.CDLL(str(so)))
+
+
+def _bind(lib):
     f, d, i, u32, u64, p = C.c_float, C.c_double, C.c_int, C.c_uint32, C.c_uint64, C.c_void_p
     fp = C.POINTER(C.c_float)
     dp = C.POINTER(C.c_double)
@@ -83,7 +96,6 @@ def load():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    _lib = lib
     return lib
 
 
@@ -132,9 +144,9 @@ def propagate_range(setup, first, count, seed=0, capacity=1 << 20, threads=None)
     return out
 
 
-def propagate(setup, seed=0, capacity=1 << 20, threads=None) -> PhotonHits:
+def propagate(setup, seed=0, capacity=1 << 20, threads=None, lib=None) -> PhotonHits:
     """Oracle propagation of a PhotonPropSetup; hits ordered by (source, photon)."""
-    lib = load()
+    lib = lib or load()
     srcs = _abi.as_c_array(_abi.nt_source, [s.to_abi() for s in setup.sources])
     tgts = _abi.as_c_array(_abi.nt_target, [t.to_abi() for t in setup.targets])
     med = setup.medium.to_abi()
