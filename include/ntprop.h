@@ -13,7 +13,10 @@
  *    NT_ERR_HIT_OVERFLOW with the first `capacity` hits kept and the number of
  *    lost hits reported.
  *  - A context (nt_ctx) owns device-resident state: medium, spectrum table,
- *    targets, sources, counters.  set_* calls are single-writer.
+ *    targets, sources, counters.  set_* calls are single-writer; each waits
+ *    for the kernels of the context's last launch (nt_launch on any stream,
+ *    nt_propagate) to finish before it rewrites the tables they read.  A
+ *    server (nt_server_*) needs them left unchanged while it runs.
  *  - Random numbers: Philox4x32-10 (Salmon et al., SC'11).  key = (seed_lo32,
  *    seed_hi32); counter = (block, photon_lo32, photon_hi32, source_index),
  *    where photon is the photon's index inside its source.  Block 0 holds the

@@ -787,6 +787,7 @@ struct nt_ctx {
 
     int64_t* d_counters = nullptr;
     char* scratch = nullptr;            // this device's address of the file-scope scratch array
+    hipEvent_t last_launch = nullptr;   // recorded after the kernels of every launch (any stream)
     // hit epilogue
     std::vector<nt_target> host_targets;
     bool epi_enabled = false;
@@ -945,7 +946,10 @@ int32_t launch(nt_ctx* c, uint64_t seed, const HitCols& cols, int64_t* d_counter
         return NT_ERR_STATE;
     }
     hipLaunchKernelGGL(clear_counters, dim3(1), dim3(64), 0, s, d_counters);
-    if (c->total == 0) return NT_OK;
+    if (c->total == 0) {
+        HIP_TRY(hipEventRecord(c->last_launch, s));
+        return NT_OK;
+    }
     LaunchPlan lp{c->d_sources, c->d_offsets, c->n_sources, c->total, c->variant, nullptr, cols, d_counters};
     const int32_t st = launch_plan(c, seed, lp, s);
     if (st) return st;
@@ -955,6 +959,16 @@ int32_t launch(nt_ctx* c, uint64_t seed, const HitCols& cols, int64_t* d_counter
                            c->d_target_of_module);
         HIP_TRY(hipGetLastError());
     }
+    // the set_* calls wait for this before they rewrite what the kernels read
+    HIP_TRY(hipEventRecord(c->last_launch, s));
+    return NT_OK;
+}
+
+// Wait until the kernels of the last launch (on whatever stream the caller
+// gave nt_launch) have finished, so that a set_* call can rewrite the
+// device tables they read.
+int32_t wait_last_launch(nt_ctx* c) {
+    HIP_TRY(hipEventSynchronize(c->last_launch));
     return NT_OK;
 }
 
@@ -1222,6 +1236,12 @@ int32_t nt_create(int32_t device, nt_ctx** out) {
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete c; return NT_ERR_DEVICE; }
     c->num_cus = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return NT_ERR_DEVICE; }
+    if (hipEventCreateWithFlags(&c->last_launch, hipEventDisableTiming) != hipSuccess ||
+        hipEventRecord(c->last_launch, c->stream) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return NT_ERR_DEVICE;
+    }
     for (int dm = 0; dm < 2; ++dm)
         for (int v = 0; v < kKernelVariants; ++v) {
             int per_cu = 0;
@@ -1256,6 +1276,7 @@ void nt_destroy(nt_ctx* c) {
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     free_hits(c->d_hits);
+    if (c->last_launch) (void)hipEventDestroy(c->last_launch);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1355,6 +1376,7 @@ int32_t nt_set_medium(nt_ctx* c, const nt_medium* m) {
         return NT_ERR_ARG;
     }
     std::lock_guard<std::mutex> lk(c->mu);
+    if (int32_t w = wait_last_launch(c)) return w;
     Medium& d = c->medium;
     d.a01 = m->quan_fry[0]; d.a2 = m->quan_fry[1]; d.a3 = m->quan_fry[2]; d.a4 = m->quan_fry[3];
     d.vcs = m->vol_conc_small_part;
@@ -1378,6 +1400,7 @@ int32_t nt_set_medium(nt_ctx* c, const nt_medium* m) {
 int32_t nt_set_spectrum(nt_ctx* c, const nt_spectrum* s) {
     if (!c || !s) return NT_ERR_ARG;
     std::lock_guard<std::mutex> lk(c->mu);
+    if (int32_t w = wait_last_launch(c)) return w;
     HIP_TRY(hipSetDevice(c->device));
     Spectrum d{};
     d.kind = s->kind;
@@ -1451,6 +1474,7 @@ int32_t nt_set_targets(nt_ctx* c, const nt_target* t, int32_t n) {
         ids[i] = t[i].module_id;
     }
     std::lock_guard<std::mutex> lk(c->mu);
+    if (int32_t w = wait_last_launch(c)) return w;
     HIP_TRY(hipSetDevice(c->device));
     // buffers change from here on: a failure below leaves the context without
     // targets rather than with freed or half-written tables
@@ -1546,6 +1570,7 @@ int32_t nt_set_sources_device(nt_ctx* c, const nt_source* d_src, int32_t n, int3
     if (!c || n < 0 || (n > 0 && !d_src) || kind_hint < -1 || kind_hint > NT_SRC_POINTLIKE_CHERENKOV)
         return NT_ERR_ARG;
     std::lock_guard<std::mutex> lk(c->mu);
+    if (int32_t w = wait_last_launch(c)) return w;
     HIP_TRY(hipSetDevice(c->device));
     c->has_sources = false;                  // records change from here on (see nt_set_targets)
     int64_t cap = c->sources_cap, cap2 = c->sources_cap ? c->sources_cap + 1 : 0;
@@ -1598,6 +1623,7 @@ int32_t nt_set_sources(nt_ctx* c, const nt_source* s, int32_t n) {
         offs[i + 1] = offs[i] + round_up64(s[i].photons);
     }
     std::lock_guard<std::mutex> lk(c->mu);
+    if (int32_t w = wait_last_launch(c)) return w;
     HIP_TRY(hipSetDevice(c->device));
     c->has_sources = false;                  // records change from here on (see nt_set_targets)
     int64_t cap = c->sources_cap, cap2 = c->sources_cap ? c->sources_cap + 1 : 0;
@@ -1620,6 +1646,7 @@ int32_t nt_set_sources(nt_ctx* c, const nt_source* s, int32_t n) {
 int32_t nt_set_epilogue(nt_ctx* c, const nt_epilogue* e) {
     if (!c || !e) return NT_ERR_ARG;
     std::lock_guard<std::mutex> lk(c->mu);
+    if (int32_t w = wait_last_launch(c)) return w;
     HIP_TRY(hipSetDevice(c->device));
     if (!e->enabled) {
         c->epi_enabled = false;

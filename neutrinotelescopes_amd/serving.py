@@ -141,12 +141,15 @@ def cascade_requests(n: int, medium, rng: np.random.Generator, radius: float = 1
     return out
 
 
-def run_poisson_stream(server: PhotonServer, requests, rate_hz: float, rng: np.random.Generator):
+def run_poisson_stream(server: PhotonServer, requests, rate_hz: float, rng: np.random.Generator,
+                       keep_hits: bool = False):
     """Submit `requests` at Poisson arrival times (rate `rate_hz`) from one
     thread while another collects results; returns per-request latencies
-    (ms), photons, wall time and hit counts."""
+    (ms), photons, wall time and hit counts (and the hits themselves with
+    `keep_hits`)."""
     latencies = [None] * len(requests)
     hits = [0] * len(requests)
+    results = [None] * len(requests)
     ids = [None] * len(requests)
     submitted = threading.Semaphore(0)
     errors = []
@@ -158,6 +161,8 @@ def run_poisson_stream(server: PhotonServer, requests, rate_hz: float, rng: np.r
                 h, lat = server.wait(ids[i], raise_on_overflow=False)
                 latencies[i] = lat
                 hits[i] = len(h)
+                if keep_hits:
+                    results[i] = h
         except Exception as exc:   # pragma: no cover - reported to the caller
             errors.append(exc)
 
@@ -178,6 +183,9 @@ def run_poisson_stream(server: PhotonServer, requests, rate_hz: float, rng: np.r
         raise errors[0]
     lat = np.array(latencies)
     photons = sum(s.photons for r in requests for s in r.sources)
-    return {"latency_ms": lat, "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-            "wall_s": wall, "photons": photons, "photons_per_s": photons / wall, "hits": int(sum(hits)),
-            "requests": len(requests)}
+    out = {"latency_ms": lat, "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+           "wall_s": wall, "photons": photons, "photons_per_s": photons / wall, "hits": int(sum(hits)),
+           "requests": len(requests)}
+    if keep_hits:
+        out["results"] = results
+    return out

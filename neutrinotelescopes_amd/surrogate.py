@@ -411,6 +411,17 @@ class Surrogate:
         return FlowRows.from_numpy(self.norm(features), t, count, flags, aux)
 
 
+def scale_only_surrogate(sur: Surrogate) -> Surrogate:
+    """The method without the learned part: a network whose weights are all
+    zero, so that every output is 0 and log λ of a PMT is the count scale's
+    offset ln(yield(E)·g(d)) alone (the time density is the untransformed
+    base normal)."""
+    from .flow import FlowModel
+    m = FlowModel(sur.model.device, n_in=N_FEATURES)
+    m.set_weights(np.zeros_like(m.get_weights()))
+    return Surrogate(m, sur.norm, sur.scale)
+
+
 def train_surrogate(tables: TableSet, steps: int = 3000, batch: int = 16384, lr: float = 1e-3,
                     seed: int = 0, model=None, log_every: int = 0, holdout: float = 0.1,
                     count_weight: float = 1.0, average_every: int = 64, count_power: float = 1.0):
@@ -674,7 +685,6 @@ def fallback_terms(engine, medium, spectrum, casc: Cascades, modules: np.ndarray
     if evs.size == 0:
         return out
     mods = np.flatnonzero(near[evs].any(axis=0))
-    mod_slot = {int(m): j for j, m in enumerate(mods)}
     srcs, scales = [], []
     for i in evs:
         e = float(casc.energy[i])
@@ -687,11 +697,11 @@ def fallback_terms(engine, medium, spectrum, casc: Cascades, modules: np.ndarray
     engine.set_targets(make_targets(modules[mods]))
     engine.set_sources(srcs)
     engine.set_epilogue(medium, None, casc.pos[evs[0]])
-    # host hit columns sized for the usual case (tens of thousands of hits);
-    # an overflow repeats the launch with room for 4M
+    # host hit columns sized for the usual case; an overflow repeats the
+    # launch with room for all of its hits
     h = engine.propagate(seed=seed, capacity=1 << 18, raise_on_overflow=False)
     if h.n_dropped:
-        h = engine.propagate(seed=seed, capacity=1 << 22)
+        h = engine.propagate(seed=seed, capacity=len(h) + h.n_dropped)
     si = h.source_index.astype(np.int64)
     ev = evs[si]
     m = mods[h.module_id.astype(np.int64) - 1]
@@ -714,17 +724,41 @@ def fallback_terms(engine, medium, spectrum, casc: Cascades, modules: np.ndarray
         pois = n_obs * np.log(np.where(lam > 0, lam, 1.0)) - lam - np.vectorize(math.lgamma)(n_obs + 1.0)
     pois = np.where((lam == 0) & (n_obs > 0), -1e30, pois)
     np.add.at(out, all_keys // (n_mod * 16), pois)
-    # time densities: per-PMT weighted histograms, looked up at the observed times
+    # time densities: per-PMT weighted histograms, evaluated at the observed
+    # times only (sparse: one entry per occupied (PMT, bin))
     if len(near_hits):
         nb = bins.size - 1
         hb = np.clip(np.searchsorted(bins, tres, side="right") - 1, 0, nb - 1)
-        hist = np.zeros((all_keys.size, nb))
-        np.add.at(hist, (pos, hb), w)
-        dens = hist / np.maximum(hist.sum(axis=1, keepdims=True), 1e-300) / np.diff(bins) + 1e-6
+        cell = pos * nb + hb
+        cells, inv = np.unique(cell, return_inverse=True)
+        cell_w = np.bincount(inv, weights=w)
         ob = np.clip(np.searchsorted(bins, near_hits[:, 2], side="right") - 1, 0, nb - 1)
         op = np.searchsorted(all_keys, near_hits[:, 0].astype(np.int64))
-        np.add.at(out, near_hits[:, 1].astype(np.int64), np.log(dens[op, ob]))
+        oc = op * nb + ob
+        if cells.size:
+            j = np.clip(np.searchsorted(cells, oc), 0, cells.size - 1)
+            w_cell = np.where(cells[j] == oc, cell_w[j], 0.0)
+        else:
+            w_cell = np.zeros(oc.size)
+        w_pmt = lam[op]
+        dens = w_cell / np.maximum(w_pmt, 1e-300) / np.diff(bins)[ob] + 1e-6
+        np.add.at(out, near_hits[:, 1].astype(np.int64), np.log(dens))
     return out
+
+
+def direct_likelihood(engine, medium, spectrum, casc: Cascades, modules: np.ndarray, obs: Observation,
+                      photons: int, seed: int = 0) -> np.ndarray:
+    """The event log-likelihood from direct propagation alone: every
+    (cascade, module) pair takes the terms fallback_terms computes (Poisson
+    count per PMT from the summed weights, histogram time density per PMT),
+    all cascades in one launch of ≤ `photons` photons each."""
+    n_mod = len(modules)
+    allp = np.ones((len(casc.energy), n_mod), bool)
+    hp = ((obs.hits[:, 0].astype(np.int64) * n_mod + obs.hits[:, 1].astype(np.int64)) * 16
+          + obs.hits[:, 2].astype(np.int64))
+    hits = np.column_stack([hp, obs.hits[:, 0].astype(np.int64), obs.hits[:, 3]])
+    return fallback_terms(engine, medium, spectrum, casc, modules, obs.counts, allp, hits, photons=photons,
+                          seed=seed)
 
 
 @dataclass
@@ -854,8 +888,16 @@ def surrogate_study(n_runs: int = 128, photons_per_run: int = 2 * 10 ** 8, train
     nh_far = np.where(dist >= FALLBACK_DISTANCE, nh, 0)
     lam_sur = surrogate_expectations(sur, ev, modules)
     err = module_errors(lam_sur, lam_dir, nh_far)
+    err50 = module_errors(lam_sur, lam_dir, nh_far, min_hits=50)
+    base = scale_only_surrogate(sur)
+    lam_base = surrogate_expectations(base, ev, modules)
+    base.model.close()
+    err_base = module_errors(lam_base, lam_dir, nh_far)
+    err50_base = module_errors(lam_base, lam_dir, nh_far, min_hits=50)
     sel_pairs = nh_far >= 400
     truth_stat = float(np.median(1.0 / np.sqrt(np.maximum(nh_truth[sel_pairs], 1)))) if sel_pairs.any() else None
+    sel50 = nh_far >= 50
+    truth_stat50 = float(np.median(1.0 / np.sqrt(np.maximum(nh_truth[sel50], 1)))) if sel50.any() else None
     untrained = Surrogate(FlowModel(device, seed=seed + 99, n_in=N_FEATURES), sur.norm, sur.scale)
     err0 = module_errors(surrogate_expectations(untrained, ev, modules), lam_dir, nh_far)
     sel = np.argwhere(nh_far >= 400)
@@ -890,9 +932,21 @@ def surrogate_study(n_runs: int = 128, photons_per_run: int = 2 * 10 ** 8, train
     linfo = {"seconds": t_fb + t_flow, "flow_seconds": t_flow, "fallback_seconds": t_fb,
              "rows": int(dev_ev.counts.shape[0]), "hits": int(dev_ev.hit_t.shape[0]),
              "fallback_pairs": int(dev_ev.near.sum())}
+    # direct likelihood at matching statistics: error_photons per cascade is
+    # what gives the compared pairs >= 400 hits (<= 5 % statistical error, the
+    # surrogate's count error); measured on a subset of the same events
+    n_direct = min(32, n_events)
+    sub = Cascades(casc.pos[:n_direct], casc.dir[:n_direct], casc.energy[:n_direct])
+    hsub = obs.hits[obs.hits[:, 0] < n_direct]
+    obs_sub = Observation(obs.counts[:n_direct], hsub)
+    direct_likelihood(eng, med, spec, Cascades(sub.pos[:2], sub.dir[:2], sub.energy[:2]), modules,
+                      Observation(obs.counts[:2], hsub[hsub[:, 0] < 2]), min(error_photons, 10 ** 7),
+                      seed=seed + 7)                                  # warm-up
+    t0 = time.perf_counter()
+    llh_direct = direct_likelihood(eng, med, spec, sub, modules, obs_sub, error_photons, seed=seed + 8)
+    direct_llh_s = time.perf_counter() - t0
+    direct_events_per_s = n_direct / direct_llh_s
     rate = direct_rate(eng, med, spec, modules)
-    n_real = np.array([cascade_photons(float(e), med) for e in casc.energy])
-    direct_s = float(n_real.sum() / rate)
     out = {
         "config": "tables_log10E_2-6_d_1-300m; RQ flow 8 bins x 3 layers, MLP 5-128-128-78, bf16 MFMA; "
                   "likelihood: 10 strings x 20 modules x 16 PMTs, cascades log-uniform 1e3-1e6 GeV",
@@ -904,16 +958,23 @@ def surrogate_study(n_runs: int = 128, photons_per_run: int = 2 * 10 ** 8, train
         "holdout_loss_last_step": hist.get("last_holdout"),
         "error_pairs": err["pairs"], "count_err_median": err["median_abs_rel_err"],
         "count_err_mean": err["mean_abs_rel_err"], "count_bias": err["bias"],
+        "error_pairs_50hits": err50["pairs"], "count_err_median_50hits": err50["median_abs_rel_err"],
+        "count_err_mean_50hits": err50["mean_abs_rel_err"], "count_bias_50hits": err50["bias"],
         "count_err_median_untrained": err0["median_abs_rel_err"],
-        "truth_rel_stat_err_median": truth_stat,
+        "count_err_median_scale_only": err_base["median_abs_rel_err"],
+        "count_err_median_50hits_scale_only": err50_base["median_abs_rel_err"],
+        "truth_rel_stat_err_median": truth_stat, "truth_rel_stat_err_median_50hits": truth_stat50,
         "time_kl_median": float(np.median(kl)) if kl.size else None,
         "time_kl_median_untrained": float(np.median(kl0)) if kl0.size else None,
         "events": n_events, "llh_seconds": linfo["seconds"], "llh_flow_seconds": linfo["flow_seconds"],
         "llh_fallback_seconds": linfo["fallback_seconds"],
         "llh_pairs": linfo["rows"], "llh_hits": linfo["hits"], "fallback_pairs": linfo["fallback_pairs"],
         "surrogate_events_per_s": n_events / linfo["seconds"],
-        "direct_photons_per_s": rate, "direct_photons": float(n_real.sum()), "direct_seconds": direct_s,
-        "direct_events_per_s": n_events / direct_s, "speedup": direct_s / linfo["seconds"],
+        "direct_llh_events": n_direct, "direct_llh_photons_per_event": int(error_photons),
+        "direct_llh_seconds": direct_llh_s, "direct_events_per_s": direct_events_per_s,
+        "direct_llh_finite": bool(np.all(np.isfinite(llh_direct))),
+        "speedup": (n_events / linfo["seconds"]) / direct_events_per_s,
+        "direct_photons_per_s": rate,
         "llh_finite": bool(np.all(np.isfinite(llh))),
     }
     eng.close()

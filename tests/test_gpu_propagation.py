@@ -390,6 +390,38 @@ def test_device_launch_with_torch_buffers(engine):
     assert np.array_equal(np.sort(pid), np.sort(host.photon_id))
 
 
+def test_setters_wait_for_async_launch(engine):
+    """set_* right after an asynchronous nt_launch waits for its kernels: the
+    launch's hits are those of the targets it was launched with."""
+    torch = pytest.importorskip("torch")
+    s = iso_sphere_setup(30_000_000, r=1.0)
+    ref = run_gpu(engine, s, seed=12).sorted()
+    engine.setup(s)
+    cap = 1 << 18
+    dev = torch.device("cuda:0")
+    cols = dict(position=torch.zeros(cap, 3, device=dev), direction=torch.zeros(cap, 3, device=dev),
+                initial_direction=torch.zeros(cap, 3, device=dev), wavelength=torch.zeros(cap, device=dev),
+                time=torch.zeros(cap, device=dev), dist_travelled=torch.zeros(cap, device=dev),
+                module_id=torch.zeros(cap, dtype=torch.int16, device=dev),
+                source_index=torch.zeros(cap, dtype=torch.int32, device=dev),
+                photon_id=torch.zeros(cap, dtype=torch.int64, device=dev))
+    counters = torch.zeros(4, dtype=torch.int64, device=dev)
+    h = _abi.nt_hits()
+    for k, v in cols.items():
+        setattr(h, k, v.data_ptr())
+    h.capacity = cap
+    stream = torch.cuda.Stream()
+    _abi.check(engine.lib.nt_launch(engine.handle, 12, h, counters.data_ptr(), stream.cuda_stream), "nt_launch")
+    # immediately: other targets (the launch must still see the sphere at 10 m)
+    engine.set_targets([nt.DetectionSphere(np.array([0, 0, -3.0], np.float32), 2.0, 16, PMT_AREA, 9)])
+    stream.synchronize()
+    n = int(counters[0].item())
+    assert n == len(ref)
+    pid = np.sort(cols["photon_id"][:n].cpu().numpy().astype(np.uint64))
+    assert np.array_equal(pid, ref.photon_id)
+    assert set(cols["module_id"][:n].cpu().numpy().tolist()) == {1}
+
+
 def cluster_layout(n_strings=7, per_string=20, spacing=50.0, radius=0.21):
     """A centre string plus a hexagon at 50 m, 20 modules each (the serving cluster)."""
     coords = nt.make_pom_pmt_coordinates(np.float32)

@@ -105,3 +105,38 @@ def test_poisson_stream_latency(serving_engine):
         res = run_poisson_stream(srv, reqs, rate_hz=400.0, rng=rng)
     assert res["requests"] == 40 and np.all(res["latency_ms"] > 0)
     assert res["p99_ms"] >= res["p50_ms"] > 0
+
+
+def test_full_size_serving_configuration():
+    """The serving configuration at full size: r = 0.21 m modules on the
+    7-string x 20-module cluster, cascades log-uniform over 1e2-1e6 GeV in the
+    300 m cylinder (one of 1e6 GeV, 1e8 photons), submitted as a Poisson
+    stream: every result equals the same sources propagated alone, and
+    p50 <= p99."""
+    import os
+    if not os.path.exists("/dev/kfd"):
+        pytest.skip("no GPU node on this machine")
+    eng = nt.PropagationEngine(0)
+    try:
+        med = nt.make_cascadia_medium_properties(0.95)
+        eng.set_medium(med)
+        eng.set_spectrum(nt.CherenkovSpectrum((300.0, 800.0), 30, med))
+        eng.set_targets(cluster_targets())
+        rng = np.random.default_rng(8)
+        reqs = cascade_requests(30, med, rng)
+        big = cascade_requests(1, med, rng, e_range=(1e6, 1e6))[0]
+        reqs.insert(11, big)
+        assert max(s.photons for r in reqs for s in r.sources) >= 5 * 10 ** 7
+        with PhotonServer(eng, device_budget=8 << 30, host_budget=4 << 30, round_photons=10_000_000) as srv:
+            res = run_poisson_stream(srv, reqs, rate_hz=300.0, rng=rng, keep_hits=True)
+        assert res["requests"] == len(reqs) and np.all(res["latency_ms"] > 0)
+        assert res["p50_ms"] <= res["p99_ms"]
+        total = 0
+        for r, h in zip(reqs, res["results"]):
+            ref = alone(eng, r.sources, r.seed, r.capacity)
+            assert h.n_simulated == sum(s.photons for s in r.sources) == ref.n_simulated
+            assert_same(h.sorted(), ref)
+            total += len(ref)
+        assert total > 1000
+    finally:
+        eng.close()

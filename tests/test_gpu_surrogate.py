@@ -55,12 +55,20 @@ def test_trained_network_beats_untrained_against_direct(trained):
     kl = S.time_kl(sur, casc, modules, lam_sur, hits, [tuple(p) for p in sel], bins)
     kl0 = S.time_kl(untrained, casc, modules, lam_sur, hits, [tuple(p) for p in sel], bins)
     untrained.model.close()
+    # the method without the network: log λ = the count scale's offset alone
+    base = S.scale_only_surrogate(sur)
+    err_b = S.module_errors(S.surrogate_expectations(base, casc, modules), lam_dir, nh)
+    kl_b = S.time_kl(base, casc, modules, lam_sur, hits, [tuple(p) for p in sel], bins)
+    base.model.close()
     print("pairs", err["pairs"], "count error median", err["median_abs_rel_err"], "untrained",
-          err0["median_abs_rel_err"], "KL", np.median(kl), "untrained", np.median(kl0))
+          err0["median_abs_rel_err"], "count scale alone", err_b["median_abs_rel_err"],
+          "KL", np.median(kl), "untrained", np.median(kl0), "count scale alone", np.median(kl_b))
     assert err["pairs"] >= 10
     assert err["median_abs_rel_err"] < 0.3
     assert err["median_abs_rel_err"] < err0["median_abs_rel_err"] / 2
+    assert err["median_abs_rel_err"] < err_b["median_abs_rel_err"] / 2
     assert np.median(kl) < np.median(kl0) / 4
+    assert np.median(kl) < np.median(kl_b) / 4
 
 
 def test_device_likelihood_matches_host_terms(trained):
@@ -110,3 +118,56 @@ def test_fallback_near_modules(trained):
     assert fb[0] != 0 and np.all(fb[1:] == 0) and np.isfinite(fb[0])
     llh = S.surrogate_llh(sur, ev, fb).cpu().numpy()
     assert np.all(np.isfinite(llh))
+
+
+def test_full_size_likelihood_1024_cascades(trained):
+    """The event-likelihood configuration at full size: 1024 cascades
+    (log-uniform 1e3-1e6 GeV) on the 10-string x 20-module x 16-PMT array,
+    3 276 800 (cascade, PMT) pairs in one pass, cascades placed within 5 m of
+    modules so that fallback pairs occur: every event's likelihood is finite,
+    the fallback terms are nonzero exactly for the events with near pairs,
+    and a sample of events equals the host evaluation of the same terms."""
+    from neutrinotelescopes_amd import surrogate as S
+    eng, med, spec, sur, _, _ = trained
+    rng = np.random.default_rng(51)
+    modules = S.array_detector()
+    n_ev = 1024
+    casc = S.random_cascades(n_ev, rng)
+    for j, m in enumerate((3, 57, 110, 199)):                     # four events with a near module
+        casc.pos[j] = modules[m] + np.array([2.0 + 0.5 * j, 1.0, 0.0])
+    counts = rng.poisson(1.0, (n_ev, len(modules), 16))
+    # near pairs observe nothing: their direct-propagation terms stay moderate
+    # (an observed hit on a PMT the direct run never reached scores -1e30)
+    near_h = np.linalg.norm(modules[None] - casc.pos[:, None], axis=-1) < S.FALLBACK_DISTANCE
+    counts[near_h] = 0
+    k = np.repeat(np.arange(counts.size), counts.reshape(-1))
+    hits = np.column_stack([k // (len(modules) * 16), (k // 16) % len(modules), k % 16,
+                            rng.uniform(-10, 300, k.size)])
+    obs = S.Observation(counts, hits)
+    ev = S.to_device(casc, modules, obs, str(sur.model.dev))
+    assert ev.counts.shape[0] == n_ev * len(modules) * 16 == 3_276_800
+    assert ev.near[:4].any(axis=1).all()
+    fb = S.fallback_terms(eng, med, spec, casc, modules, obs.counts, ev.near, ev.near_hits, photons=10 ** 7,
+                          seed=52)
+    has_near = ev.near.any(axis=1)
+    assert np.array_equal(has_near, near_h.any(axis=1))
+    assert np.all(fb[has_near] < 0) and np.all(fb[~has_near] == 0) and np.all(fb > -1e29)
+    llh = S.surrogate_llh(sur, ev, fb).cpu().numpy()
+    assert llh.shape == (n_ev,) and np.all(np.isfinite(llh))
+    # host terms for a sample of events (pairs masked where near, as on the device)
+    pick = np.array([0, 5, 100, 517, 1023])
+    sub = S.Cascades(casc.pos[pick], casc.dir[pick], casc.energy[pick])
+    f = S.pair_features(sub, modules).reshape(-1, 5)
+    near = np.repeat(ev.near[pick].reshape(-1), 16)
+    _, lc = sur.model.forward(sur.rows(f, None, counts[pick].reshape(-1), np.full(len(f), 2, np.uint8)),
+                              want_params=False)
+    host = np.zeros(len(pick))
+    np.add.at(host, np.repeat(np.arange(len(pick)), len(modules) * 16), np.where(near, 0, -lc.double().cpu().numpy()))
+    for j, e in enumerate(pick):
+        he = hits[hits[:, 0] == e]
+        hp = (he[:, 1].astype(int) * 16 + he[:, 2].astype(int))
+        fe = f[j * len(modules) * 16:(j + 1) * len(modules) * 16]
+        _, lt = sur.model.forward(sur.rows(fe[hp], he[:, 3], None, np.full(len(hp), 1, np.uint8)),
+                                  want_params=False)
+        host[j] += np.sum(np.where(near[j * len(modules) * 16 + hp], 0, -lt.double().cpu().numpy()))
+    np.testing.assert_allclose(llh[pick] - fb[pick], host, rtol=2e-4, atol=1.0)
