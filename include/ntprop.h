@@ -248,6 +248,28 @@ int32_t nt_propagate_noalloc(nt_ctx* ctx, uint64_t seed, const nt_hits* out_host
 int32_t nt_launch(nt_ctx* ctx, uint64_t seed, const nt_hits* out_dev,
                   int64_t* d_counters, void* stream);
 
+/* resample_simulation (src/pmt_frontend/pipeline.jl:86-116) on the device:
+   the hits' weighted photons → one realisation of detected hits.  Per group
+   of hits — (module_id, pmt_id) with per_pmt != 0, module_id otherwise — draw
+   n = min(Poisson(Σ total_weight · downsample), #hits with weight > 0) and
+   select n of the group's hits without replacement with probabilities
+   ∝ total_weight; d_detected[i] = 1 for the selected hits, 0 otherwise.
+   dev_hits are DEVICE columns (module_id, total_weight, source_index,
+   photon_id, and pmt_id with per_pmt; e.g. the output of nt_launch with the
+   hit epilogue), n_hits their count.  Random numbers: Philox4x32-10 with
+   key = seed: each hit's 53-bit uniform u from counter (0x71000000, photon
+   lo, photon hi, source index) gives it the key ln(u)/w (the n largest keys
+   of a group are the sample; Efraimidis & Spirakis 2006), each group's
+   Poisson draw uses counter (trial, group, 0, 0x70000000) with group =
+   module_id·17 + min(pmt_id, 16) (per_pmt) or module_id.  The result depends
+   on the hits, not on their order.  The reference draws with
+   PoissonRandom.pois_rand and StatsBase.sample (unreproducible streams).
+   Asynchronous on `stream`; *d_n_detected (device int64, optional) receives
+   the number of detected hits.  Calls on one context run one at a time (they
+   share the context's workspace). */
+int32_t nt_resample(nt_ctx* ctx, const nt_hits* dev_hits, int64_t n_hits, uint64_t seed, double downsample,
+                    int32_t per_pmt, uint8_t* d_detected, int64_t* d_n_detected, void* stream);
+
 /* Options (no reference counterpart).
    NT_OPT_TARGET_CULLING (default 1): test a step against the bounding
    cylinder of each vertical string of targets (targets sharing x, y) before

@@ -202,6 +202,8 @@ SIGNATURES = {
     "nt_propagate_noalloc": (C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(nt_hits), C.POINTER(C.c_int64),
                                  C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "nt_launch": (C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(nt_hits), C.c_void_p, C.c_void_p]),
+    "nt_resample": (C.c_int32, [C.c_void_p, C.POINTER(nt_hits), C.c_int64, C.c_uint64, C.c_double, C.c_int32,
+                                C.c_void_p, C.c_void_p, C.c_void_p]),
     "nt_set_epilogue": (C.c_int32, [C.c_void_p, C.POINTER(nt_epilogue)]),
     "nt_set_option": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int64]),
     "nt_launch_shape": (C.c_int32, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),

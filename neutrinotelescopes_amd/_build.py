@@ -43,15 +43,34 @@ def sources() -> list[Path]:
 
 
 def build_library(verbose: bool = False) -> Path:
-    """Compile every HIP source into the package's shared library (always)."""
+    """Compile every HIP source into the package's shared library (always):
+    one object per source, compiled in parallel, then one link."""
+    import tempfile
+    from concurrent.futures import ThreadPoolExecutor
     out = PKG_DIR / LIB_NAME
     tmp = out.with_suffix(".so.tmp")
-    cmd = [hipcc_path(), *HIP_FLAGS, f"-I{INCLUDE}", "-o", str(tmp), *map(str, sources())]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed ({res.returncode}):\n{res.stderr[-8000:]}")
+    hipcc = hipcc_path()
+    compile_flags = [f for f in HIP_FLAGS if f != "-shared"]
+    with tempfile.TemporaryDirectory(prefix="ntprop_build_") as d:
+        objs = [Path(d) / (src.stem + ".o") for src in sources()]
+
+        def compile_one(src_obj):
+            src, obj = src_obj
+            cmd = [hipcc, *compile_flags, f"-I{INCLUDE}", "-c", "-o", str(obj), str(src)]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            return subprocess.run(cmd, capture_output=True, text=True)
+
+        workers = max(1, min(len(objs), os.cpu_count() or 1, 8))
+        with ThreadPoolExecutor(workers) as pool:
+            results = list(pool.map(compile_one, zip(sources(), objs)))
+        for res in results:
+            if res.returncode != 0:
+                raise RuntimeError(f"hipcc failed ({res.returncode}):\n{res.stderr[-8000:]}")
+        cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"link failed ({res.returncode}):\n{res.stderr[-8000:]}")
     os.replace(tmp, out)
     return out
 

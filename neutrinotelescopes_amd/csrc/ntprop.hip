@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "ntprop_device.h"
+#include "ntresample.h"
 
 using namespace ntp;
 
@@ -788,6 +789,9 @@ struct nt_ctx {
     int64_t* d_counters = nullptr;
     char* scratch = nullptr;            // this device's address of the file-scope scratch array
     hipEvent_t last_launch = nullptr;   // recorded after the kernels of every launch (any stream)
+    char* d_ws = nullptr;               // resampling workspace
+    int64_t ws_cap = 0;
+    hipEvent_t last_resample = nullptr; // recorded after the kernels of every nt_resample
     // hit epilogue
     std::vector<nt_target> host_targets;
     bool epi_enabled = false;
@@ -1237,7 +1241,9 @@ int32_t nt_create(int32_t device, nt_ctx** out) {
     c->num_cus = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return NT_ERR_DEVICE; }
     if (hipEventCreateWithFlags(&c->last_launch, hipEventDisableTiming) != hipSuccess ||
-        hipEventRecord(c->last_launch, c->stream) != hipSuccess) {
+        hipEventRecord(c->last_launch, c->stream) != hipSuccess ||
+        hipEventCreateWithFlags(&c->last_resample, hipEventDisableTiming) != hipSuccess ||
+        hipEventRecord(c->last_resample, c->stream) != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
         delete c;
         return NT_ERR_DEVICE;
@@ -1277,6 +1283,8 @@ void nt_destroy(nt_ctx* c) {
         if (q) (void)hipFree(q);
     free_hits(c->d_hits);
     if (c->last_launch) (void)hipEventDestroy(c->last_launch);
+    if (c->last_resample) (void)hipEventDestroy(c->last_resample);
+    if (c->d_ws) (void)hipFree(c->d_ws);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1761,6 +1769,33 @@ int32_t nt_propagate_noalloc(nt_ctx* c, uint64_t seed, const nt_hits* out, int64
     h.total_weight = (float*)(b + L::weight); h.tres = (float*)(b + L::tres);
     h.capacity = out->capacity;
     return launch_and_copy_back(c, seed, h, (int64_t*)(b + L::counters), out, n_hits, n_sim, n_dropped);
+}
+
+int32_t nt_resample(nt_ctx* c, const nt_hits* h, int64_t n, uint64_t seed, double downsample, int32_t per_pmt,
+                    uint8_t* d_detected, int64_t* d_n_detected, void* stream) {
+    if (!c || !h || n < 0 || n > INT32_MAX || !(downsample >= 0.0) || !std::isfinite(downsample)) {
+        snprintf(g_err, sizeof g_err, "nt_resample: 0 <= n <= 2^31 - 1 hits and a finite downsample >= 0");
+        return NT_ERR_ARG;
+    }
+    if (n > 0 && (!h->module_id || !h->total_weight || !h->source_index || !h->photon_id || !d_detected ||
+                  (per_pmt && !h->pmt_id))) {
+        snprintf(g_err, sizeof g_err, "nt_resample: module_id, total_weight, source_index, photon_id%s and "
+                 "the detected column are required", per_pmt ? ", pmt_id" : "");
+        return NT_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    // the workspace is the context's: the previous call's kernels finish first
+    HIP_TRY(hipEventSynchronize(c->last_resample));
+    const int64_t need = (int64_t)resample_workspace_bytes(n);
+    int32_t st = grow(&c->d_ws, need, &c->ws_cap);
+    if (st) return st;
+    ResampleArgs a{h->module_id, h->pmt_id, h->total_weight, h->source_index, h->photon_id, n, seed, downsample,
+                   per_pmt ? 1 : 0, d_detected, d_n_detected};
+    const hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(resample_run(a, c->d_ws, (size_t)c->ws_cap, s));
+    HIP_TRY(hipEventRecord(c->last_resample, s));
+    return NT_OK;
 }
 
 int32_t nt_eval(nt_ctx* c, int32_t fn, const float* in, int32_t iw, float* out, int32_t ow, int64_t n) {

@@ -42,43 +42,8 @@ __device__ __forceinline__ TypeParams type_params(int type) {
     }
 }
 
-// Uniform double in [0, 1) from two words (53 bits).
-__device__ __forceinline__ double u53(uint32_t hi, uint32_t lo) {
-    return ((double)(hi >> 5) * 67108864.0 + (double)(lo >> 6)) * 0x1p-53;
-}
-
-__device__ int64_t poisson(double mu, uint32_t k0, uint32_t k1, uint32_t ilo, uint32_t ihi) {
-    if (!(mu > 0.0)) return 0;
-    if (mu < 10.0) {
-        const uint4 r = ntp::philox(make_uint4(0u, ilo, ihi, kPoissonWord), k0, k1);
-        const double u = u53(r.x, r.y);
-        double p = exp(-mu), f = p;
-        int64_t k = 0;
-        while (u > f && k < 1000) {
-            ++k;
-            p *= mu / (double)k;
-            f += p;
-        }
-        return k;
-    }
-    const double smu = sqrt(mu);
-    const double b = 0.931 + 2.53 * smu;
-    const double a = -0.059 + 0.02483 * b;
-    const double inv_alpha = 1.1239 + 1.1328 / (b - 3.4);
-    const double vr = 0.9277 - 3.6224 / (b - 2.0);
-    const double log_mu = log(mu);
-    for (uint32_t trial = 0; trial < 1000u; ++trial) {
-        const uint4 r = ntp::philox(make_uint4(trial, ilo, ihi, kPoissonWord), k0, k1);
-        const double U = u53(r.x, r.y) - 0.5;
-        const double V = u53(r.z, r.w);
-        const double us = 0.5 - fabs(U);
-        const double kd = floor((2.0 * a / us + b) * U + mu + 0.43);
-        if (us >= 0.07 && V <= vr) return (int64_t)kd;
-        if (kd < 0.0 || (us < 0.013 && V > us)) continue;
-        if (log(V) + log(inv_alpha) - log(a / (us * us) + b) <= -mu + kd * log_mu - lgamma(kd + 1.0))
-            return (int64_t)kd;
-    }
-    return (int64_t)floor(mu + 0.5);          // not reached in practice (acceptance ≥ 0.75 per trial)
+__device__ __forceinline__ int64_t poisson(double mu, uint32_t k0, uint32_t k1, uint32_t ilo, uint32_t ihi) {
+    return ntp::poisson_philox(mu, k0, k1, ilo, ihi, kPoissonWord);
 }
 
 __global__ __launch_bounds__(256) void build_sources_kernel(const nt_particle* __restrict__ in, int n,

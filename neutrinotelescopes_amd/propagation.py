@@ -219,6 +219,34 @@ class PropagationEngine:
         out.n_dropped = nd.value
         return out
 
+    def resample(self, hits: PhotonHits, seed: int, downsample: float = 1.0, per_pmt: bool = True) -> np.ndarray:
+        """resample_simulation (pipeline.jl:86-116) of host hits on the device
+        (nt_resample): the hits are uploaded, and the detected flags (bool per
+        hit) come back."""
+        import torch
+        dev = torch.device(f"cuda:{self.device}")
+        n = len(hits)
+        t = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a).astype(dt, copy=False)).to(dev)
+        cols = {"module_id": t(hits.module_id, np.uint16).view(torch.int16),
+                "total_weight": t(hits.total_weight, np.float32),
+                "source_index": t(hits.source_index, np.uint32).view(torch.int32),
+                "photon_id": t(hits.photon_id, np.uint64).view(torch.int64)}
+        if per_pmt:
+            cols["pmt_id"] = t(hits.pmt_id, np.uint8)
+        h = _abi.nt_hits()
+        for k, v in cols.items():
+            setattr(h, k, v.data_ptr())
+        h.capacity = n
+        flags = torch.zeros(max(n, 1), dtype=torch.uint8, device=dev)
+        count = torch.zeros(1, dtype=torch.int64, device=dev)
+        stream = torch.cuda.current_stream(dev)
+        _abi.check(self.lib.nt_resample(self.handle, C.byref(h), n, C.c_uint64(seed & (2**64 - 1)), float(downsample),
+                                        int(per_pmt), C.c_void_p(flags.data_ptr()), C.c_void_p(count.data_ptr()),
+                                        C.c_void_p(stream.cuda_stream)), "nt_resample")
+        out = flags[:n].cpu().numpy().astype(bool)
+        assert int(count.item()) == int(out.sum())
+        return out
+
     def eval(self, fn: int, inputs: np.ndarray, out_width: int) -> np.ndarray:
         """Run one device function of the kernel on rows of `inputs`."""
         x = np.ascontiguousarray(inputs, dtype=np.float32)

@@ -55,14 +55,14 @@ def ptrs_trial(mu: float, U, V):
     return np.where(quick | (~skip & full), kd, -1.0)
 
 
-def poisson(mu: float, philox, seed: int, index: int) -> int:
-    """philox(ctr[4], key[2]) -> 4 words."""
+def poisson(mu: float, philox, seed: int, index: int, word: int = POISSON_WORD) -> int:
+    """philox(ctr[4], key[2]) -> 4 words; counter (trial, index lo, hi, word)."""
     key = (seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF)
     ilo, ihi = index & 0xFFFFFFFF, (index >> 32) & 0xFFFFFFFF
     if not mu > 0.0:
         return 0
     if mu < 10.0:
-        r = philox((0, ilo, ihi, POISSON_WORD), key)
+        r = philox((0, ilo, ihi, word), key)
         u = u53(r[0], r[1])
         p = math.exp(-mu)
         f = p
@@ -73,7 +73,7 @@ def poisson(mu: float, philox, seed: int, index: int) -> int:
             f += p
         return k
     for trial in range(1000):
-        r = philox((trial, ilo, ihi, POISSON_WORD), key)
+        r = philox((trial, ilo, ihi, word), key)
         k = int(ptrs_trial(mu, u53(r[0], r[1]) - 0.5, u53(r[2], r[3])))
         if k >= 0:
             return k
