@@ -178,7 +178,7 @@ def cpu_baseline(workload: str, target_s: float = 12.0):
 
 
 def measure_kernel(workload: str, photons: int, arithmetic: str = "fp64", steps: int = 20, warmup: int = 3,
-                   culling: bool = True, epilogue: bool = False) -> dict:
+                   culling: bool = True, epilogue: bool = False, config_mode: bool = False) -> dict:
     """One workload on nt_launch: `warmup` untimed launches, then `steps`
     timed launches (HIP events on the launch stream bracket each one; the
     wall clock brackets all of them), inputs and hit columns resident in HBM.
@@ -191,8 +191,14 @@ def measure_kernel(workload: str, photons: int, arithmetic: str = "fp64", steps:
     eng = PropagationEngine(0)
     try:
         eng.setup(setup)
+        if config_mode:
+            # the configuration's physics where it departs from the reference:
+            # emission at 41.2°, a tabulated angular acceptance
+            eng.set_cherenkov_angle(math.radians(41.2))
         if epilogue:
-            eng.set_epilogue(setup.medium, source_position=setup.sources[0].position)
+            import neutrinotelescopes_amd as nt
+            eng.set_epilogue(setup.medium, source_position=setup.sources[0].position,
+                             angular_acceptance=nt.lambertian_angular_acceptance() if config_mode else None)
         _abi.check(eng.lib.nt_set_option(eng.handle, _abi.NT_OPT_TARGET_CULLING, int(culling)), "nt_set_option")
         cap = 1 << 22
         cols = dict(position=torch.empty(cap, 3, device=dev), direction=torch.empty(cap, 3, device=dev),
@@ -247,6 +253,7 @@ def measure_kernel(workload: str, photons: int, arithmetic: str = "fp64", steps:
     return {"value": photons * steps / wall, "ms_per_step": wall / steps * 1e3, "kernel_ms": k_ms,
             "config": {"photons_per_step": photons, "targets": n_targets, "hits_per_step": hits_per_step,
                        "direction_arithmetic": arithmetic, "string_culling": culling, "hit_epilogue": epilogue,
+                       "config_mode": config_mode,
                        "scattering_steps_per_photon": steps_total / photons,
                        "sphere_tests_per_scattering_step": tests_per_step / max(steps_total, 1),
                        "string_bound_tests_per_scattering_step": bound_tests_per_step / max(steps_total, 1)},
@@ -268,6 +275,10 @@ def other_legs(args) -> dict:
     r = measure_kernel("cascade", args.photons, args.direction_arithmetic, args.steps, args.warmup, epilogue=True)
     legs["cascade_with_hit_epilogue"] = {"value": r["value"], "ms_per_step": r["ms_per_step"],
                                          "hits_per_step": r["config"]["hits_per_step"]}
+    r = measure_kernel("cascade", args.photons, args.direction_arithmetic, args.steps, args.warmup, epilogue=True,
+                       config_mode=True)
+    legs["cascade_config_mode_41deg_angular_acceptance"] = {"value": r["value"], "ms_per_step": r["ms_per_step"],
+                                                            "hits_per_step": r["config"]["hits_per_step"]}
     for arith in (args.direction_arithmetic, "fp32"):
         r = measure_kernel("isotropic_r021", 1_000_000_000, arith, 3, 1)
         legs[f"isotropic_1e9ph_r021_{arith}"] = {"value": r["value"], "kernel_ms": r["kernel_ms"],

@@ -275,7 +275,12 @@ int32_t nt_resample(nt_ctx* ctx, const nt_hits* dev_hits, int64_t n_hits, uint64
    cylinder of each vertical string of targets (targets sharing x, y) before
    its spheres.  The bounds are inflated beyond the FP32 rounding of the sphere
    test, so results are identical to testing every target in list order (0). */
-enum { NT_OPT_TARGET_CULLING = 0 };
+/* NT_OPT_CHERENKOV_ANGLE_URAD (default 0): 0 emits Cherenkov photons at
+   θc = acos(1/n(λ)) as the reference (cherenkov_angle, medium.jl:309-311);
+   a value v in 1..3141592 emits them at the fixed angle v·1e-6 rad (the
+   "emission at 41.2°" configuration: 719076), around the same sampled track
+   direction.  Kept across nt_set_medium. */
+enum { NT_OPT_TARGET_CULLING = 0, NT_OPT_CHERENKOV_ANGLE_URAD = 1 };
 int32_t nt_set_option(nt_ctx* ctx, int32_t option, int64_t value);
 
 /* ---- hit epilogue ------------------------------------------------------
@@ -288,7 +293,8 @@ int32_t nt_set_option(nt_ctx* ctx, int32_t option, int64_t value);
                   rotated by `orientation`) lies within asin(sqrt(area/π)/r)
                   of the hit direction seen from the module centre, in FP64
                   (detection.jl:88-104); DetectionSphere: 1
-     total_weight area_acc × wl_acc × exp(−dist/L_abs(λ)): area_acc =
+     total_weight area_acc × wl_acc × exp(−dist/L_abs(λ)) [× angular
+                  acceptance, when a table is given]: area_acc =
                   n_pmts·pmt_area/(4πr²) for DetectionSphere, 1 for
                   MultiPMTDetector; L_abs linear in the absorption table
                   (× abs_len_scale); wl_acc linear in the acceptance table
@@ -307,6 +313,13 @@ typedef struct nt_epilogue {
     const double* acc_value;
     double orientation[9];           /* row-major rotation of the PMT axes */
     float source_position[3];        /* time-residual origin */
+    /* Angular acceptance (no reference counterpart; the reference has only
+       the opening-angle cap): for a hit on a PMT of a MultiPMTDetector the
+       weight is multiplied by this table, linear in the cosine of the angle of
+       incidence −direction·(PMT axis), clamped at the ends.  n_ang = 0: none. */
+    int32_t n_ang;                   /* 0 or 2..1024 rows */
+    const double* ang_cos;           /* ascending */
+    const double* ang_value;
 } nt_epilogue;
 int32_t nt_set_epilogue(nt_ctx* ctx, const nt_epilogue* epilogue);
 

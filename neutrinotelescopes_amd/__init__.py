@@ -15,7 +15,7 @@ from .lightyield import (Particle, PEPlus, PEMinus, PGamma, LongitudinalParamete
                          PointlikeTimeRangeEmitter, AxiconeEmitter, PencilEmitter,
                          ExtendedCherenkovEmitter, PointlikeCherenkovEmitter, split_source,
                          oversample_source, total_lightyield, cherenkov_track_length)
-from .detection import (DetectionSphere, MultiPMTDetector, make_pom_pmt_coordinates,
+from .detection import (DetectionSphere, MultiPMTDetector, make_pom_pmt_coordinates, lambertian_angular_acceptance,
                         check_pmt_hit, area_acceptance, p_one_pmt_acc, get_pmt_count)
 from .propagation import (PhotonPropSetup, PhotonHits, PropagationEngine, propagate_photons, resample_simulation,
                           run_photon_prop_no_local_cache, calc_total_weight, calc_time_residual,

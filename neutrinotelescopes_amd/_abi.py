@@ -44,6 +44,7 @@ NT_TEX_TEXEL_CENTER_CLAMP_Q8 = 1
 NT_TEX_ENDPOINT = 2
 
 NT_OPT_TARGET_CULLING = 0
+NT_OPT_CHERENKOV_ANGLE_URAD = 1
 
 NT_TARGET_SPHERE = 0
 NT_TARGET_MULTI_PMT = 1
@@ -142,6 +143,9 @@ class nt_epilogue(C.Structure):
         ("acc_value", C.POINTER(C.c_double)),
         ("orientation", C.c_double * 9),
         ("source_position", C.c_float * 3),
+        ("n_ang", C.c_int32),
+        ("ang_cos", C.POINTER(C.c_double)),
+        ("ang_value", C.POINTER(C.c_double)),
     ]
 
 

@@ -101,7 +101,9 @@ struct Epilogue {
     const float* abs_y;
     const double* acc_x;               // PMT acceptance table (FP64)
     const double* acc_y;
-    int n_abs, n_acc;
+    const double* ang_x;               // angular acceptance: cos of incidence → factor (FP64), n_ang = 0: none
+    const double* ang_y;
+    int n_abs, n_acc, n_ang;
     float abs_len_scale;
     int enabled;
 };
@@ -267,6 +269,7 @@ __device__ __forceinline__ Medium load_medium(const_kparams_ptr k) {
     m.sl_fraction = k->medium.sl_fraction; m.sca_len_scale = k->medium.sca_len_scale;
     m.dispersion_mode = k->medium.dispersion_mode; m.max_steps = k->medium.max_steps;
     m.dir_mode = k->medium.dir_mode;
+    m.cherenkov_angle = k->medium.cherenkov_angle;
     m.hg.omg = k->medium.hg.omg; m.hg.twog = k->medium.hg.twog; m.hg.omg2 = k->medium.hg.omg2;
     m.hg.k0 = k->medium.hg.k0; m.hg.inv2g = k->medium.hg.inv2g;
     return m;
@@ -309,8 +312,8 @@ __device__ __forceinline__ T interp_table(T x, const T* xs, const T* ys, int n) 
 }
 
 // PMT assignment, weight and time residual of one hit (see nt_epilogue).
-__device__ __forceinline__ void hit_epilogue(const Epilogue& ep, int ti, float3 pos, float wl, float dist,
-                                          float time, uint8_t& pmt, float& weight, float& tres) {
+__device__ __forceinline__ void hit_epilogue(const Epilogue& ep, int ti, float3 pos, float3 dir, float wl,
+                                          float dist, float time, uint8_t& pmt, float& weight, float& tres) {
     const EpiTarget* T = ep.targets + ti;
     // check_pmt_hit, detection.jl:88-104 (FP64 as in the reference)
     if (T->kind == NT_TARGET_SPHERE) {
@@ -331,7 +334,14 @@ __device__ __forceinline__ void hit_epilogue(const Epilogue& ep, int ti, float3 
     if (ep.abs_len_scale != 1.0f) abs_len *= ep.abs_len_scale;
     const double abs_w = (double)expf(-dist / abs_len);
     const double wl_acc = interp_table((double)wl, ep.acc_x, ep.acc_y, ep.n_acc);
-    weight = (float)(T->area_acc * wl_acc * abs_w);
+    double ang_acc = 1.0;
+    if (ep.n_ang > 0 && T->kind == NT_TARGET_MULTI_PMT && pmt > 0) {
+        // cos of the angle of incidence on the PMT: −direction · PMT axis
+        const double* ax = T->axes[pmt - 1];
+        const double ci = -(((double)dir.x * ax[0] + (double)dir.y * ax[1]) + (double)dir.z * ax[2]);
+        ang_acc = interp_table(ci, ep.ang_x, ep.ang_y, ep.n_ang);
+    }
+    weight = (float)(T->area_acc * wl_acc * abs_w * ang_acc);
     // calc_time_residual!, photon_prop_cuda.jl:906-917
     tres = (float)((double)time - T->tgeo);
 }
@@ -647,8 +657,9 @@ __global__ __launch_bounds__(256) void epilogue_kernel(Epilogue ep, HitCols h, c
         uint8_t pmt = 0;
         float w = 0.0f, tres = 0.0f;
         if (ti >= 0)
-            hit_epilogue(ep, ti, make_float3(h.pos[3 * i], h.pos[3 * i + 1], h.pos[3 * i + 2]), h.wl[i],
-                         h.dist[i], h.time[i], pmt, w, tres);
+            hit_epilogue(ep, ti, make_float3(h.pos[3 * i], h.pos[3 * i + 1], h.pos[3 * i + 2]),
+                         make_float3(h.dir[3 * i], h.dir[3 * i + 1], h.dir[3 * i + 2]), h.wl[i], h.dist[i],
+                         h.time[i], pmt, w, tres);
         if (h.pmt_id) h.pmt_id[i] = pmt;
         if (h.total_weight) h.total_weight[i] = w;
         if (h.tres) h.tres[i] = tres;
@@ -796,7 +807,7 @@ struct nt_ctx {
     std::vector<nt_target> host_targets;
     bool epi_enabled = false;
     std::vector<float> abs_x, abs_y;
-    std::vector<double> acc_x, acc_y;
+    std::vector<double> acc_x, acc_y, ang_x, ang_y;
     float abs_len_scale = 1.0f;
     double orientation[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     float epi_source[3] = {0, 0, 0};
@@ -804,7 +815,8 @@ struct nt_ctx {
     int32_t* d_target_of_module = nullptr;   // module id -> target list index (65536 entries)
     float* d_abs = nullptr;             // abs_x then abs_y
     double* d_acc = nullptr;            // acc_x then acc_y
-    int64_t epi_targets_cap = 0, abs_cap = 0, acc_cap = 0;
+    double* d_ang = nullptr;            // ang_x then ang_y
+    int64_t epi_targets_cap = 0, abs_cap = 0, acc_cap = 0, ang_cap = 0;
     Epilogue epi{};
     // hit columns owned by the synchronous API
     HitCols d_hits{};
@@ -1034,7 +1046,13 @@ int32_t rebuild_epilogue(nt_ctx* c) {
     if (st) return st;
     st = grow(&c->d_acc, (int64_t)(2 * c->acc_x.size()), &c->acc_cap);
     if (st) return st;
-    const size_t na = c->abs_x.size(), nc = c->acc_x.size();
+    st = grow(&c->d_ang, (int64_t)std::max<size_t>(2 * c->ang_x.size(), 1), &c->ang_cap);
+    if (st) return st;
+    const size_t na = c->abs_x.size(), nc = c->acc_x.size(), ng = c->ang_x.size();
+    if (ng) {
+        HIP_TRY(hipMemcpy(c->d_ang, c->ang_x.data(), 8 * ng, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(c->d_ang + ng, c->ang_y.data(), 8 * ng, hipMemcpyHostToDevice));
+    }
     HIP_TRY(hipMemcpy(c->d_epi_targets, et.data(), sizeof(EpiTarget) * et.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->d_abs, c->abs_x.data(), 4 * na, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->d_abs + na, c->abs_y.data(), 4 * na, hipMemcpyHostToDevice));
@@ -1048,6 +1066,9 @@ int32_t rebuild_epilogue(nt_ctx* c) {
     d.acc_y = c->d_acc + nc;
     d.n_abs = (int)na;
     d.n_acc = (int)nc;
+    d.ang_x = c->d_ang;
+    d.ang_y = c->d_ang + ng;
+    d.n_ang = (int)ng;
     d.abs_len_scale = c->abs_len_scale;
     d.enabled = 1;
     return NT_OK;
@@ -1278,7 +1299,7 @@ void nt_destroy(nt_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     void* ptrs[] = {c->d_knots, c->d_targets, c->d_module_ids, c->d_sources, c->d_offsets, c->d_counters,
                     c->d_clusters, c->d_members,
-                    c->d_member_idx, c->d_epi_targets, c->d_abs, c->d_acc, c->d_target_of_module};
+                    c->d_member_idx, c->d_epi_targets, c->d_abs, c->d_acc, c->d_ang, c->d_target_of_module};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     free_hits(c->d_hits);
@@ -1667,14 +1688,19 @@ int32_t nt_set_epilogue(nt_ctx* c, const nt_epilogue* e) {
     };
     if (e->n_abs < 2 || e->n_abs > 1024 || !e->abs_wavelength || !e->abs_length || e->n_acc < 2 ||
         e->n_acc > 1024 || !e->acc_wavelength || !e->acc_value || !(e->abs_len_scale > 0.0f) ||
-        !ascending(e->abs_wavelength, e->n_abs) || !ascending(e->acc_wavelength, e->n_acc)) {
-        snprintf(g_err, sizeof g_err, "epilogue tables need 2..1024 ascending rows and abs_len_scale > 0");
+        !ascending(e->abs_wavelength, e->n_abs) || !ascending(e->acc_wavelength, e->n_acc) ||
+        (e->n_ang != 0 && (e->n_ang < 2 || e->n_ang > 1024 || !e->ang_cos || !e->ang_value ||
+                           !ascending(e->ang_cos, e->n_ang)))) {
+        snprintf(g_err, sizeof g_err, "epilogue tables need 2..1024 ascending rows (angular: 0 or 2..1024) "
+                 "and abs_len_scale > 0");
         return NT_ERR_ARG;
     }
     c->abs_x.assign(e->abs_wavelength, e->abs_wavelength + e->n_abs);
     c->abs_y.assign(e->abs_length, e->abs_length + e->n_abs);
     c->acc_x.assign(e->acc_wavelength, e->acc_wavelength + e->n_acc);
     c->acc_y.assign(e->acc_value, e->acc_value + e->n_acc);
+    c->ang_x.assign(e->ang_cos, e->ang_cos + (e->n_ang > 0 ? e->n_ang : 0));
+    c->ang_y.assign(e->ang_value, e->ang_value + (e->n_ang > 0 ? e->n_ang : 0));
     c->abs_len_scale = e->abs_len_scale;
     memcpy(c->orientation, e->orientation, sizeof c->orientation);
     memcpy(c->epi_source, e->source_position, sizeof c->epi_source);
@@ -1687,6 +1713,15 @@ int32_t nt_set_option(nt_ctx* c, int32_t option, int64_t value) {
     std::lock_guard<std::mutex> lk(c->mu);
     switch (option) {
         case NT_OPT_TARGET_CULLING: c->culling = value != 0; return NT_OK;
+        case NT_OPT_CHERENKOV_ANGLE_URAD:
+            if (value < 0 || value >= 3141593) {
+                snprintf(g_err, sizeof g_err, "Cherenkov angle: 0 (reference) or 1..3141592 microradians");
+                return NT_ERR_ARG;
+            }
+            HIP_TRY(hipSetDevice(c->device));
+            if (int32_t w = wait_last_launch(c)) return w;
+            c->medium.cherenkov_angle = (double)value * 1e-6;
+            return NT_OK;
         default:
             snprintf(g_err, sizeof g_err, "unknown option %d", option);
             return NT_ERR_ARG;

@@ -153,6 +153,7 @@ struct Medium {
     int dispersion_mode;
     int max_steps;
     int dir_mode;             // NT_DIR_FP64 or NT_DIR_FP32_REF
+    double cherenkov_angle;   // > 0: fixed emission angle (rad); 0: acos(1/n(λ)) as the reference
 };
 
 // refractive_index_fry, medium.jl:234-244
@@ -502,11 +503,17 @@ __device__ __forceinline__ D3 track_dir64(float u_tcos, float u_tphi) {
 // cos θc = 1/n(λ), sin θc = √(n² − 1)/n, then the two rotations.
 __device__ __forceinline__ float3 cherenkov_dir(float wl, float u_phi, float u_tcos, float u_tphi,
                                                 float3 src_dir, const Medium& m) {
-    const double x = rcp64((double)wl), x2 = x * x;
-    const double n = __builtin_fma(x, (double)m.a2, (double)m.a01) +
-                     __builtin_fma(x2, (double)m.a3, x2 * x * (double)m.a4);
-    const double ctc = rcp64(n);
-    const double stc = sqrt64(__builtin_fma(n, n, -1.0)) * ctc;
+    double ctc, stc;
+    if (m.cherenkov_angle > 0.0) {                 // fixed emission angle (configuration option)
+        ctc = cos(m.cherenkov_angle);
+        stc = sin(m.cherenkov_angle);
+    } else {
+        const double x = rcp64((double)wl), x2 = x * x;
+        const double n = __builtin_fma(x, (double)m.a2, (double)m.a01) +
+                         __builtin_fma(x2, (double)m.a3, x2 * x * (double)m.a4);
+        ctc = rcp64(n);
+        stc = sqrt64(__builtin_fma(n, n, -1.0)) * ctc;
+    }
     double sp, cp;
     sincos_rev64(u_phi, sp, cp);
     const D3 ph{cp * stc, sp * stc, ctc};
@@ -526,7 +533,9 @@ __device__ __forceinline__ float3 track_dir32(float u_tcos, float u_tphi) {
 
 __device__ __forceinline__ float3 cherenkov_dir32(float wl, float u_phi, float u_tcos, float u_tphi,
                                                   float3 src_dir, const Medium& m) {
-    const float3 ph = sph_to_cart_cos(1.0f / refractive_index(wl, m), u_phi);   // θc = acos(1/n)
+    // θc = acos(1/n), or the fixed angle of the configuration option
+    const float ct = m.cherenkov_angle > 0.0 ? (float)cos(m.cherenkov_angle) : 1.0f / refractive_index(wl, m);
+    const float3 ph = sph_to_cart_cos(ct, u_phi);
     return rot_from_ez32(src_dir, rot_from_ez32(track_dir32(u_tcos, u_tphi), ph));
 }
 

@@ -205,3 +205,12 @@ def make_detector_cube(nx, ny, nz, spacing_vert, spacing_hor):
 def make_targets(positions, n_pmts, pmt_area):
     """detection.jl:208-210."""
     return [DetectionSphere(p, 0.21, n_pmts, pmt_area) for p in positions]
+
+
+def lambertian_angular_acceptance(n: int = 21):
+    """A stand-in angular acceptance table (the configuration names a
+    tabulated angular × wavelength acceptance and the reference provides
+    none): the PMT's projected area, factor cos η for incidence angles η below
+    90°, 0 beyond, at n cosines in [−1, 1]."""
+    c = np.linspace(-1.0, 1.0, n)
+    return c, np.clip(c, 0.0, None)

@@ -167,7 +167,7 @@ class PropagationEngine:
         self.set_sources(setup.sources)
 
     def set_epilogue(self, medium: WaterProperties | None = None, orientation=None,
-                     source_position=(0.0, 0.0, 0.0), enabled: bool = True):
+                     source_position=(0.0, 0.0, 0.0), enabled: bool = True, angular_acceptance=None):
         """Enable the device hit epilogue (PMT id, total weight, time residual)
         with the absorption and PMT-acceptance tables of this package."""
         e = _abi.nt_epilogue()
@@ -192,8 +192,21 @@ class PropagationEngine:
                 e.orientation[i] = float(v)
             for i, v in enumerate(np.asarray(source_position, np.float32).reshape(3)):
                 e.source_position[i] = float(v)
+            if angular_acceptance is not None:
+                cx, cy = angular_acceptance
+                self._ang = (np.ascontiguousarray(cx, np.float64), np.ascontiguousarray(cy, np.float64))
+                e.n_ang = len(self._ang[0])
+                e.ang_cos = self._ang[0].ctypes.data_as(C.POINTER(C.c_double))
+                e.ang_value = self._ang[1].ctypes.data_as(C.POINTER(C.c_double))
         _abi.check(self.lib.nt_set_epilogue(self.handle, C.byref(e)), "nt_set_epilogue")
         self._epilogue = bool(enabled)
+
+    def set_cherenkov_angle(self, angle_rad: float | None = None):
+        """Emit Cherenkov photons at a fixed angle (rad) instead of
+        acos(1/n(λ)) (NT_OPT_CHERENKOV_ANGLE_URAD); None restores the
+        reference behaviour."""
+        v = 0 if angle_rad is None else int(round(float(angle_rad) * 1e6))
+        _abi.check(self.lib.nt_set_option(self.handle, _abi.NT_OPT_CHERENKOV_ANGLE_URAD, v), "nt_set_option")
 
     def launch_shape(self):
         b, t = C.c_int32(), C.c_int32()
@@ -308,9 +321,13 @@ def _target_map(setup: PhotonPropSetup):
     return {int(t.module_id): t for t in setup.targets}
 
 
-def calc_total_weight(hits: PhotonHits, setup: PhotonPropSetup) -> PhotonHits:
+def calc_total_weight(hits: PhotonHits, setup: PhotonPropSetup, angular_acceptance=None,
+                      orientation=None) -> PhotonHits:
     """photon_prop_cuda.jl:850-882: area_acc × wl_acc × exp(−dist/L_abs(λ)),
-    added as extra columns area_acc, abs_weight, wl_acc, ref_ix, total_weight."""
+    added as extra columns area_acc, abs_weight, wl_acc, ref_ix, total_weight.
+    With `angular_acceptance` = (cos of incidence, factor) the weights of hits
+    on a PMT of a MultiPMTDetector (under `orientation`) are multiplied by the
+    table at −direction·(PMT axis) (no reference counterpart)."""
     tmap = _target_map(setup)
     area = np.array([area_acceptance(None, tmap[int(m)]) for m in hits.module_id], dtype=np.float64)
     abs_len = absorption_length(hits.wavelength, setup.medium)
@@ -320,7 +337,23 @@ def calc_total_weight(hits: PhotonHits, setup: PhotonPropSetup) -> PhotonHits:
     hits.extra["abs_weight"] = abs_weight
     hits.extra["wl_acc"] = wl_acc
     hits.extra["ref_ix"] = refractive_index(hits.wavelength, setup.medium)
-    hits.extra["total_weight"] = area * wl_acc * abs_weight
+    ang = np.ones(len(hits))
+    if angular_acceptance is not None:
+        from .detection import MultiPMTDetector, get_pmt_positions
+        cx, cy = (np.asarray(a, np.float64) for a in angular_acceptance)
+        o = np.eye(3) if orientation is None else np.asarray(orientation, np.float64)
+        for mid in np.unique(hits.module_id):
+            t = tmap[int(mid)]
+            if not isinstance(t, MultiPMTDetector):
+                continue
+            sel = np.flatnonzero(hits.module_id == mid)
+            pmt = check_pmt_hit(hits.position[sel], t, orientation)
+            axes = get_pmt_positions(t, o)
+            on = pmt > 0
+            ci = -np.sum(hits.direction[sel][on].astype(np.float64) * axes[pmt[on] - 1], axis=1)
+            ang[sel[on]] = np.interp(ci, cx, cy)
+    hits.extra["angular_acceptance"] = ang
+    hits.extra["total_weight"] = area * wl_acc * abs_weight * ang
     return hits
 
 

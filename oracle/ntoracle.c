@@ -322,10 +322,15 @@ void ntor_cherenkov_track_dir(float u_cos, float u_phi, float out[3]) {
     ntor_sph_to_cart(acosf(c), phi, out);
 }
 
+/* Fixed Cherenkov emission angle (rad) of the configuration option
+   NT_OPT_CHERENKOV_ANGLE_URAD; 0 = acos(1/n(λ)) as the reference. */
+static double g_cherenkov_angle = 0.0;
+void ntor_set_cherenkov_angle(double rad) { g_cherenkov_angle = rad; }
+
 /* sample_cherenkov_direction, photon_prop_cuda.jl:107-125 */
 void ntor_cherenkov_dir(float wl, float u_phi, float u_tcos, float u_tphi, const float src_dir[3],
                         const float qf[4], float out[3]) {
-    float theta_c = ntor_cherenkov_angle(wl, qf);
+    float theta_c = g_cherenkov_angle > 0.0 ? acosf((float)cos(g_cherenkov_angle)) : ntor_cherenkov_angle(wl, qf);
     float phi = 0.0f + u_phi * TWO_PI_F;
     float ph[3], track[3], tmp[3];
     ntor_sph_to_cart(theta_c, phi, ph);
@@ -482,6 +487,10 @@ void ntor_cherenkov_dir_f64(float wl, float u_phi, float u_tcos, float u_tphi, c
     double q[4] = {qf[0], qf[1], qf[2], qf[3]};
     double n = ntor_refractive_index_f64(wl, q);
     double ct = 1.0 / n, st = sqrt(1.0 - ct * ct), phi = 2.0 * M_PI * (double)u_phi;
+    if (g_cherenkov_angle > 0.0) {
+        ct = cos(g_cherenkov_angle);
+        st = sin(g_cherenkov_angle);
+    }
     double ph[3] = {cos(phi) * st, sin(phi) * st, ct}, track[3], tmp[3];
     double src[3] = {src_dir[0], src_dir[1], src_dir[2]};
     ntor_cherenkov_track_dir_f64(u_tcos, u_tphi, track);

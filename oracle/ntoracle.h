@@ -63,6 +63,7 @@ double ntor_hg_cos_f64(double g, double eta);
 double ntor_sl_cos_f64(double g, double xi);
 void ntor_rot_from_ez_f64(const double a[3], const double op[3], double out[3]);
 void ntor_cherenkov_track_dir_f64(double u_cos, double u_phi, double out[3]);
+void ntor_set_cherenkov_angle(double rad);
 void ntor_cherenkov_dir_f64(float wl, float u_phi, float u_tcos, float u_tphi, const float src_dir[3],
                             const float qf[4], double out[3]);
 void ntor_update_dir_f64(const float dir[3], float eta, float u_phi, float u_mix, const nt_medium* m,

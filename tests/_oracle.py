@@ -75,6 +75,7 @@ def _bind(lib):
         "ntor_sl_cos_f64": (d, [d, d]),
         "ntor_rot_from_ez_f64": (None, [dp, dp, dp]),
         "ntor_cherenkov_track_dir_f64": (None, [d, d, dp]),
+        "ntor_set_cherenkov_angle": (None, [d]),
         "ntor_cherenkov_dir_f64": (None, [f, f, f, f, fp, fp, dp]),
         "ntor_update_dir_f64": (None, [fp, f, f, f, C.POINTER(_abi.nt_medium), dp]),
         "ntor_intersect_f64": (i, [dp, dp, d, dp, d, dp]),
