@@ -12,13 +12,25 @@
 //   log_likelihood_with_poisson    extended_cascade_model.jl:132-139
 //   train_model! / Adam            extended_cascade_model.jl:237-295
 //
-// Work decomposition: a workgroup (4 wavefronts) owns 64 rows.  Each
-// wavefront runs the three matrix products of its 16 rows on 16×16×16 MFMA
-// tiles with the weights resident in LDS, the per-row flow loss and its
-// gradient with 16 lanes, and the two backward products; the workgroup then
-// forms the weight gradients of its 64 rows (K = 64 MFMA reductions) and
-// writes one partial gradient record.  The reduction kernel sums the records
-// in a fixed order (bitwise reproducible) and applies Adam.
+// Work decomposition.  A wavefront owns 16 rows.  The layer products run
+// transposed (Hᵀ = Wᵀ·Xᵀ): the accumulator fragment of one 16-wide output
+// block is, after bias/ReLU/bfloat16, exactly the B operand of the next
+// product, so the activations of the forward and the backward chain never
+// leave registers; the weights are the A operands, read from LDS as one
+// 16-byte fragment each (v_mfma_f32_16x16x32_bf16; the 16-deep input layer
+// uses the 16x16x16 form).  The weights arrive as a bfloat16 image in the
+// exact LDS layout (rows XOR-swizzled in 16-byte chunks, bank-conflict free,
+// and the K index permuted within 32-blocks to match the register order of
+// the activation fragments), refreshed by the Adam kernel after each update.
+// The per-row flow loss and its gradient run on four lanes per row, one
+// spline layer per lane, the layer chain passed along the quad with DPP.
+// The training kernel's workgroup (4 wavefronts, 64 rows) then forms the
+// weight gradients of its rows with K = 64 MFMA reductions over
+// row-contiguous (feature-major) bfloat16 copies of the activations in LDS
+// and writes one partial gradient record; the reduction kernels sum the
+// records in a fixed order (bitwise reproducible) and apply Adam.  The
+// forward kernel keeps only the forward weights in LDS (two workgroups per
+// CU) and is persistent per wavefront over 16-row tiles.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -31,74 +43,121 @@ namespace {
 
 constexpr int kIn = NTF_N_IN, kH = NTF_HIDDEN, kP = NTF_P_PAD, kK = NTF_BINS, kL = NTF_LAYERS;
 constexpr int kNP = NTF_N_PARAMS;
-constexpr int kSpl = 3 * kK + 1;
-constexpr int kRows = 64;                 // rows per workgroup
-constexpr int kThreads = 256;             // 4 wavefronts
-constexpr int kPStride = 81;              // FP32 row stride in LDS (odd: conflict-free per-lane rows)
+constexpr int kSpl = 3 * kK + 1;          // parameters of one spline layer
+constexpr int kRows = 64;                 // rows per workgroup of the gradient kernel
+constexpr int kThreads = 256;             // 4 wavefronts of 16 rows
+constexpr int kPLd = 84;                  // FP32 row stride of an output tile in LDS (16-byte rows)
 constexpr float kRangeMin = NTF_RANGE_MIN, kRangeMax = NTF_RANGE_MAX;
 constexpr float kMinBin = 1e-4f, kMinSlope = 1e-4f;
 constexpr float kSlopeOffset = 0.5411666523385311f;   // log(exp(1 − 1e-4) − 1), rq_spline_flow.jl:151
 constexpr float kLog2Pi = 1.8378770664093453f;
+static_assert(kIn == 16 && kH == 128 && kP == 80, "fragment tiling assumes 16 inputs, 128 hidden, 80 outputs");
+static_assert(kL * kSpl + 3 == kNP && kNP <= kP, "outputs");
 
 constexpr int oW1 = 0, ob1 = oW1 + kIn * kH, oW2 = ob1 + kH, ob2 = oW2 + kH * kH, oW3 = ob2 + kH,
               ob3 = oW3 + kH * kP, kNW = ob3 + kP;
 static_assert(kNW == NTF_N_WEIGHTS, "weight layout");
-static_assert(kNP <= kP, "outputs");
 
 // partial gradient record of one workgroup: dW1 | dW2 | dW3 | 4 × (db1, db2, db3)
 constexpr int pW1 = 0, pW2 = pW1 + kIn * kH, pW3 = pW2 + kH * kH, pBias = pW3 + kH * kP;
 constexpr int kBiasW = kH + kH + kP;
 constexpr int kPartW = pBias + 4 * kBiasW;
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+// bfloat16 weight image (halfword offsets), the LDS layout of the weights:
+//   W1ᵀ [128 h][16 f]      8-byte units XOR-swizzled by bit 3 of the row
+//   W2ᵀ [128 out][128 in]  forward A operands    (128-wide rows: 16-byte chunks
+//   W3ᵀ [80 c][128 h]      forward A operands     XOR (row & 15), K permuted
+//   W2  [128 in][128 out]  backward A operands    within 32-blocks)
+//   W3  [128 h][88]        backward A operands (K = c: two permuted 32-blocks, then c = 64..79 in order)
+constexpr int hW1T = 0;
+constexpr int hW2T = hW1T + kH * kIn;
+constexpr int hW3T = hW2T + kH * kH;
+constexpr int hFwd = hW3T + kP * kH;      // end of the forward set
+constexpr int hW2 = hFwd;
+constexpr int kW3Ld = 88;
+constexpr int hW3 = hW2 + kH * kH;
+constexpr int hImg = hW3 + kH * kW3Ld;
+static_assert(hFwd % 8 == 0 && hImg % 8 == 0, "16-byte copies");
+
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// FP32 → bfloat16, round to nearest even
-__device__ __forceinline__ uint16_t f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
+__device__ __forceinline__ uint16_t bits(__bf16 b) { return __builtin_bit_cast(uint16_t, b); }
+__host__ __device__ constexpr int qperm(int L) { return 8 * ((L & 15) >> 2) + ((L >> 4) << 2) + (L & 3); }
+// halfword column of logical index k in a swizzled 128-wide row r
+__host__ __device__ constexpr int sw_col(int k, int r) {
+    return ((((32 * (k >> 5) + qperm(k & 31)) >> 3) ^ (r & 15)) << 3) + (qperm(k & 31) & 7);
 }
 
-__device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
+// 16×16 MFMA operand layout (wavefront of 64, lane = 16·g + i): A supplies row
+// i, B column i, the K slots of lane group g; the accumulator holds C[4g + j][i].
+// Activation fragments: element e of lane group g of 32-block kb is feature
+// 32·kb + L(8g + e), L(8g + e) = 4g + e (e < 4) or 16 + 4g + e − 4, which is
+// where the accumulators of output blocks 2kb and 2kb + 1 put them.
+__device__ __forceinline__ f32x4 mfma16(s16x4 a, s16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
-
-struct __align__(16) Smem {
-    uint16_t w1t[kH][kIn];      // W1ᵀ [hidden][feature]
-    uint16_t w2[kH][kH];        // W2 [in][out]
-    uint16_t w3[kH][kP];        // W3 [in][out]
-    uint16_t x[kRows][kIn];
-    uint16_t h1[kRows][kH];
-    uint16_t h2[kRows][kH];
-    uint16_t dp[kRows][kP];
-    uint16_t dh2[kRows][kH];
-    uint16_t dh1[kRows][kH];
-    float p[kRows][kPStride];   // network outputs, then the FP32 output gradient
-    float b[kH + kH + kP];      // b1 | b2 | b3
-    float loss[kRows];
-};
-
-// 16×16×16 bf16 MFMA operand layout (wavefront of 64): lane = 16·g + i holds
-//   A: A[i][4g .. 4g+3]      B: B[4g .. 4g+3][i]      C: C[4g + j][i], j = 0..3
-// A fragment, row-major source M[.][ld]: rows r0+i, columns k0+4g..
-template <int LD>
-__device__ __forceinline__ bf16x4 frag_rows(const uint16_t (*m)[LD], int r0, int k0, int i, int g) {
-    return *reinterpret_cast<const bf16x4*>(&m[r0 + i][k0 + 4 * g]);
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
-// fragment gathered down a column: M[k0+4g+j][c0+i], j = 0..3
-template <int LD>
-__device__ __forceinline__ bf16x4 frag_cols(const uint16_t (*m)[LD], int k0, int c0, int i, int g) {
-    bf16x4 v;
-    v[0] = (short)m[k0 + 4 * g + 0][c0 + i];
-    v[1] = (short)m[k0 + 4 * g + 1][c0 + i];
-    v[2] = (short)m[k0 + 4 * g + 2][c0 + i];
-    v[3] = (short)m[k0 + 4 * g + 3][c0 + i];
+// the K slots 0..3 / 4..7 of a 32-deep fragment: the two 16-deep blocks 2kb, 2kb+1 in natural order
+__device__ __forceinline__ s16x4 half_lo(bf16x8 v) {
+    return __builtin_bit_cast(s16x4, __builtin_shufflevector(v, v, 0, 1, 2, 3));
+}
+__device__ __forceinline__ s16x4 half_hi(bf16x8 v) {
+    return __builtin_bit_cast(s16x4, __builtin_shufflevector(v, v, 4, 5, 6, 7));
+}
+// a 16-deep fragment as a 32-deep one with zero upper K slots.  Every
+// accumulation chain keeps one MFMA opcode: this compiler does not pad a
+// 16x16x16 MFMA that reads the accumulator of a 16x16x32 one with the wait
+// states the hardware needs.
+__device__ __forceinline__ bf16x8 widen(s16x4 v) {
+    const bf16x4 z = {};
+    return __builtin_shufflevector(__builtin_bit_cast(bf16x4, v), z, 0, 1, 2, 3, 4, 5, 6, 7);
+}
+__device__ __forceinline__ s16x4 ld_w1t(const uint16_t* s, int mb, int i, int g) {
+    return *reinterpret_cast<const s16x4*>(s + hW1T + (16 * mb + i) * kIn + ((g ^ ((i >> 3) << 1)) << 2));
+}
+__device__ __forceinline__ bf16x8 ld_sw(const uint16_t* base, int row, int kb, int i, int g) {
+    return *reinterpret_cast<const bf16x8*>(base + row * 128 + (((4 * kb + g) ^ i) << 3));
+}
+__device__ __forceinline__ bf16x8 ld_w3(const uint16_t* s, int row, int kb, int g) {
+    return *reinterpret_cast<const bf16x8*>(s + hW3 + row * kW3Ld + 32 * kb + 8 * g);
+}
+__device__ __forceinline__ s16x4 ld_w3_tail(const uint16_t* s, int row, int g) {
+    return *reinterpret_cast<const s16x4*>(s + hW3 + row * kW3Ld + 64 + 4 * g);
+}
+// feature-major activation copies T[feature][64 rows], 16-byte chunks XOR ((feature >> 1) & 7)
+__device__ __forceinline__ int t_idx(int feat, int r) { return feat * 64 + ((((r >> 3) ^ ((feat >> 1) & 7))) << 3) + (r & 7); }
+__device__ __forceinline__ bf16x8 ld_t(const uint16_t* T, int feat, int ks, int i, int g) {
+    return *reinterpret_cast<const bf16x8*>(T + feat * 64 + (((4 * ks + g) ^ ((i >> 1) & 7)) << 3));
+}
+
+template <int Ctrl>
+__device__ __forceinline__ float dpp(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), Ctrl, 0xF, 0xF, false));
+}
+template <int L>
+__device__ __forceinline__ float quad_bcast(float v) { return dpp<L | (L << 2) | (L << 4) | (L << 6)>(v); }
+// Σ over the 16 lanes of a DPP row (fixed order; every lane gets the same sum)
+__device__ __forceinline__ float row16_sum(float v) {
+    v += dpp<0xB1>(v);         // quad_perm [1,0,3,2]
+    v += dpp<0x4E>(v);         // quad_perm [2,3,0,1]
+    v += dpp<0x141>(v);        // row_half_mirror
+    v += dpp<0x140>(v);        // row_mirror
     return v;
+}
+// LDS written by other lanes of this wavefront is visible to the next reads
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // ---------------------------------------------------------------------------
-// Spline of one layer (FP32)
+// Spline of one layer (FP32), register resident: every array index is static
 // ---------------------------------------------------------------------------
 struct Knots {
     float xp[kK + 1], yp[kK + 1], d[kK + 1];
@@ -122,7 +181,7 @@ __device__ __forceinline__ void softmax8(const float* u, float* s) {
 }
 
 // constrain_spline_params, rq_spline_flow.jl:170-216
-__device__ __forceinline__ void make_knots(const float* u, Knots& k) {
+__device__ __forceinline__ void make_knots(const float (&u)[kSpl], Knots& k) {
     softmax8(u, k.sw);
     softmax8(u + kK, k.sh);
     const float span = (kRangeMax - kRangeMin) - kK * kMinBin;
@@ -155,167 +214,274 @@ __device__ __forceinline__ int find_bin(const Knots& k, float v) {
     return j;
 }
 
+// a[j], a[j + 1] by selects (no dynamic register indexing)
+__device__ __forceinline__ void pick2(const float (&a)[kK + 1], int j, float& lo, float& hi) {
+    lo = a[0];
+    hi = a[1];
+#pragma unroll
+    for (int i = 1; i < kK; ++i) {
+        lo = i == j ? a[i] : lo;
+        hi = i == j ? a[i + 1] : hi;
+    }
+}
+
 __device__ __forceinline__ float safe_root(float a, float b, float c) {
     const float disc = b * b - 4.0f * a * c;
     const float s = disc > 0.0f ? sqrtf(disc) : 0.0f;
     return b >= 0.0f ? (2.0f * c) / (-b - s) : (-b + s) / (2.0f * a);
 }
 
-// inv_rqs_univariate (rq_spline_flow.jl:334-389) at v; returns x, ld = log|dx/dv|.
-// With grad: accumulates d/du of (gx·x + gld·ld) into gu and returns d/dv.
-template <bool kGrad>
-__device__ float inv_layer(const float* u, float v, float& out, float gx, float gld, float* gu) {
-    Knots k;
-    make_knots(u, k);
+// inv_rqs_univariate (rq_spline_flow.jl:334-389) at v: returns x, ld = log|dx/dv|
+__device__ __forceinline__ float layer_fwd(const Knots& k, float v, float& ld) {
     if (v <= k.yp[0] || v >= k.yp[kK]) {          // linear tails
-        const int e = v <= k.yp[0] ? 0 : kK;
-        const float de = k.d[e], edge = k.yp[e];
-        if (!kGrad) {
-            out = -logf(de);
-            return (v - edge) / de + k.xp[e];
-        }
-        const float gde = -gx * (v - edge) / (de * de) - gld / de;
-        gu[2 * kK + e] += gde * k.sg[e];
-        return gx / de;
+        const bool lo = v <= k.yp[0];
+        const float de = lo ? k.d[0] : k.d[kK], edge = lo ? k.yp[0] : k.yp[kK], xe = lo ? k.xp[0] : k.xp[kK];
+        ld = -logf(de);
+        return (v - edge) / de + xe;
     }
     const int j = find_bin(k, v);
-    const float x0 = k.xp[j], x1 = k.xp[j + 1], y0 = k.yp[j], y1 = k.yp[j + 1];
-    const float d0 = k.d[j], d1 = k.d[j + 1];
+    float x0, x1, y0, y1, d0, d1;
+    pick2(k.xp, j, x0, x1);
+    pick2(k.yp, j, y0, y1);
+    pick2(k.d, j, d0, d1);
     const float w = x1 - x0, h = y1 - y0, s = h / w;
     const float wv = fminf(fmaxf((v - y0) / h, 0.0f), 1.0f);
     const float st = d1 + d0 - 2.0f * s;
     const float qc = -s * wv;
     const float qb = d0 - st * wv;
     const float qa = s - qb;
-    const float zr = safe_root(qa, qb, qc);
-    const float z = fminf(fmaxf(zr, 0.0f), 1.0f);
+    const float z = fminf(fmaxf(safe_root(qa, qb, qc), 0.0f), 1.0f);
     const float sq_z = z * z, z1mz = z - sq_z, omz = 1.0f - z, sq_1mz = omz * omz;
     const float Q = d1 * sq_z + 2.0f * s * z1mz + d0 * sq_1mz;
     const float D = s + st * z1mz;
-    if (!kGrad) {
-        out = -2.0f * logf(s) - logf(Q) + 2.0f * logf(D);
-        return w * z + x0;
-    }
-    const float N = s * sq_z + d0 * z1mz;
-    const float R = N / D;
-    const float dDz = st * (1.0f - 2.0f * z);
-    const float dQz = 2.0f * d1 * z + 2.0f * s * (1.0f - 2.0f * z) - 2.0f * d0 * omz;
-    const float dNz = 2.0f * s * z + d0 * (1.0f - 2.0f * z);
-    // ld = −2 log s − log Q + 2 log D
-    const float ldz = -dQz / Q + 2.0f * dDz / D;
-    const float lds = -2.0f / s - 2.0f * z1mz / Q + 2.0f * (1.0f - 2.0f * z1mz) / D;
-    const float ldd0 = -sq_1mz / Q + 2.0f * z1mz / D;
-    const float ldd1 = -sq_z / Q + 2.0f * z1mz / D;
-    const float gz = gx * w + gld * ldz;
-    float gw = gx * z, gs = gld * lds, gd0 = gld * ldd0, gd1 = gld * ldd1;
-    float gv = 0.0f, gy0 = 0.0f, gh = 0.0f;
-    if (zr == z) {
-        // z solves h·R(z; s, d0, d1) = v − y0 (the forward spline in the bin)
-        const float Rp = (dNz - R * dDz) / D;
-        const float kappa = gz / (h * Rp);
-        const float Rs = (sq_z - R * (1.0f - 2.0f * z1mz)) / D;
-        const float Rd0 = (z1mz - R * z1mz) / D;
-        const float Rd1 = (-R * z1mz) / D;
-        gv = kappa;
-        gy0 = -kappa;
-        gh = -kappa * R;
-        gs -= kappa * h * Rs;
-        gd0 -= kappa * h * Rd0;
-        gd1 -= kappa * h * Rd1;
-    }
-    gh += gs / w;                      // s = h / w
-    gw -= gs * s / w;
-    // knots: x0 = xp[j], w = xp[j+1] − xp[j] (likewise y); interior knots only
-    float gxp[kK + 1], gyp[kK + 1];
-#pragma unroll
-    for (int i = 0; i <= kK; ++i) gxp[i] = gyp[i] = 0.0f;
-    gxp[j] += gx - gw;
-    gxp[j + 1] += gw;
-    gyp[j] += gy0 - gh;
-    gyp[j + 1] += gh;
-    gu[2 * kK + j] += gd0 * k.sg[j];
-    gu[2 * kK + j + 1] += gd1 * k.sg[j + 1];
-    // xp[i] = lo + Σ_{m<i} (sw[m]·span + min): d xp[i] / d sw[m] = span for m < i < K
-    const float span = (kRangeMax - kRangeMin) - kK * kMinBin;
-    float gsw[kK], gsh[kK];
-    float accx = 0.0f, accy = 0.0f;
-#pragma unroll
-    for (int m = kK - 1; m >= 0; --m) {
-        if (m + 1 <= kK - 1) {
-            accx += gxp[m + 1];
-            accy += gyp[m + 1];
-        }
-        gsw[m] = accx * span;
-        gsh[m] = accy * span;
-    }
-    float dotw = 0.0f, doth = 0.0f;
-#pragma unroll
-    for (int m = 0; m < kK; ++m) {
-        dotw += k.sw[m] * gsw[m];
-        doth += k.sh[m] * gsh[m];
-    }
-#pragma unroll
-    for (int m = 0; m < kK; ++m) {
-        gu[m] += k.sw[m] * (gsw[m] - dotw);
-        gu[kK + m] += k.sh[m] * (gsh[m] - doth);
-    }
-    return gv;
+    ld = -2.0f * logf(s) - logf(Q) + 2.0f * logf(D);
+    return w * z + x0;
 }
 
-// Row loss ω·(−[time·logpdf(t) + count·(n log λ − λ − lgamma(n+1))]) from the
-// FP32 outputs p (LDS row), with log λ = p_λ + offset; with kGrad, writes
-// scale·d loss / d p into g.
+// Gradient of one inverse layer at v: d/du of (gx·x + gld·ld) and d/dv.  init
+// holds everything that does not depend on the incoming gradient gx, so the
+// layer chain only passes gv(gx) from lane to lane.
+struct LayerBwd {
+    bool tail, root;
+    int j;
+    float de, dv;                                    // tails: edge slope, v − edge
+    float w, h, s, z, R, ldz, lds, ldd0, ldd1, Rp, Rs, Rd0, Rd1, sg0, sg1;
+
+    __device__ __forceinline__ void init(const Knots& k, float v) {
+        const bool lo = v <= k.yp[0];
+        tail = lo || v >= k.yp[kK];
+        j = lo ? 0 : kK;
+        de = lo ? k.d[0] : k.d[kK];
+        dv = v - (lo ? k.yp[0] : k.yp[kK]);
+        root = false;
+        w = h = s = z = R = ldz = lds = ldd0 = ldd1 = Rp = Rs = Rd0 = Rd1 = sg0 = sg1 = 0.0f;
+        if (tail) return;
+        j = find_bin(k, v);
+        float x0, x1, y0, y1, d0, d1;
+        pick2(k.xp, j, x0, x1);
+        pick2(k.yp, j, y0, y1);
+        pick2(k.d, j, d0, d1);
+        pick2(k.sg, j, sg0, sg1);
+        w = x1 - x0;
+        h = y1 - y0;
+        s = h / w;
+        const float wv = fminf(fmaxf((v - y0) / h, 0.0f), 1.0f);
+        const float st = d1 + d0 - 2.0f * s;
+        const float qc = -s * wv;
+        const float qb = d0 - st * wv;
+        const float qa = s - qb;
+        const float zr = safe_root(qa, qb, qc);
+        z = fminf(fmaxf(zr, 0.0f), 1.0f);
+        root = zr == z;
+        const float sq_z = z * z, z1mz = z - sq_z, omz = 1.0f - z, sq_1mz = omz * omz;
+        const float Q = d1 * sq_z + 2.0f * s * z1mz + d0 * sq_1mz;
+        const float D = s + st * z1mz;
+        const float N = s * sq_z + d0 * z1mz;
+        R = N / D;
+        const float dDz = st * (1.0f - 2.0f * z);
+        const float dQz = 2.0f * d1 * z + 2.0f * s * (1.0f - 2.0f * z) - 2.0f * d0 * omz;
+        const float dNz = 2.0f * s * z + d0 * (1.0f - 2.0f * z);
+        // ld = −2 log s − log Q + 2 log D
+        ldz = -dQz / Q + 2.0f * dDz / D;
+        lds = -2.0f / s - 2.0f * z1mz / Q + 2.0f * (1.0f - 2.0f * z1mz) / D;
+        ldd0 = -sq_1mz / Q + 2.0f * z1mz / D;
+        ldd1 = -sq_z / Q + 2.0f * z1mz / D;
+        // z solves h·R(z; s, d0, d1) = v − y0 (the forward spline in the bin)
+        Rp = (dNz - R * dDz) / D;
+        Rs = (sq_z - R * (1.0f - 2.0f * z1mz)) / D;
+        Rd0 = (z1mz - R * z1mz) / D;
+        Rd1 = (-R * z1mz) / D;
+    }
+
+    __device__ __forceinline__ float gv(float gx, float gld) const {
+        if (tail) return gx / de;
+        if (!root) return 0.0f;
+        const float gz = gx * w + gld * ldz;
+        return gz / (h * Rp);
+    }
+
+    __device__ __forceinline__ void grad(const Knots& k, float gx, float gld, float (&gu)[kSpl]) const {
+#pragma unroll
+        for (int m = 0; m < kSpl; ++m) gu[m] = 0.0f;
+        if (tail) {
+            const float gde = -gx * dv / (de * de) - gld / de;
+            gu[2 * kK] = j == 0 ? gde * k.sg[0] : 0.0f;
+            gu[3 * kK] = j == kK ? gde * k.sg[kK] : 0.0f;
+            return;
+        }
+        const float gz = gx * w + gld * ldz;
+        float gw = gx * z, gs = gld * lds, gd0 = gld * ldd0, gd1 = gld * ldd1;
+        float gy0 = 0.0f, gh = 0.0f;
+        if (root) {
+            const float kappa = gz / (h * Rp);
+            gy0 = -kappa;
+            gh = -kappa * R;
+            gs -= kappa * h * Rs;
+            gd0 -= kappa * h * Rd0;
+            gd1 -= kappa * h * Rd1;
+        }
+        gh += gs / w;                      // s = h / w
+        gw -= gs * s / w;
+        // knots: x0 = xp[j], w = xp[j+1] − xp[j] (likewise y); interior knots only
+        float gxp[kK + 1], gyp[kK + 1];
+#pragma unroll
+        for (int i = 0; i <= kK; ++i) {
+            gxp[i] = i == j ? gx - gw : (i == j + 1 ? gw : 0.0f);
+            gyp[i] = i == j ? gy0 - gh : (i == j + 1 ? gh : 0.0f);
+            gu[2 * kK + i] = i == j ? gd0 * sg0 : (i == j + 1 ? gd1 * sg1 : 0.0f);
+        }
+        // xp[i] = lo + Σ_{m<i} (sw[m]·span + min): d xp[i] / d sw[m] = span for m < i < K
+        const float span = (kRangeMax - kRangeMin) - kK * kMinBin;
+        float gsw[kK], gsh[kK];
+        float accx = 0.0f, accy = 0.0f;
+#pragma unroll
+        for (int m = kK - 1; m >= 0; --m) {
+            if (m + 1 <= kK - 1) {
+                accx += gxp[m + 1];
+                accy += gyp[m + 1];
+            }
+            gsw[m] = accx * span;
+            gsh[m] = accy * span;
+        }
+        float dotw = 0.0f, doth = 0.0f;
+#pragma unroll
+        for (int m = 0; m < kK; ++m) {
+            dotw += k.sw[m] * gsw[m];
+            doth += k.sh[m] * gsh[m];
+        }
+#pragma unroll
+        for (int m = 0; m < kK; ++m) {
+            gu[m] = k.sw[m] * (gsw[m] - dotw);
+            gu[kK + m] = k.sh[m] * (gsh[m] - doth);
+        }
+    }
+};
+
+// Row loss ω·(−[time·logpdf(t) + count·(n log λ − λ − lgamma(n+1))]) of the
+// FP32 outputs prow (an LDS row), log λ = p_λ + offset, evaluated by the four
+// lanes s = 0..3 of a quad (one row): lane s < 3 owns spline layer s, lane 3
+// the tail parameters.  All four lanes return the loss.  With kGrad, the
+// quad overwrites prow with scale·d loss / d p and stores its bfloat16 copy
+// in the feature-major tile tdp at row r.
 template <bool kGrad>
-__device__ float row_loss(const float* p, float t, float cnt, int flags, float offset, float weight, float* g,
-                          float scale) {
+__device__ float quad_row_loss(float* prow, int s, float t, float cnt, int flags, float offset, float weight,
+                               float scale, uint16_t* tdp, int r) {
     const bool ft = (flags & NTF_ROW_TIME) != 0, fc = (flags & NTF_ROW_COUNT) != 0;
-    const float* tail = p + kL * kSpl;
-    const float loglam = tail[2] + offset;
+    const int seg = s < kL ? s : kL - 1;
+    const float* tail = prow + kL * kSpl;
+    const float mu = tail[0], sraw = tail[1], loglam = tail[2] + offset;
     float loss = 0.0f;
+    float g[kSpl];
+    float gt0 = 0.0f, gt1 = 0.0f, gt2 = 0.0f;
     if (kGrad) {
 #pragma unroll
-        for (int c = 0; c < kP; ++c) g[c] = 0.0f;
+        for (int m = 0; m < kSpl; ++m) g[m] = 0.0f;
     }
     if (ft) {
-        float z[kL + 1];
-        z[0] = t;
-        float ldsum = 0.0f;
+        float u[kSpl];
 #pragma unroll
-        for (int l = 0; l < kL; ++l) {
-            float ld;
-            z[l + 1] = inv_layer<false>(p + l * kSpl, z[l], ld, 0.0f, 0.0f, nullptr);
-            ldsum += ld;
-        }
-        const float sig = 1.0f / (1.0f + expf(-tail[1]));
+        for (int m = 0; m < kSpl; ++m) u[m] = prow[seg * kSpl + m];
+        Knots k;
+        make_knots(u, k);
+        // z_{l+1} = inverse layer l (z_l), layer l on lane l
+        float z = t, z_in = t, ld_mine = 0.0f, ld, y;
+        y = layer_fwd(k, z, ld);
+        if (s == 0) { z_in = z; ld_mine = ld; }
+        z = quad_bcast<0>(y);
+        y = layer_fwd(k, z, ld);
+        if (s == 1) { z_in = z; ld_mine = ld; }
+        z = quad_bcast<1>(y);
+        y = layer_fwd(k, z, ld);
+        if (s == 2) { z_in = z; ld_mine = ld; }
+        z = quad_bcast<2>(y);
+        const float ldsum = (quad_bcast<0>(ld_mine) + quad_bcast<1>(ld_mine)) + quad_bcast<2>(ld_mine);
+        const float sig = 1.0f / (1.0f + expf(-sraw));
         const float S = 100.0f * sig;
-        const float q = (z[kL] - tail[0]) / S;
+        const float q = (z - mu) / S;
         loss -= -logf(S) - 0.5f * (kLog2Pi + q * q) + ldsum;
         if (kGrad) {
-            g[kL * kSpl] = -q / S;
+            gt0 = -q / S;
             const float gS = 1.0f / S - q * q / S;
-            g[kL * kSpl + 1] = gS * S * (1.0f - sig);
-            float gx = q / S;
-            for (int l = kL - 1; l >= 0; --l) {
-                float unused;
-                gx = inv_layer<true>(p + l * kSpl, z[l], unused, gx, -1.0f, g + l * kSpl);
-            }
+            gt1 = gS * S * (1.0f - sig);
+            LayerBwd b;
+            b.init(k, z_in);
+            float gx = q / S, gx_mine = gx;          // d/dz of the last layer's output
+            gx = quad_bcast<2>(b.gv(gx, -1.0f));
+            if (s == 1) gx_mine = gx;
+            gx = quad_bcast<1>(b.gv(gx, -1.0f));
+            if (s == 0) gx_mine = gx;
+            if (s < kL) b.grad(k, gx_mine, -1.0f, g);
         }
     }
     if (fc) {
         const float lam = expf(loglam);
         loss -= cnt * loglam - lam - lgammaf(cnt + 1.0f);
-        if (kGrad) g[kL * kSpl + 2] = lam - cnt;
+        if (kGrad) gt2 = lam - cnt;
     }
     if (kGrad) {
         const float ws = weight * scale;
+        if (s < kL) {
 #pragma unroll
-        for (int c = 0; c < kP; ++c) g[c] *= ws;
+            for (int m = 0; m < kSpl; ++m) {
+                const float v = g[m] * ws;
+                prow[s * kSpl + m] = v;
+                tdp[t_idx(s * kSpl + m, r)] = bits((__bf16)v);
+            }
+        } else {
+            const float v[kP - kL * kSpl] = {gt0 * ws, gt1 * ws, gt2 * ws, 0.0f, 0.0f};
+#pragma unroll
+            for (int m = 0; m < kP - kL * kSpl; ++m) {
+                prow[kL * kSpl + m] = v[m];
+                tdp[t_idx(kL * kSpl + m, r)] = bits((__bf16)v[m]);
+            }
+        }
     }
     return loss * weight;
 }
 
+// log-density of time t under the flow outputs p of one row (one lane)
+__device__ float time_logpdf_row(const float* p, float t) {
+    float z = t, ldsum = 0.0f;
+#pragma unroll
+    for (int l = 0; l < kL; ++l) {
+        float u[kSpl];
+#pragma unroll
+        for (int m = 0; m < kSpl; ++m) u[m] = p[l * kSpl + m];
+        Knots k;
+        make_knots(u, k);
+        float ld;
+        z = layer_fwd(k, z, ld);
+        ldsum += ld;
+    }
+    const float* tail = p + kL * kSpl;
+    const float sig = 1.0f / (1.0f + expf(-tail[1]));
+    const float S = 100.0f * sig;
+    const float q = (z - tail[0]) / S;
+    return -logf(S) - 0.5f * (kLog2Pi + q * q) + ldsum;
+}
+
 // ---------------------------------------------------------------------------
-// Fused kernel
+// MLP on the matrix cores
 // ---------------------------------------------------------------------------
 struct Rows {
     const float* x;
@@ -326,220 +492,364 @@ struct Rows {
     int64_t n;
 };
 
-template <bool kGrad>
-__global__ __launch_bounds__(kThreads, 1) void flow_kernel(const float* __restrict__ w, Rows rows, float scale,
-                                                          float* __restrict__ params_out,
-                                                          float* __restrict__ loss_out,
-                                                          float* __restrict__ partial,
-                                                          double* __restrict__ partial_loss) {
-    __shared__ Smem sm;
+__device__ __forceinline__ void copy_lds(uint16_t* dst, const uint16_t* __restrict__ src, int halfs, int tid) {
+    const uint4* s = reinterpret_cast<const uint4*>(src);
+    uint4* d = reinterpret_cast<uint4*>(dst);
+    const int n16 = halfs / 8;
+#pragma unroll 4
+    for (int e = tid; e < n16; e += kThreads) d[e] = s[e];
+}
+
+__device__ __forceinline__ void load_bias(float* b, const float* __restrict__ w, int tid) {
+    for (int e = tid; e < kH; e += kThreads) {
+        b[e] = w[ob1 + e];
+        b[kH + e] = w[ob2 + e];
+    }
+    for (int e = tid; e < kP; e += kThreads) b[2 * kH + e] = w[ob3 + e];
+}
+
+// Input fragment of row `row` (B operand, K = 16 features): X[row][4g .. 4g+3]
+__device__ __forceinline__ s16x4 x_frag(const Rows& rows, int64_t row, int g, uint16_t* tx, int r) {
+    f32x4 v = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (row < rows.n) v = *reinterpret_cast<const f32x4*>(rows.x + row * kIn + 4 * g);
+    s16x4 f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        f[j] = (short)bits((__bf16)v[j]);
+        if (tx) tx[t_idx(4 * g + j, r)] = (uint16_t)f[j];
+    }
+    return f;
+}
+
+// Forward chain of one wavefront's 16 rows: H1ᵀ = relu(W1ᵀXᵀ + b1), H2ᵀ, Pᵀ.
+// Lane (i, g) ends with row i's outputs c = 16·mb + 4g + j in pv[mb][j] and
+// the ReLU masks of its hidden units 16·mb + 4g + j in bit 4·mb + j of m1, m2
+// (set where the bfloat16 activation is nonzero, i.e. > 0); with th1/th2 the
+// bfloat16 activations also go to the feature-major tiles at row r.
+__device__ __forceinline__ void forward_chain(const uint16_t* sw, const float* bias, s16x4 xf, int i, int g,
+                                              uint32_t& m1, uint32_t& m2, f32x4 (&pv)[5], uint16_t* th1,
+                                              uint16_t* th2, int r) {
+    const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    bf16x8 h1f[4], h2f[4];
+    m1 = m2 = 0u;
+#pragma unroll
+    for (int mb = 0; mb < kH / 16; ++mb) {
+        const f32x4 c = mfma16(ld_w1t(sw, mb, i, g), xf, zero);
+        const f32x4 b = *reinterpret_cast<const f32x4*>(bias + 16 * mb + 4 * g);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const __bf16 v = (__bf16)fmaxf(c[j] + b[j], 0.0f);
+            h1f[mb >> 1][(mb & 1) * 4 + j] = v;
+            m1 |= (bits(v) != 0 ? 1u : 0u) << (4 * mb + j);
+            if (th1) th1[t_idx(16 * mb + 4 * g + j, r)] = bits(v);
+        }
+    }
+    // H2 is rounded to bfloat16 after a 128-deep sum: it accumulates in 16-deep
+    // steps in natural K order (the rounding of sequential FP32 accumulation),
+    // the other products use the 32-deep instruction
+#pragma unroll
+    for (int mb = 0; mb < kH / 16; ++mb) {
+        f32x4 c = zero;
+#pragma unroll
+        for (int kb = 0; kb < kH / 32; ++kb) {
+            const bf16x8 a = ld_sw(sw + hW2T, 16 * mb + i, kb, i, g);
+            c = mfma16(half_lo(a), half_lo(h1f[kb]), c);
+            c = mfma16(half_hi(a), half_hi(h1f[kb]), c);
+        }
+        const f32x4 b = *reinterpret_cast<const f32x4*>(bias + kH + 16 * mb + 4 * g);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const __bf16 v = (__bf16)fmaxf(c[j] + b[j], 0.0f);
+            h2f[mb >> 1][(mb & 1) * 4 + j] = v;
+            m2 |= (bits(v) != 0 ? 1u : 0u) << (4 * mb + j);
+            if (th2) th2[t_idx(16 * mb + 4 * g + j, r)] = bits(v);
+        }
+    }
+#pragma unroll
+    for (int mb = 0; mb < kP / 16; ++mb) {
+        f32x4 c = zero;
+#pragma unroll
+        for (int kb = 0; kb < kH / 32; ++kb) c = mfma32(ld_sw(sw + hW3T, 16 * mb + i, kb, i, g), h2f[kb], c);
+        const f32x4 b = *reinterpret_cast<const f32x4*>(bias + 2 * kH + 16 * mb + 4 * g);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) pv[mb][j] = c[j] + b[j];
+    }
+}
+
+// LDS of the gradient kernel (bytes).  After the forward pass the forward
+// weights' space holds the FP32 output tile and the feature-major dP tile.
+constexpr int bB = hFwd * 2;                           // backward weights
+constexpr int bTx = hImg * 2;                          // T_x  [16][64]
+constexpr int bTh1 = bTx + kIn * 64 * 2;               // T_h1 [128][64], later T_dh1
+constexpr int bTh2 = bTh1 + kH * 64 * 2;               // T_h2 [128][64], later T_dh2
+constexpr int bBias = bTh2 + kH * 64 * 2;
+constexpr int bLoss = bBias + (2 * kH + kP) * 4;
+constexpr int bGradEnd = bLoss + kRows * 4;
+constexpr int bP = 0;                                  // [64][84] FP32 outputs, then dP
+constexpr int bTdp = bP + kRows * kPLd * 4;            // T_dp [80][64]
+static_assert(bTdp + kP * 64 * 2 <= bB, "output tiles fit in the forward weights' space");
+static_assert(bGradEnd <= 160 * 1024, "LDS");
+
+__global__ __launch_bounds__(kThreads, 1) void flow_grad_kernel(const uint16_t* __restrict__ img,
+                                                               const float* __restrict__ w, Rows rows, float scale,
+                                                               float* __restrict__ partial,
+                                                               double* __restrict__ partial_loss) {
+    __shared__ __align__(16) unsigned char sm[bGradEnd];
+    uint16_t* sw = reinterpret_cast<uint16_t*>(sm);
+    uint16_t* tx = reinterpret_cast<uint16_t*>(sm + bTx);
+    uint16_t* th1 = reinterpret_cast<uint16_t*>(sm + bTh1);
+    uint16_t* th2 = reinterpret_cast<uint16_t*>(sm + bTh2);
+    float* bias = reinterpret_cast<float*>(sm + bBias);
+    float* sloss = reinterpret_cast<float*>(sm + bLoss);
+    float* P = reinterpret_cast<float*>(sm + bP);
+    uint16_t* tdp = reinterpret_cast<uint16_t*>(sm + bTdp);
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int i = lane & 15, g = lane >> 4;
     const int r0 = wave * 16;
-    const int64_t n_tiles = (rows.n + kRows - 1) / kRows;
+    const int64_t row0 = (int64_t)blockIdx.x * kRows;
+    const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
 
-    // weights → LDS (bfloat16) and biases (FP32), once per workgroup
-    for (int e = tid; e < kIn * kH; e += kThreads) {
-        const int f = e / kH, hcol = e % kH;
-        sm.w1t[hcol][f] = f2bf(w[oW1 + e]);
-    }
-    for (int e = tid; e < kH * kH; e += kThreads) sm.w2[e / kH][e % kH] = f2bf(w[oW2 + e]);
-    for (int e = tid; e < kH * kP; e += kThreads) sm.w3[e / kP][e % kP] = f2bf(w[oW3 + e]);
-    for (int e = tid; e < kH; e += kThreads) {
-        sm.b[e] = w[ob1 + e];
-        sm.b[kH + e] = w[ob2 + e];
-    }
-    for (int e = tid; e < kP; e += kThreads) sm.b[2 * kH + e] = w[ob3 + e];
+    copy_lds(sw, img, hImg, tid);
+    load_bias(bias, w, tid);
+    const s16x4 xf = x_frag(rows, row0 + r0 + i, g, tx, r0 + i);
+    __syncthreads();
 
-    // Row tiles: the gradient launch has one tile per workgroup (one partial
-    // record each); the forward launch is persistent over the tiles.
-    for (int64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
-    const int64_t row0 = tile * kRows;
-    for (int e = tid; e < kRows * kIn; e += kThreads) {
-        const int r = e / kIn, f = e % kIn;
+    // ---- forward ------------------------------------------------------------
+    uint32_t m1, m2;
+    f32x4 pv[5];
+    forward_chain(sw, bias, xf, i, g, m1, m2, pv, th1, th2, r0 + i);
+    __syncthreads();                                   // forward weights no longer read
+#pragma unroll
+    for (int mb = 0; mb < kP / 16; ++mb) *reinterpret_cast<f32x4*>(P + (r0 + i) * kPLd + 16 * mb + 4 * g) = pv[mb];
+    wave_sync();
+
+    // ---- per-row loss and d loss / d P: four lanes per row ---------------------
+    {
+        const int s = lane & 3, r = r0 + (lane >> 2);
         const int64_t gr = row0 + r;
-        sm.x[r][f] = f2bf(gr < rows.n ? rows.x[gr * kIn + f] : 0.0f);
-    }
-    __syncthreads();
-
-    // ---- forward: H1 = relu(X W1 + b1) -------------------------------------
-    {
-        const bf16x4 a = frag_rows(sm.x, r0, 0, i, g);
-        for (int nb = 0; nb < kH / 16; ++nb) {
-            const bf16x4 b = frag_rows(sm.w1t, nb * 16, 0, i, g);
-            const f32x4 c = mfma16(a, b, f32x4{0.f, 0.f, 0.f, 0.f});
-            const int col = nb * 16 + i;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) sm.h1[r0 + 4 * g + j][col] = f2bf(fmaxf(c[j] + sm.b[col], 0.0f));
-        }
-    }
-    __syncthreads();
-    // ---- H2 = relu(H1 W2 + b2) ----------------------------------------------
-    {
-        bf16x4 a[kH / 16];
-#pragma unroll
-        for (int kb = 0; kb < kH / 16; ++kb) a[kb] = frag_rows(sm.h1, r0, kb * 16, i, g);
-        for (int nb = 0; nb < kH / 16; ++nb) {
-            f32x4 c = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int kb = 0; kb < kH / 16; ++kb) c = mfma16(a[kb], frag_cols(sm.w2, kb * 16, nb * 16, i, g), c);
-            const int col = nb * 16 + i;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) sm.h2[r0 + 4 * g + j][col] = f2bf(fmaxf(c[j] + sm.b[kH + col], 0.0f));
-        }
-    }
-    __syncthreads();
-    // ---- P = H2 W3 + b3 ------------------------------------------------------
-    {
-        bf16x4 a[kH / 16];
-#pragma unroll
-        for (int kb = 0; kb < kH / 16; ++kb) a[kb] = frag_rows(sm.h2, r0, kb * 16, i, g);
-        for (int nb = 0; nb < kP / 16; ++nb) {
-            f32x4 c = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int kb = 0; kb < kH / 16; ++kb) c = mfma16(a[kb], frag_cols(sm.w3, kb * 16, nb * 16, i, g), c);
-            const int col = nb * 16 + i;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) sm.p[r0 + 4 * g + j][col] = c[j] + sm.b[2 * kH + col];
-        }
-    }
-    __syncthreads();
-    if (params_out) {
-        for (int e = tid; e < kRows * kP; e += kThreads) {
-            const int r = e / kP, c = e % kP;
-            if (row0 + r < rows.n) params_out[(row0 + r) * kP + c] = sm.p[r][c];
-        }
-    }
-
-    // ---- per-row loss (and d loss / d P): 16 lanes per wavefront -------------
-    if (lane < 16) {
-        const int r = r0 + lane;
-        const int64_t gr = row0 + r;
-        float loss = 0.0f;
+        float t = 0.0f, cnt = 0.0f, off = 0.0f, wgt = 1.0f;
+        int flags = 0;
         if (gr < rows.n) {
-            float gp[kP];
-            const float off = rows.aux ? rows.aux[2 * gr] : 0.0f;
-            const float wgt = rows.aux ? rows.aux[2 * gr + 1] : 1.0f;
-            loss = row_loss<kGrad>(sm.p[r], rows.t[gr], rows.cnt[gr], rows.flags[gr], off, wgt, gp, scale);
-            if (kGrad) {
-#pragma unroll
-                for (int c = 0; c < kP; ++c) sm.p[r][c] = gp[c];
+            t = rows.t[gr];
+            cnt = rows.cnt[gr];
+            flags = rows.flags[gr];
+            if (rows.aux) {
+                off = rows.aux[2 * gr];
+                wgt = rows.aux[2 * gr + 1];
             }
-        } else if (kGrad) {
-#pragma unroll
-            for (int c = 0; c < kP; ++c) sm.p[r][c] = 0.0f;
         }
-        sm.loss[r] = loss;
-        if (loss_out && gr < rows.n) loss_out[gr] = loss;
+        const float loss = quad_row_loss<true>(P + r * kPLd, s, t, cnt, flags, off, wgt, scale, tdp, r);
+        if (s == 0) sloss[r] = loss;
     }
     __syncthreads();
     if (tid == 0 && partial_loss) {
-        double s = 0.0;
-        for (int r = 0; r < kRows; ++r) s += (double)sm.loss[r];
-        partial_loss[blockIdx.x] = s;
-    }
-    if (!kGrad) {
-        __syncthreads();        // the next tile overwrites the LDS tiles
-        continue;
+        double acc = 0.0;
+        for (int r = 0; r < kRows; ++r) acc += (double)sloss[r];
+        partial_loss[blockIdx.x] = acc;
     }
 
     float* part = partial + (int64_t)blockIdx.x * kPartW;
     float* pb = part + pBias + wave * kBiasW;
-    // dP in bfloat16 for the products; db3 from the FP32 values of this wave's rows
-    for (int e = lane; e < 16 * kP; e += 64) {
-        const int r = r0 + e / kP, c = e % kP;
-        sm.dp[r][c] = f2bf(sm.p[r][c]);
-    }
+    // db3 over this wave's rows from the FP32 dP
     for (int c = lane; c < kP; c += 64) {
-        float s = 0.0f;
-        for (int r = 0; r < 16; ++r) s += sm.p[r0 + r][c];
-        pb[2 * kH + c] = s;
+        float acc = 0.0f;
+        for (int rr = 0; rr < 16; ++rr) acc += P[(r0 + rr) * kPLd + c];
+        pb[2 * kH + c] = acc;
     }
-    __syncthreads();
-    // ---- dH2 = (dP W3ᵀ) ⊙ [H2 > 0] --------------------------------------------
-    {
-        bf16x4 a[kP / 16];
+    // dW3 = H2ᵀ dP over the 64 rows: this wave's h blocks 2w, 2w+1
 #pragma unroll
-        for (int kb = 0; kb < kP / 16; ++kb) a[kb] = frag_rows(sm.dp, r0, kb * 16, i, g);
-        for (int nb = 0; nb < kH / 16; ++nb) {
-            f32x4 c = {0.f, 0.f, 0.f, 0.f};
+    for (int mi = 0; mi < 2; ++mi) {
+        const int mb = 2 * wave + mi;
+        const bf16x8 a0 = ld_t(th2, 16 * mb + i, 0, i, g), a1 = ld_t(th2, 16 * mb + i, 1, i, g);
 #pragma unroll
-            for (int kb = 0; kb < kP / 16; ++kb) c = mfma16(a[kb], frag_rows(sm.w3, nb * 16, kb * 16, i, g), c);
-            const int col = nb * 16 + i;
-            float colsum = 0.0f;
+        for (int nb = 0; nb < kP / 16; ++nb) {
+            f32x4 c = mfma32(a0, ld_t(tdp, 16 * nb + i, 0, i, g), zero);
+            c = mfma32(a1, ld_t(tdp, 16 * nb + i, 1, i, g), c);
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int r = r0 + 4 * g + j;
-                const float v = sm.h2[r][col] != 0 ? c[j] : 0.0f;   // H2 ≥ 0: nonzero bits ⇔ > 0
-                sm.dh2[r][col] = f2bf(v);
-                colsum += v;
-            }
-            colsum += __shfl_xor(colsum, 16);
-            colsum += __shfl_xor(colsum, 32);
-            if (g == 0) pb[kH + col] = colsum;
+            for (int j = 0; j < 4; ++j) part[pW3 + (16 * mb + 4 * g + j) * kP + 16 * nb + i] = c[j];
         }
     }
-    __syncthreads();
-    // ---- dH1 = (dH2 W2ᵀ) ⊙ [H1 > 0] -------------------------------------------
+    // dH2ᵀ = W3 dPᵀ for this wave's rows, masked by H2 > 0; db2
+    bf16x8 dh2f[4];
     {
-        bf16x4 a[kH / 16];
+        const float* prow = P + (r0 + i) * kPLd;
+        bf16x8 dpf[2];
 #pragma unroll
-        for (int kb = 0; kb < kH / 16; ++kb) a[kb] = frag_rows(sm.dh2, r0, kb * 16, i, g);
-        for (int nb = 0; nb < kH / 16; ++nb) {
-            f32x4 c = {0.f, 0.f, 0.f, 0.f};
+        for (int kb = 0; kb < 2; ++kb) {
+            const f32x4 lo = *reinterpret_cast<const f32x4*>(prow + 32 * kb + 4 * g);
+            const f32x4 hi = *reinterpret_cast<const f32x4*>(prow + 32 * kb + 16 + 4 * g);
 #pragma unroll
-            for (int kb = 0; kb < kH / 16; ++kb) c = mfma16(a[kb], frag_rows(sm.w2, nb * 16, kb * 16, i, g), c);
-            const int col = nb * 16 + i;
-            float colsum = 0.0f;
+            for (int e = 0; e < 4; ++e) {
+                dpf[kb][e] = (__bf16)lo[e];
+                dpf[kb][4 + e] = (__bf16)hi[e];
+            }
+        }
+        const f32x4 tl = *reinterpret_cast<const f32x4*>(prow + 64 + 4 * g);
+        s16x4 dpt;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) dpt[e] = (short)bits((__bf16)tl[e]);
+#pragma unroll
+        for (int mb = 0; mb < kH / 16; ++mb) {
+            f32x4 c = mfma32(ld_w3(sw, 16 * mb + i, 0, g), dpf[0], zero);
+            c = mfma32(ld_w3(sw, 16 * mb + i, 1, g), dpf[1], c);
+            c = mfma32(widen(ld_w3_tail(sw, 16 * mb + i, g)), widen(dpt), c);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int r = r0 + 4 * g + j;
-                const float v = sm.h1[r][col] != 0 ? c[j] : 0.0f;
-                sm.dh1[r][col] = f2bf(v);
-                colsum += v;
+                const int e = (mb & 1) * 4 + j;
+                const float v = (m2 >> (4 * mb + j)) & 1u ? c[j] : 0.0f;
+                const float cs = row16_sum(v);
+                if (i == 0) pb[kH + 16 * mb + 4 * g + j] = cs;
+                dh2f[mb >> 1][e] = (__bf16)v;
             }
-            colsum += __shfl_xor(colsum, 16);
-            colsum += __shfl_xor(colsum, 32);
-            if (g == 0) pb[col] = colsum;
         }
     }
+    __syncthreads();                                   // dW3 done with T_h2
+#pragma unroll
+    for (int mb = 0; mb < kH / 16; ++mb)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) th2[t_idx(16 * mb + 4 * g + j, r0 + i)] = bits(dh2f[mb >> 1][(mb & 1) * 4 + j]);
+    // dH1ᵀ = W2 dH2ᵀ masked by H1 > 0; db1
+    bf16x8 dh1f[4];
+#pragma unroll
+    for (int mb = 0; mb < kH / 16; ++mb) {
+        f32x4 c = zero;
+#pragma unroll
+        for (int kb = 0; kb < kH / 32; ++kb) c = mfma32(ld_sw(sw + hW2, 16 * mb + i, kb, i, g), dh2f[kb], c);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int e = (mb & 1) * 4 + j;
+            const float v = (m1 >> (4 * mb + j)) & 1u ? c[j] : 0.0f;
+            const float cs = row16_sum(v);
+            if (i == 0) pb[16 * mb + 4 * g + j] = cs;
+            dh1f[mb >> 1][e] = (__bf16)v;
+        }
+    }
+    __syncthreads();                                   // T_dh2 complete
+    // dW2 = H1ᵀ dH2: this wave's h1 blocks 2w, 2w+1
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi) {
+        const int mb = 2 * wave + mi;
+        const bf16x8 a0 = ld_t(th1, 16 * mb + i, 0, i, g), a1 = ld_t(th1, 16 * mb + i, 1, i, g);
+#pragma unroll
+        for (int nb = 0; nb < kH / 16; ++nb) {
+            f32x4 c = mfma32(a0, ld_t(th2, 16 * nb + i, 0, i, g), zero);
+            c = mfma32(a1, ld_t(th2, 16 * nb + i, 1, i, g), c);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) part[pW2 + (16 * mb + 4 * g + j) * kH + 16 * nb + i] = c[j];
+        }
+    }
+    __syncthreads();                                   // dW2 done with T_h1
+#pragma unroll
+    for (int mb = 0; mb < kH / 16; ++mb)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) th1[t_idx(16 * mb + 4 * g + j, r0 + i)] = bits(dh1f[mb >> 1][(mb & 1) * 4 + j]);
     __syncthreads();
-    // ---- weight gradients over the workgroup's 64 rows (K = 64) ----------------
-    // dW3 = H2ᵀ dP: 8 × 5 tiles
-    for (int tile = wave; tile < (kH / 16) * (kP / 16); tile += 4) {
-        const int mb = tile / (kP / 16), nb = tile % (kP / 16);
-        f32x4 c = {0.f, 0.f, 0.f, 0.f};
+    // dW1 = Xᵀ dH1: this wave's h blocks 2w, 2w+1
+    {
+        const bf16x8 a0 = ld_t(tx, i, 0, i, g), a1 = ld_t(tx, i, 1, i, g);
 #pragma unroll
-        for (int ks = 0; ks < kRows / 16; ++ks)
-            c = mfma16(frag_cols(sm.h2, ks * 16, mb * 16, i, g), frag_cols(sm.dp, ks * 16, nb * 16, i, g), c);
+        for (int ni = 0; ni < 2; ++ni) {
+            const int nb = 2 * wave + ni;
+            f32x4 c = mfma32(a0, ld_t(th1, 16 * nb + i, 0, i, g), zero);
+            c = mfma32(a1, ld_t(th1, 16 * nb + i, 1, i, g), c);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) part[pW3 + (mb * 16 + 4 * g + j) * kP + nb * 16 + i] = c[j];
+            for (int j = 0; j < 4; ++j) part[pW1 + (4 * g + j) * kH + 16 * nb + i] = c[j];
+        }
     }
-    // dW2 = H1ᵀ dH2: 8 × 8 tiles
-    for (int tile = wave; tile < (kH / 16) * (kH / 16); tile += 4) {
-        const int mb = tile / (kH / 16), nb = tile % (kH / 16);
-        f32x4 c = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int ks = 0; ks < kRows / 16; ++ks)
-            c = mfma16(frag_cols(sm.h1, ks * 16, mb * 16, i, g), frag_cols(sm.dh2, ks * 16, nb * 16, i, g), c);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) part[pW2 + (mb * 16 + 4 * g + j) * kH + nb * 16 + i] = c[j];
-    }
-    // dW1 = Xᵀ dH1: 1 × 8 tiles
-    for (int nb = wave; nb < kH / 16; nb += 4) {
-        f32x4 c = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int ks = 0; ks < kRows / 16; ++ks)
-            c = mfma16(frag_cols(sm.x, ks * 16, 0, i, g), frag_cols(sm.dh1, ks * 16, nb * 16, i, g), c);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) part[pW1 + (4 * g + j) * kH + nb * 16 + i] = c[j];
-    }
+}
+
+// LDS of the forward kernel: forward weights, one 16-row output tile per
+// wavefront, biases (two workgroups per CU)
+constexpr int bfP = hFwd * 2;
+constexpr int bfBias = bfP + kRows * kPLd * 4;
+constexpr int bFwdEnd = bfBias + (2 * kH + kP) * 4;
+static_assert(2 * bFwdEnd <= 160 * 1024, "two forward workgroups per CU");
+
+__global__ __launch_bounds__(kThreads, 2) void flow_fwd_kernel(const uint16_t* __restrict__ img,
+                                                              const float* __restrict__ w, Rows rows,
+                                                              float* __restrict__ params_out,
+                                                              float* __restrict__ loss_out) {
+    __shared__ __align__(16) unsigned char sm[bFwdEnd];
+    uint16_t* sw = reinterpret_cast<uint16_t*>(sm);
+    float* bias = reinterpret_cast<float*>(sm + bfBias);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int i = lane & 15, g = lane >> 4;
+    float* P = reinterpret_cast<float*>(sm + bfP) + wave * 16 * kPLd;
+    copy_lds(sw, img, hFwd, tid);
+    load_bias(bias, w, tid);
     __syncthreads();
-    }  // tiles
+    // every wavefront walks its own 16-row tiles; no workgroup barrier below
+    const int64_t n_tiles = (rows.n + 15) / 16;
+    for (int64_t tile = (int64_t)blockIdx.x * 4 + wave; tile < n_tiles; tile += (int64_t)gridDim.x * 4) {
+        const int64_t row0 = tile * 16;
+        const s16x4 xf = x_frag(rows, row0 + i, g, nullptr, 0);
+        uint32_t m1, m2;
+        f32x4 pv[5];
+        forward_chain(sw, bias, xf, i, g, m1, m2, pv, nullptr, nullptr, 0);
+        const bool out_row = row0 + i < rows.n;
+#pragma unroll
+        for (int mb = 0; mb < kP / 16; ++mb) {
+            *reinterpret_cast<f32x4*>(P + i * kPLd + 16 * mb + 4 * g) = pv[mb];
+            if (params_out && out_row) *reinterpret_cast<f32x4*>(params_out + (row0 + i) * kP + 16 * mb + 4 * g) = pv[mb];
+        }
+        wave_sync();
+        const int s = lane & 3, rl = lane >> 2;
+        const int64_t gr = row0 + rl;
+        float t = 0.0f, cnt = 0.0f, off = 0.0f, wgt = 1.0f;
+        int flags = 0;
+        if (gr < rows.n) {
+            t = rows.t[gr];
+            cnt = rows.cnt[gr];
+            flags = rows.flags[gr];
+            if (rows.aux) {
+                off = rows.aux[2 * gr];
+                wgt = rows.aux[2 * gr + 1];
+            }
+        }
+        const float loss = quad_row_loss<false>(P + rl * kPLd, s, t, cnt, flags, off, wgt, 1.0f, nullptr, 0);
+        if (loss_out && s == 0 && gr < rows.n) loss_out[gr] = loss;
+        wave_sync();                                   // the next tile overwrites P
+    }
+}
+
+// image positions of weight q (first, and second for W2/W3); biases have none
+__device__ __forceinline__ int img_pos(int q, int& pos2) {
+    pos2 = -1;
+    if (q < ob1) {
+        const int f = q / kH, h = q % kH;
+        return hW1T + h * kIn + (((f >> 2) ^ (((h >> 3) & 1) << 1)) << 2) + (f & 3);
+    }
+    if (q >= oW2 && q < ob2) {
+        const int in = (q - oW2) / kH, o = (q - oW2) % kH;
+        pos2 = hW2 + in * kH + sw_col(o, in);
+        return hW2T + o * kH + sw_col(in, o);
+    }
+    if (q >= oW3 && q < ob3) {
+        const int h = (q - oW3) / kP, c = (q - oW3) % kP;
+        pos2 = hW3 + h * kW3Ld + (c < 64 ? 32 * (c >> 5) + qperm(c & 31) : c);
+        return hW3T + c * kH + sw_col(h, c);
+    }
+    return -1;
+}
+
+__device__ __forceinline__ void store_img(uint16_t* img, int q, float v) {
+    int p2;
+    const int p1 = img_pos(q, p2);
+    if (p1 < 0) return;
+    const uint16_t b = bits((__bf16)v);
+    img[p1] = b;
+    if (p2 >= 0) img[p2] = b;
+}
+
+__global__ __launch_bounds__(256) void pack_kernel(const float* __restrict__ w, uint16_t* __restrict__ img) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < kNW) store_img(img, q, w[q]);
 }
 
 constexpr int kSplits = 16;           // record ranges summed in parallel by the first reduction pass
+constexpr int kUnroll = 8;            // records in flight per thread
 
 // Source offset of weight q in a partial record and its number of sub-entries
 // (biases: one per wavefront).
@@ -554,7 +864,8 @@ __device__ __forceinline__ int record_source(int q, int& nsub) {
     return pBias + 2 * kH + (q - ob3);
 }
 
-// First pass: tmp[split][q] = Σ over records [split·nb/S, (split+1)·nb/S) in order.
+// First pass: tmp[split][q] = Σ over records [split·nb/S, (split+1)·nb/S) in
+// order; kUnroll record loads in flight, added in record order.
 __global__ __launch_bounds__(256) void reduce_partials_kernel(const float* __restrict__ partial, int nblocks,
                                                              float* __restrict__ tmp) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -564,7 +875,22 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(const float* __res
     const int src = record_source(q, nsub);
     const int b0 = (int)((int64_t)nblocks * split / kSplits), b1 = (int)((int64_t)nblocks * (split + 1) / kSplits);
     float acc = 0.0f;
-    for (int b = b0; b < b1; ++b) {
+    int b = b0;
+    for (; b + kUnroll <= b1; b += kUnroll) {
+        float v[kUnroll][4];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const float* rec = partial + (int64_t)(b + u) * kPartW + src;
+#pragma unroll
+            for (int s = 0; s < 4; ++s) v[u][s] = s < nsub ? rec[s * kBiasW] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u)
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+                if (s < nsub) acc += v[u][s];
+    }
+    for (; b < b1; ++b) {
         const float* rec = partial + (int64_t)b * kPartW + src;
         for (int s = 0; s < nsub; ++s) acc += rec[s * kBiasW];
     }
@@ -572,12 +898,14 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(const float* __res
 }
 
 // Second pass: gradient = Σ_split tmp (fixed order), then either the gradient
-// or the Adam update; thread 0 also sums the per-block losses.
+// or the Adam update (which also refreshes the bfloat16 image); thread 0 also
+// sums the per-block losses.
 __global__ __launch_bounds__(256) void reduce_kernel(const float* __restrict__ tmp, int nblocks,
                                                     const double* __restrict__ partial_loss, double loss_scale,
                                                     double* __restrict__ loss_out, float* __restrict__ grad_out,
                                                     float* __restrict__ w, float* __restrict__ m,
-                                                    float* __restrict__ v, float lr, float c1, float c2) {
+                                                    float* __restrict__ v, uint16_t* __restrict__ img, float lr,
+                                                    float c1, float c2) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q == 0 && loss_out) {
         double s = 0.0;
@@ -597,7 +925,9 @@ __global__ __launch_bounds__(256) void reduce_kernel(const float* __restrict__ t
     const float vq = 0.999f * v[q] + 0.001f * gsum * gsum;
     m[q] = mq;
     v[q] = vq;
-    w[q] -= lr * (mq / c1) / (sqrtf(vq / c2) + 1e-8f);
+    const float wq = w[q] - lr * (mq / c1) / (sqrtf(vq / c2) + 1e-8f);
+    w[q] = wq;
+    store_img(img, q, wq);
 }
 
 __global__ void zero_kernel(float* p, int n) {
@@ -725,7 +1055,7 @@ __global__ __launch_bounds__(256) void time_logpdf_kernel(const float* __restric
     const float* src = params + pq * kP;
 #pragma unroll
     for (int c = 0; c < kNP; ++c) p[c] = src[c];
-    out[q] = -row_loss<false>(p, t[q], 0.0f, NTF_ROW_TIME, 0.0f, 1.0f, nullptr, 1.0f);
+    out[q] = time_logpdf_row(p, t[q]);
 }
 
 }  // namespace
@@ -746,6 +1076,7 @@ struct ntf_ctx {
     double* d_chunk = nullptr;
     int64_t chunk_cap = 0;
     float* d_tmp = nullptr;              // kSplits × NTF_N_WEIGHTS first-pass sums
+    uint16_t* d_img = nullptr;           // bfloat16 weight image in the kernels' LDS layout
 };
 
 namespace {
@@ -775,9 +1106,8 @@ int32_t run_grad(ntf_ctx* c, const float* x, const float* t, const float* cnt, c
     const int64_t blocks = (n + kRows - 1) / kRows;
     const int32_t st = ensure_partials(c, blocks);
     if (st != NT_OK) return st;
-    hipLaunchKernelGGL(flow_kernel<true>, dim3((unsigned)blocks), dim3(kThreads), 0, s, c->d_w,
-                       Rows{x, t, cnt, flags, aux, n}, scale, (float*)nullptr, (float*)nullptr, c->d_partial,
-                       c->d_ploss);
+    hipLaunchKernelGGL(flow_grad_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, c->d_img, c->d_w,
+                       Rows{x, t, cnt, flags, aux, n}, scale, c->d_partial, c->d_ploss);
     return hip_status(hipGetLastError());
 }
 
@@ -799,7 +1129,9 @@ int32_t ntf_create(int32_t device, ntf_ctx** out) {
         c->n_cu = cus;
     if (hipMalloc(&c->d_w, kNW * sizeof(float)) != hipSuccess || hipMalloc(&c->d_m, kNW * sizeof(float)) != hipSuccess ||
         hipMalloc(&c->d_v, kNW * sizeof(float)) != hipSuccess ||
-        hipMalloc(&c->d_tmp, (size_t)kSplits * kNW * sizeof(float)) != hipSuccess) {
+        hipMalloc(&c->d_tmp, (size_t)kSplits * kNW * sizeof(float)) != hipSuccess ||
+        hipMalloc(&c->d_img, (size_t)hImg * sizeof(uint16_t)) != hipSuccess ||
+        hipMemset(c->d_img, 0, (size_t)hImg * sizeof(uint16_t)) != hipSuccess) {
         ntf_destroy(c);
         return NT_ERR_OOM;
     }
@@ -825,6 +1157,7 @@ void ntf_destroy(ntf_ctx* c) {
     if (c->d_ploss) (void)hipFree(c->d_ploss);
     if (c->d_chunk) (void)hipFree(c->d_chunk);
     if (c->d_tmp) (void)hipFree(c->d_tmp);
+    if (c->d_img) (void)hipFree(c->d_img);
     delete c;
 }
 
@@ -833,6 +1166,7 @@ int32_t ntf_set_weights(ntf_ctx* c, const float* weights) {
     if (hipSetDevice(c->device) != hipSuccess) return NT_ERR_DEVICE;
     if (hipMemcpy(c->d_w, weights, kNW * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) return NT_ERR_DEVICE;
     const int blocks = (kNW + 255) / 256;
+    hipLaunchKernelGGL(pack_kernel, dim3(blocks), dim3(256), 0, 0, c->d_w, c->d_img);
     hipLaunchKernelGGL(zero_kernel, dim3(blocks), dim3(256), 0, 0, c->d_m, kNW);
     hipLaunchKernelGGL(zero_kernel, dim3(blocks), dim3(256), 0, 0, c->d_v, kNW);
     c->step = 0;
@@ -851,11 +1185,10 @@ int32_t ntf_forward(ntf_ctx* c, const float* x, const float* t, const float* cnt
     if (!c || !rows_ok(x, t, cnt, flags, n)) return NT_ERR_ARG;
     if (n == 0) return NT_OK;
     if (hipSetDevice(c->device) != hipSuccess) return NT_ERR_DEVICE;
-    const int64_t tiles = (n + kRows - 1) / kRows;
-    const int64_t blocks = tiles < c->n_cu ? tiles : c->n_cu;       // one resident workgroup per CU
-    hipLaunchKernelGGL(flow_kernel<false>, dim3((unsigned)blocks), dim3(kThreads), 0, (hipStream_t)stream,
-                       c->d_w, Rows{x, t, cnt, flags, aux, n}, 1.0f, d_params, d_row_loss, (float*)nullptr,
-                       (double*)nullptr);
+    const int64_t groups = (n + 4 * 16 - 1) / (4 * 16);               // four 16-row tiles per workgroup pass
+    const int64_t blocks = groups < 2 * c->n_cu ? groups : 2 * c->n_cu; // two resident workgroups per CU
+    hipLaunchKernelGGL(flow_fwd_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, (hipStream_t)stream, c->d_img,
+                       c->d_w, Rows{x, t, cnt, flags, aux, n}, d_params, d_row_loss);
     return hip_status(hipGetLastError());
 }
 
@@ -870,7 +1203,7 @@ int32_t ntf_loss_grad(ntf_ctx* c, const float* x, const float* t, const float* c
     hipLaunchKernelGGL(reduce_partials_kernel, dim3((kNW + 255) / 256, kSplits), dim3(256), 0, s, c->d_partial,
                        (int)blocks, c->d_tmp);
     hipLaunchKernelGGL(reduce_kernel, dim3((kNW + 255) / 256), dim3(256), 0, s, c->d_tmp, (int)blocks,
-                       c->d_ploss, 1.0, d_loss, d_grad, c->d_w, c->d_m, c->d_v, 0.0f, 1.0f, 1.0f);
+                       c->d_ploss, 1.0, d_loss, d_grad, c->d_w, c->d_m, c->d_v, c->d_img, 0.0f, 1.0f, 1.0f);
     return hip_status(hipGetLastError());
 }
 
@@ -888,7 +1221,8 @@ int32_t ntf_train_step(ntf_ctx* c, const float* x, const float* t, const float* 
     hipLaunchKernelGGL(reduce_partials_kernel, dim3((kNW + 255) / 256, kSplits), dim3(256), 0, s, c->d_partial,
                        (int)blocks, c->d_tmp);
     hipLaunchKernelGGL(reduce_kernel, dim3((kNW + 255) / 256), dim3(256), 0, s, c->d_tmp, (int)blocks,
-                       c->d_ploss, 1.0 / (double)n, d_loss, (float*)nullptr, c->d_w, c->d_m, c->d_v, lr, c1, c2);
+                       c->d_ploss, 1.0 / (double)n, d_loss, (float*)nullptr, c->d_w, c->d_m, c->d_v, c->d_img, lr,
+                       c1, c2);
     return hip_status(hipGetLastError());
 }
 
