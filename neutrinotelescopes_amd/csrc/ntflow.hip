@@ -492,20 +492,28 @@ struct Rows {
     int64_t n;
 };
 
-__device__ __forceinline__ void copy_lds(uint16_t* dst, const uint16_t* __restrict__ src, int halfs, int tid) {
+// global → LDS copy of kHalfs halfwords: every load of a thread in flight at once
+template <int kHalfs>
+__device__ __forceinline__ void copy_lds(uint16_t* dst, const uint16_t* __restrict__ src, int tid) {
+    constexpr int n16 = kHalfs / 8, full = n16 / kThreads, rem = n16 % kThreads;
     const uint4* s = reinterpret_cast<const uint4*>(src);
     uint4* d = reinterpret_cast<uint4*>(dst);
-    const int n16 = halfs / 8;
-#pragma unroll 4
-    for (int e = tid; e < n16; e += kThreads) d[e] = s[e];
+    uint4 v[full], t = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int u = 0; u < full; ++u) v[u] = s[tid + u * kThreads];
+    if (rem > 0 && tid < rem) t = s[full * kThreads + tid];
+#pragma unroll
+    for (int u = 0; u < full; ++u) d[tid + u * kThreads] = v[u];
+    if (rem > 0 && tid < rem) d[full * kThreads + tid] = t;
 }
 
 __device__ __forceinline__ void load_bias(float* b, const float* __restrict__ w, int tid) {
-    for (int e = tid; e < kH; e += kThreads) {
-        b[e] = w[ob1 + e];
-        b[kH + e] = w[ob2 + e];
+    static_assert(kH <= kThreads && kP <= kThreads, "one bias per thread");
+    if (tid < kH) {
+        b[tid] = w[ob1 + tid];
+        b[kH + tid] = w[ob2 + tid];
     }
-    for (int e = tid; e < kP; e += kThreads) b[2 * kH + e] = w[ob3 + e];
+    if (tid < kP) b[2 * kH + tid] = w[ob3 + tid];
 }
 
 // Input fragment of row `row` (B operand, K = 16 features): X[row][4g .. 4g+3]
@@ -544,9 +552,9 @@ __device__ __forceinline__ void forward_chain(const uint16_t* sw, const float* b
             if (th1) th1[t_idx(16 * mb + 4 * g + j, r)] = bits(v);
         }
     }
-    // H2 is rounded to bfloat16 after a 128-deep sum: it accumulates in 16-deep
-    // steps in natural K order (the rounding of sequential FP32 accumulation),
-    // the other products use the 32-deep instruction
+    // H2 (rounded to bfloat16 after a 128-deep sum) accumulates in 16-deep
+    // steps in natural K order, the same FP32 accumulation sequence as the
+    // 16-deep kernels before; the other products use the 32-deep instruction
 #pragma unroll
     for (int mb = 0; mb < kH / 16; ++mb) {
         f32x4 c = zero;
@@ -610,7 +618,7 @@ __global__ __launch_bounds__(kThreads, 1) void flow_grad_kernel(const uint16_t* 
     const int64_t row0 = (int64_t)blockIdx.x * kRows;
     const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
 
-    copy_lds(sw, img, hImg, tid);
+    copy_lds<hImg>(sw, img, tid);
     load_bias(bias, w, tid);
     const s16x4 xf = x_frag(rows, row0 + r0 + i, g, tx, r0 + i);
     __syncthreads();
@@ -777,7 +785,7 @@ __global__ __launch_bounds__(kThreads, 2) void flow_fwd_kernel(const uint16_t* _
     const int lane = tid & 63, wave = tid >> 6;
     const int i = lane & 15, g = lane >> 4;
     float* P = reinterpret_cast<float*>(sm + bfP) + wave * 16 * kPLd;
-    copy_lds(sw, img, hFwd, tid);
+    copy_lds<hFwd>(sw, img, tid);
     load_bias(bias, w, tid);
     __syncthreads();
     // every wavefront walks its own 16-row tiles; no workgroup barrier below
