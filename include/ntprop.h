@@ -94,7 +94,7 @@ typedef struct nt_medium {           /* 48 bytes */
     float sca_len_scale;             /* multiplies the scattering length; 1 = reference */
     int32_t dispersion_mode;         /* NT_DISPERSION_* */
     int32_t max_steps;               /* scattering steps per photon; 15 = reference
-                                        (photon_prop_cuda.jl:449) */
+                                        (photon_prop_cuda.jl:449); 0..NT_MAX_STEPS */
     int32_t direction_arithmetic;    /* NT_DIR_* */
 } nt_medium;
 
@@ -147,6 +147,9 @@ enum {
 
 #define NT_MAX_PMTS 16
 #define NT_MAX_TARGETS 4096
+/* largest nt_medium.max_steps: one photon then adds at most
+   (NT_MAX_STEPS + 1)·NT_MAX_TARGETS = 2^31 to the sphere-test counters */
+#define NT_MAX_STEPS 524287
 
 typedef struct nt_target {           /* 160 bytes */
     float position[3];               /* m */

@@ -50,6 +50,7 @@ NT_TARGET_SPHERE = 0
 NT_TARGET_MULTI_PMT = 1
 NT_MAX_PMTS = 16
 NT_MAX_TARGETS = 4096
+NT_MAX_STEPS = 524287
 NT_MAX_KNOTS = 1024
 NT_NOALLOC_MAX_HITS = 1 << 17
 

@@ -449,6 +449,7 @@ __global__ __launch_bounds__(kThreads, NTP_MIN_WAVES_PER_SIMD) void propagate_ke
     int count_slot = -1;                // slot the counts below belong to
     int64_t simulated = 0;              // wave-uniform
     uint32_t tests = 0, ctests = 0;     // per lane, since the last commit
+    bool commit_due = false;            // wave-uniform: a lane's count reached 2^31
 
     const Medium& m = p.medium;
     const int64_t stride = (int64_t)gridDim.x * kThreads;
@@ -466,7 +467,7 @@ __global__ __launch_bounds__(kThreads, NTP_MIN_WAVES_PER_SIMD) void propagate_ke
         }
         const const_src_ptr S = (const_src_ptr)(const void*)kp->sources + src;
         const int slot = S->slot;
-        if (slot != count_slot) {
+        if (slot != count_slot || commit_due) {
             if (count_slot >= 0) commit_counts(count_slot, simulated, tests, ctests, lane);
             simulated = tests = ctests = 0;
             count_slot = slot;
@@ -584,6 +585,10 @@ __global__ __launch_bounds__(kThreads, NTP_MIN_WAVES_PER_SIMD) void propagate_ke
                 tests += steps_taken * (uint32_t)p.n_targets;
         }
         simulated += __popcll(__ballot(active));
+        // The per-lane test counts are 32-bit: one photon adds less than
+        // (NT_MAX_STEPS + 1)·NT_MAX_TARGETS = 2^31, so they are committed at
+        // the next photon once any lane has reached 2^31 and never wrap.
+        commit_due = __ballot(((tests | ctests) >> 31) != 0u) != 0;
 
         // wavefront compaction of this round's hits into the LDS stage
         const uint64_t mask = __ballot(hit);
@@ -1397,7 +1402,7 @@ static void refresh_sca_poly(nt_ctx* c) {
 int32_t nt_set_medium(nt_ctx* c, const nt_medium* m) {
     if (!c || !m) return NT_ERR_ARG;
     if (!(m->mean_scattering_angle > 0.0f && m->mean_scattering_angle < 1.0f) ||
-        m->max_steps < 0 || !(m->sl_fraction >= 0.0f && m->sl_fraction <= 1.0f) ||
+        m->max_steps < 0 || m->max_steps > NT_MAX_STEPS || !(m->sl_fraction >= 0.0f && m->sl_fraction <= 1.0f) ||
         (m->direction_arithmetic != NT_DIR_FP64 && m->direction_arithmetic != NT_DIR_FP32_REF) ||
         !(m->sca_len_scale > 0.0f) ||
         (m->dispersion_mode != NT_DISPERSION_REF && m->dispersion_mode != NT_DISPERSION_QUAN_FRY)) {

@@ -80,6 +80,8 @@ class WaterProperties:
             raise ValueError("dispersion_mode must be 'ref' or 'quan_fry'")
         if self.direction_arithmetic not in ("fp64", "fp32"):
             raise ValueError("direction_arithmetic must be 'fp64' or 'fp32'")
+        if not 0 <= int(self.max_steps) <= _abi.NT_MAX_STEPS:
+            raise ValueError(f"max_steps must be in 0..{_abi.NT_MAX_STEPS}")
 
     def to_abi(self) -> _abi.nt_medium:
         m = _abi.nt_medium()

@@ -43,6 +43,26 @@ def test_abi_version_and_status_strings():
         assert lib.nt_status_string(st)
 
 
+def test_limits_match_header():
+    text = (ROOT / "include" / "ntprop.h").read_text()
+    for name in ("NT_MAX_PMTS", "NT_MAX_TARGETS", "NT_MAX_STEPS"):
+        m = re.search(rf"#define {name} (\d+)", text)
+        assert int(m.group(1)) == getattr(_abi, name), name
+    # one photon's sphere-test count stays below 2^31 (the kernel's 32-bit per-lane counters)
+    assert (_abi.NT_MAX_STEPS + 1) * _abi.NT_MAX_TARGETS <= 2 ** 31
+
+
+def test_max_steps_bounded():
+    import dataclasses
+    import neutrinotelescopes_amd as nt
+    med = nt.make_cascadia_medium_properties(0.95)
+    assert dataclasses.replace(med, max_steps=_abi.NT_MAX_STEPS).to_abi().max_steps == _abi.NT_MAX_STEPS
+    with pytest.raises(ValueError):
+        dataclasses.replace(med, max_steps=_abi.NT_MAX_STEPS + 1)
+    with pytest.raises(ValueError):
+        dataclasses.replace(med, max_steps=-1)
+
+
 def test_flow_constants_match_header():
     text = (ROOT / "include" / "ntflow.h").read_text()
     for name in ("NTF_N_IN", "NTF_HIDDEN", "NTF_BINS", "NTF_LAYERS", "NTF_P_PAD"):
