@@ -21,4 +21,6 @@ from .propagation import (PhotonPropSetup, PhotonHits, PropagationEngine, propag
                           run_photon_prop_no_local_cache, calc_total_weight, calc_time_residual,
                           make_hits_from_photons, get_engine)
 
+from . import refflow  # noqa: E402  (the reference flow's input layout and model file)
+
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -281,6 +281,17 @@ def loss_and_grad(x, t, n, flags, w, n_bins, n_layers, use_bf16=True, offset=0.0
     return float(np.mean(loss)), g, p
 
 
+def ref_mlp_forward(cond, w1, b1, w2, b2):
+    """The embedding of the reference's saved RQNormFlow (extended_cascade_model.jl:
+    71-97; with hidden_structure [768, 768] the loop at :77 adds no second hidden
+    layer): Dense(24 → H, relu), Dropout (inactive at inference), Dense(H → 3K + 4),
+    evaluated in FP64 as `m.embedding(Float64.(cond))` asks (:109).
+    cond: [N, 24]; w1: [H, 24], w2: [3K + 4, H] (Flux's [out, in] layout)."""
+    x = np.asarray(cond, dtype=np.float64)
+    h = np.maximum(x @ np.asarray(w1, np.float64).T + np.asarray(b1, np.float64), 0.0)
+    return h @ np.asarray(w2, np.float64).T + np.asarray(b2, np.float64)
+
+
 def adam_update(w, g, m, v, step, lr, beta1=0.9, beta2=0.999, eps=1e-8):
     """Flux.Optimise.Adam (the reference's optimiser, extended_cascade_model.jl:203)
     with bias correction; step counts from 1."""
