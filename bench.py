@@ -59,6 +59,12 @@ WORKLOAD_NAMES = {"cascade": "cherenkov_cascade_1e4GeV_1e7ph_20x16pmt_string",
 
 
 PROFILE = ROOT / "profiles" / "propagate_kernel_cascade_1e7ph.md"
+_T0 = time.perf_counter()
+
+
+def progress(msg: str) -> None:
+    """One line on stderr when a part of the run ends (stdout carries only the JSON result)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
 
 
 def profiled_traffic(args):
@@ -272,9 +278,11 @@ def other_legs(args) -> dict:
     r = measure_kernel("cascade", args.photons, "fp32", args.steps, args.warmup)
     legs["cascade_fp32_reference_arithmetic"] = {"value": r["value"], "kernel_ms": r["kernel_ms"],
                                                  "frac": r["roofline"]["frac"]}
+    progress("leg cascade_fp32_reference_arithmetic")
     r = measure_kernel("cascade", args.photons, args.direction_arithmetic, args.steps, args.warmup, epilogue=True)
     legs["cascade_with_hit_epilogue"] = {"value": r["value"], "ms_per_step": r["ms_per_step"],
                                          "hits_per_step": r["config"]["hits_per_step"]}
+    progress("leg cascade_with_hit_epilogue")
     r = measure_kernel("cascade", args.photons, args.direction_arithmetic, args.steps, args.warmup, epilogue=True,
                        config_mode=True)
     legs["cascade_config_mode_41deg_angular_acceptance"] = {"value": r["value"], "ms_per_step": r["ms_per_step"],
@@ -284,15 +292,18 @@ def other_legs(args) -> dict:
         legs[f"isotropic_1e9ph_r021_{arith}"] = {"value": r["value"], "kernel_ms": r["kernel_ms"],
                                                 "frac": r["roofline"]["frac"],
                                                 "hits_per_step": r["config"]["hits_per_step"]}
+        progress(f"leg isotropic_1e9ph_r021_{arith}")
     return legs
 
 
-def serving_benchmark(duration_s: float = 15.0, seed: int = 11, round_photons: int = 10_000_000,
-                      max_inflight_rounds: int = 2):
+def serving_benchmark(mode: str = "worklist", rate_hz: float | None = None, duration_s: float = 15.0,
+                      seed: int = 11, round_photons: int = 10_000_000, max_inflight_rounds: int = 2):
     """Serving configuration: Poisson stream of cascade requests (position
     uniform in a 300 m cylinder, isotropic direction, E log-uniform 1e2-1e6 GeV,
     1e4·E/100 GeV photons) against a 7-string × 20-module cluster, arrival
-    rate 0.7 × the server's measured capacity."""
+    rate `rate_hz`, by default 0.7 × the server's measured capacity.  `mode`:
+    "worklist" (one persistent kernel claims photon tickets of published
+    requests) or "rounds" (requests packed into launches of `round_photons`)."""
     import neutrinotelescopes_amd as nt
     from neutrinotelescopes_amd.propagation import PropagationEngine
     from neutrinotelescopes_amd.serving import PhotonServer, cascade_requests, cluster_targets, run_poisson_stream
@@ -304,7 +315,7 @@ def serving_benchmark(duration_s: float = 15.0, seed: int = 11, round_photons: i
     rng = np.random.default_rng(seed)
     mean_photons = 1e4 * (1e6 - 1e2) / math.log(1e4) / 100.0
     with PhotonServer(eng, device_budget=32 << 30, host_budget=8 << 30, round_photons=round_photons,
-                      max_inflight_rounds=max_inflight_rounds) as srv:
+                      max_inflight_rounds=max_inflight_rounds, mode=mode) as srv:
         # capacity: a burst submitted at once (256 requests, ~3e9 photons, so
         # that the partly filled first and last rounds weigh little)
         burst = cascade_requests(256, med, rng)
@@ -313,17 +324,19 @@ def serving_benchmark(duration_s: float = 15.0, seed: int = 11, round_photons: i
         for i in ids:
             srv.wait(i, raise_on_overflow=False)
         cap_rate = sum(s.photons for r in burst for s in r.sources) / (time.perf_counter() - t0)
-        rate = 0.7 * cap_rate / mean_photons
+        progress(f"serving ({mode}) capacity burst: {cap_rate:.4g} photons/s")
+        rate = rate_hz if rate_hz is not None else 0.7 * cap_rate / mean_photons
         n = max(50, int(rate * duration_s))
         reqs = cascade_requests(n, med, rng)
         res = run_poisson_stream(srv, reqs, rate_hz=rate, rng=rng)
         st = srv.stats()
     eng.close()
-    return {"config": "poisson_cascades_1e2-1e6GeV_7x20_cluster", "requests": res["requests"],
+    return {"config": "poisson_cascades_1e2-1e6GeV_7x20_cluster", "mode": mode, "requests": res["requests"],
             "arrival_rate_hz": rate, "p50_ms": res["p50_ms"], "p99_ms": res["p99_ms"],
             "photons_per_s": res["photons_per_s"], "capacity_photons_per_s": cap_rate,
-            "rounds": st["rounds"], "round_photons": round_photons, "max_inflight_rounds": max_inflight_rounds,
-            "device_busy_fraction": st["busy_ms"] / 1e3 / res["wall_s"]}
+            "launches": st["rounds"], "device_busy_fraction": st["busy_ms"] / 1e3 / res["wall_s"],
+            **({"round_photons": round_photons, "max_inflight_rounds": max_inflight_rounds}
+               if mode == "rounds" else {})}
 
 
 def surrogate_benchmark():
@@ -435,7 +448,9 @@ def main():
 
     import torch
     if args.serving_only:
-        print(json.dumps({"serving": serving_benchmark()}), flush=True)
+        srv = serving_benchmark()
+        print(json.dumps({"serving": srv, "serving_rounds": serving_benchmark("rounds", srv["arrival_rate_hz"])}),
+              flush=True)
         return
     if args.surrogate_only:
         print(json.dumps({"surrogate": surrogate_benchmark()}), flush=True)
@@ -454,6 +469,7 @@ def main():
     leg = measure_kernel(args.workload, args.photons, args.direction_arithmetic, args.steps, args.warmup,
                          culling=not args.no_culling)
     value = leg["value"]
+    progress(f"headline {WORKLOAD_NAMES[args.workload]}: {value:.4g} photons/s")
     out = {
         "metric": "photons propagated/s (1 GPU)",
         "value": value,
@@ -472,10 +488,14 @@ def main():
     }
     if args.workload == "cascade" and not args.no_legs:
         out["legs"] = other_legs(args)
+        progress("legs done")
     if not args.no_serving and args.workload == "cascade":
         out["serving"] = serving_benchmark()
+        out["serving_rounds"] = serving_benchmark("rounds", out["serving"]["arrival_rate_hz"])
+        progress("serving done")
     if not args.no_surrogate and args.workload == "cascade":
         out["surrogate"] = surrogate_benchmark()
+        progress("surrogate done")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload)
         out["gpu_vs_cpu"] = value / out["cpu_baseline"]["value"]

@@ -331,35 +331,58 @@ int32_t nt_set_epilogue(nt_ctx* ctx, const nt_epilogue* epilogue);
    A server runs a scheduler thread over the context's medium, spectrum and
    targets (set them before nt_server_create and leave them unchanged while
    it runs).  Requests (sources + seed + hit capacity) are admitted in
-   submission order as soon as their hit buffers fit the device and pinned
-   host budgets; every round packs photon chunks of all admitted requests
-   into one launch of the propagation kernel (at most round_photons photons,
-   shared between requests), so requests that arrive while others run join
-   the next round.  A request's photons use Philox key = its seed, counter
-   source word = its source index and photon index = index inside the
-   source, exactly as nt_propagate with that seed: a result is the same
-   alone and in any mix.  Results are copied to pinned memory on a second
-   stream; latency is measured from submission to the end of that copy. */
+   submission order as soon as their hit buffers fit the budgets.
+   NT_SERVER_WORKLIST: admitted requests are published to a work list that
+   one persistent kernel instance drains; its wavefronts claim photon
+   tickets (the request's photons over the number of wavefronts, 64..4096,
+   sources of at most 2^21 photons before larger ones), so a request that
+   arrives while others run is picked up by the next free wavefront, and
+   write hits straight into the request's pinned host buffer (only the host
+   budget is used).  The instance leaves after 2 ms without work and is
+   launched again for the next request.  NT_SERVER_ROUNDS: every round
+   packs photon chunks of all admitted requests into one launch of the
+   propagation kernel (at most round_photons photons, shared between
+   requests), so requests that arrive while others run join the next round;
+   hits are copied to pinned memory on a second stream.  In both modes a
+   request's photons use Philox key = its seed, counter source word = its
+   source index and photon index = index inside the source, exactly as
+   nt_propagate with that seed: a result is the same alone and in any mix.
+   Latency is measured from submission until the result is in host memory. */
 typedef struct nt_server nt_server;
+
+enum nt_server_mode {
+    NT_SERVER_ROUNDS = 0,            /* host-scheduled rounds, one launch each */
+    NT_SERVER_WORKLIST = 1           /* persistent kernel pulling from a device work list */
+};
 
 typedef struct nt_server_config {
     uint64_t device_budget_bytes;    /* HBM for request hit buffers */
     uint64_t host_budget_bytes;      /* pinned host memory for results */
-    int64_t round_photons;           /* photons per batched launch */
-    int32_t max_inflight_rounds;     /* launches queued ahead of the device (>= 1) */
-    int32_t reserved;
+    int64_t round_photons;           /* NT_SERVER_ROUNDS: photons per batched launch */
+    int32_t max_inflight_rounds;     /* NT_SERVER_ROUNDS: launches queued ahead of the device (1..16) */
+    int32_t mode;                    /* nt_server_mode */
 } nt_server_config;
 
 typedef struct nt_server_stats {
-    int64_t submitted, completed, rounds, photons;
+    int64_t submitted, completed, rounds, photons;   /* rounds: launches (work list: kernel instances) */
     int64_t queued, active;          /* requests waiting for memory / admitted */
-    double busy_ms;                  /* sum of round durations (device time) */
+    double busy_ms;                  /* device time spent propagating: Σ round durations, or (work list)
+                                        Σ ticket durations of all wavefronts ÷ wavefronts per instance */
 } nt_server_stats;
 
 int32_t nt_server_create(nt_ctx* ctx, const nt_server_config* cfg, nt_server** out);
 /* Thread-safe.  NT_ERR_BUDGET if the request can never fit the budgets. */
 int32_t nt_server_submit(nt_server* srv, const nt_source* sources, int32_t n_sources, uint64_t seed,
                          int64_t hit_capacity, uint64_t* request_id);
+/* As nt_server_submit, and the served hits also get the epilogue columns
+   (pmt_id, total_weight, tres; see nt_epilogue) with the time residual
+   measured from ref_position[3] instead of the context's epilogue source
+   position.  NT_SERVER_WORKLIST only, after nt_set_epilogue; the epilogue
+   settings must stay unchanged while the server runs.  No reference
+   counterpart (calc_total_weight!/calc_time_residual!, photon_prop_cuda.jl:
+   850-917, applied per request). */
+int32_t nt_server_submit_epilogue(nt_server* srv, const nt_source* sources, int32_t n_sources, uint64_t seed,
+                                  int64_t hit_capacity, const float* ref_position, uint64_t* request_id);
 /* NT_OK when the request is complete, NT_ERR_NOT_READY otherwise. */
 int32_t nt_server_poll(nt_server* srv, uint64_t request_id);
 /* Blocks until the request is complete, copies its hits to host columns

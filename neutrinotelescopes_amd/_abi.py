@@ -179,8 +179,12 @@ class nt_server_config(C.Structure):
         ("host_budget_bytes", C.c_uint64),
         ("round_photons", C.c_int64),
         ("max_inflight_rounds", C.c_int32),
-        ("reserved", C.c_int32),
+        ("mode", C.c_int32),
     ]
+
+
+NT_SERVER_ROUNDS = 0
+NT_SERVER_WORKLIST = 1
 
 
 class nt_server_stats(C.Structure):
@@ -215,6 +219,8 @@ SIGNATURES = {
     "nt_server_create": (C.c_int32, [C.c_void_p, C.POINTER(nt_server_config), C.POINTER(C.c_void_p)]),
     "nt_server_submit": (C.c_int32, [C.c_void_p, C.POINTER(nt_source), C.c_int32, C.c_uint64, C.c_int64,
                                      C.POINTER(C.c_uint64)]),
+    "nt_server_submit_epilogue": (C.c_int32, [C.c_void_p, C.POINTER(nt_source), C.c_int32, C.c_uint64, C.c_int64,
+                                              C.POINTER(C.c_float), C.POINTER(C.c_uint64)]),
     "nt_server_poll": (C.c_int32, [C.c_void_p, C.c_uint64]),
     "nt_server_wait": (C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(nt_hits), C.POINTER(C.c_int64),
                                    C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_double)]),

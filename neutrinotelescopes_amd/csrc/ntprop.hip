@@ -60,6 +60,7 @@ struct DevSource {                     // 96 bytes
     float gamma_d, gamma_c, gamma_scale, lrad;
 };
 static_assert(sizeof(DevSource) == 96, "DevSource layout");
+constexpr int kSrcWords = sizeof(DevSource) / 4;
 
 struct HitCols {
     float* pos;
@@ -79,7 +80,7 @@ struct HitCols {
 
 // Output of one request: hit columns and counters
 // ([0] hits found, [1] photons simulated, [2] sphere tests, [3] string-bound tests).
-struct SlotDesc {
+struct alignas(128) SlotDesc {
     HitCols cols;
     int64_t* counters;
 };
@@ -90,6 +91,7 @@ struct EpiTarget {
     double area_acc;                   // area acceptance (detection.jl:139-146)
     double tgeo;                       // geometric arrival time for tres (ns)
     double cos_open;                   // cos of the PMT opening half-angle
+    float radius;                      // m (time residual from another reference position)
     int32_t kind;
     int32_t n_pmts;
     double axes[NT_MAX_PMTS][3];       // PMT axes after the orientation
@@ -106,6 +108,7 @@ struct Epilogue {
     int n_abs, n_acc, n_ang;
     float abs_len_scale;
     int enabled;
+    double c_med;                      // speed of light in the medium at 800 nm (m/ns), for tgeo
 };
 
 // Target clusters.  Targets sharing (x, y) form a vertical string; a photon
@@ -311,9 +314,12 @@ __device__ __forceinline__ T interp_table(T x, const T* xs, const T* ys, int n) 
     return ys[a] + (ys[b] - ys[a]) * along;
 }
 
-// PMT assignment, weight and time residual of one hit (see nt_epilogue).
+// PMT assignment, weight and time residual of one hit (see nt_epilogue);
+// `ref` (served requests): the time residual's reference position instead of
+// the context's, with the same FP32 distance as the host's tgeo.
 __device__ __forceinline__ void hit_epilogue(const Epilogue& ep, int ti, float3 pos, float3 dir, float wl,
-                                          float dist, float time, uint8_t& pmt, float& weight, float& tres) {
+                                          float dist, float time, uint8_t& pmt, float& weight, float& tres,
+                                          const float* ref = nullptr) {
     const EpiTarget* T = ep.targets + ti;
     // check_pmt_hit, detection.jl:88-104 (FP64 as in the reference)
     if (T->kind == NT_TARGET_SPHERE) {
@@ -343,7 +349,13 @@ __device__ __forceinline__ void hit_epilogue(const Epilogue& ep, int ti, float3 
     }
     weight = (float)(T->area_acc * wl_acc * abs_w * ang_acc);
     // calc_time_residual!, photon_prop_cuda.jl:906-917
-    tres = (float)((double)time - T->tgeo);
+    double tgeo = T->tgeo;
+    if (ref) {
+        const float dx = ref[0] - (float)T->pos[0], dy = ref[1] - (float)T->pos[1], dz = ref[2] - (float)T->pos[2];
+        const float d = sqrtf((dx * dx + dy * dy) + dz * dz);
+        tgeo = (double)(d - T->radius) / ep.c_med;
+    }
+    tres = (float)((double)time - tgeo);
 }
 
 // Output columns and counters of a slot.
@@ -353,6 +365,7 @@ __device__ __forceinline__ HitCols slot_cols(const_kparams_ptr kp, int slot, int
         h.pos = kp->hits.pos; h.dir = kp->hits.dir; h.idir = kp->hits.idir; h.wl = kp->hits.wl;
         h.time = kp->hits.time; h.dist = kp->hits.dist; h.module_id = kp->hits.module_id;
         h.source_index = kp->hits.source_index; h.photon_id = kp->hits.photon_id;
+        h.pmt_id = kp->hits.pmt_id; h.total_weight = kp->hits.total_weight; h.tres = kp->hits.tres;
         h.capacity = kp->hits.capacity;
         counters = kp->counters;
     } else {
@@ -360,6 +373,7 @@ __device__ __forceinline__ HitCols slot_cols(const_kparams_ptr kp, int slot, int
         h.pos = sd->cols.pos; h.dir = sd->cols.dir; h.idir = sd->cols.idir; h.wl = sd->cols.wl;
         h.time = sd->cols.time; h.dist = sd->cols.dist; h.module_id = sd->cols.module_id;
         h.source_index = sd->cols.source_index; h.photon_id = sd->cols.photon_id;
+        h.pmt_id = sd->cols.pmt_id; h.total_weight = sd->cols.total_weight; h.tres = sd->cols.tres;
         h.capacity = sd->cols.capacity;
         counters = sd->counters;
     }
@@ -424,6 +438,164 @@ __device__ __forceinline__ void flush_stage(const uint32_t* __restrict__ st, int
 #ifndef NTP_MIN_WAVES_PER_SIMD
 #define NTP_MIN_WAVES_PER_SIMD 6
 #endif
+#ifndef NTP_SERVE_WAVES_PER_SIMD
+#define NTP_SERVE_WAVES_PER_SIMD 6
+#endif
+
+// State of one photon at the end of its propagation (the hit columns when `hit`).
+struct PhotonOut {
+    float3 pos, dir, idir;
+    float wl, time, dist;
+    uint32_t mid;
+    uint32_t steps;                    // steps that tested the targets (the hit step included)
+    uint32_t tests;                    // sphere tests behind the string bounds
+    bool hit;
+};
+
+// One photon of source S from initialisation to its hit or its last step
+// (photon_prop_cuda.jl:133-188, 435-506).
+template <int KIND, int DIRM, class SrcPtr>
+__device__ __forceinline__ void simulate_photon(const KParams& p, const_kparams_ptr kp, SrcPtr S,
+                                                const float* s_knots, const double* s_poly, uint64_t pid,
+                                                uint32_t sidx, PhotonOut& o) {
+    uint32_t tests = 0;
+    const Medium& m = p.medium;
+    const Medium mi = load_medium(kp);      // initialisation-only fields
+    const Rng rng{p.seed_lo ^ S->key_lo, p.seed_hi ^ S->key_hi, (uint32_t)pid, (uint32_t)(pid >> 32), sidx};
+    float3 pos, dir, idir;
+    float wl, time, dist = 0.0f;
+    uint32_t mid = 0;
+    bool hit = false;
+
+    // initialize_photon_state, photon_prop_cuda.jl:133-188
+    const uint4 r0 = rng.block(0u);
+    wl = sample_spectrum(load_spectrum(kp), s_knots, u01(r0.x));
+    pos = make_float3(S->pos[0], S->pos[1], S->pos[2]);
+    const float3 sdir = make_float3(S->dir[0], S->dir[1], S->dir[2]);
+    time = S->t0;
+    switch (KIND < 0 ? S->kind : KIND) {
+        case NT_SRC_POINTLIKE_ISOTROPIC:
+            dir = iso_dir(u01(r0.y), u01(r0.z));
+            break;
+        case NT_SRC_POINTLIKE_TIME_RANGE:
+            dir = iso_dir(u01(r0.y), u01(r0.z));
+            time = S->t0 + u01(r0.w) * (S->t1 - S->t0);
+            break;
+        case NT_SRC_AXICONE:
+            dir = rot_from_ez_mode(sdir, sph_to_cart(S->angle, 0.0f + u01(r0.z) * kTwoPi), DIRM);
+            break;
+        case NT_SRC_PENCIL:
+            dir = sdir;
+            break;
+        case NT_SRC_EXTENDED_CHERENKOV: {
+            const float lp = rand_gamma(S->gamma_d, S->gamma_c, S->gamma_scale, rng, nullptr) * S->lrad;
+            pos = make_float3(pos.x + lp * sdir.x, pos.y + lp * sdir.y, pos.z + lp * sdir.z);
+            time = S->t0 + lp / kCVacMNs;
+            dir = DIRM == NT_DIR_FP32_REF ? cherenkov_dir32(wl, u01(r0.y), u01(r0.z), u01(r0.w), sdir, mi)
+                                          : cherenkov_dir(wl, u01(r0.y), u01(r0.z), u01(r0.w), sdir, mi);
+            break;
+        }
+        default:  // NT_SRC_POINTLIKE_CHERENKOV
+            dir = DIRM == NT_DIR_FP32_REF ? cherenkov_dir32(wl, u01(r0.y), u01(r0.z), u01(r0.w), sdir, mi)
+                                          : cherenkov_dir(wl, u01(r0.y), u01(r0.z), u01(r0.w), sdir, mi);
+            break;
+    }
+    idir = dir;
+    const float sca_len = scattering_length_fit(wl, &kp->sca_poly, s_poly, mi);
+    const float c_grp = group_velocity(wl, mi);
+
+    // Scattering steps read the Philox stream of blocks 1, 2, ... three
+    // words per step (four steps per three blocks): step, HG/SL uniform
+    // and azimuth from the words' top 24 bits, the mixture uniform from
+    // their low bytes.  The loop is unrolled by four so that every
+    // step's words are fixed registers; the block index is wave-uniform.
+    // A step returns true when the photon hit a target (state at the hit).
+    // −ln u · L_sca = log2 u · (−ln 2 · L_sca): the hardware log2 of u ∈ (0, 1]
+    // times a per-photon constant (≤ 2 ulp, like −ln u · L_sca itself)
+    const float step_scale = -0.693147182f * sca_len;
+    auto scatter_step = [&](uint32_t wx, uint32_t wy, uint32_t wz) -> bool {
+        const float step = __builtin_amdgcn_logf(u01_oc(wx)) * step_scale;
+        // first listed target hit within this step (photon_prop_cuda.jl:457-468)
+        float dt = 0.0f;
+        const int hit_t = first_hit(p, pos, dir, step, dt, tests);
+        if (hit_t < 0) {
+            pos = make_float3(pos.x + dir.x * step, pos.y + dir.y * step, pos.z + dir.z * step);
+            dist += step;
+            // the mixture uniform matters only with a Simplified-Liu share
+            // (a wave-uniform medium field): with none, u = 1 never selects it
+            float u_mix = 1.0f;
+            if (m.sl_fraction > 0.0f)
+                u_mix = u01(((wx & 0xFFu) << 8) | ((wy & 0xFFu) << 16) | ((wz & 0xFFu) << 24));
+            if (DIRM == NT_DIR_FP32_REF) {
+                dir = update_direction32(dir, u01(wy), u01(wz), u_mix, m);
+            } else {
+                const const_kparams_ptr kh = kargs();
+                const HgConst hg{kh->medium.hg.omg, kh->medium.hg.twog, kh->medium.hg.omg2, kh->medium.hg.k0,
+                                 kh->medium.hg.inv2g};
+                dir = update_direction(dir, u01(wy), u01(wz), u_mix, m, hg);
+            }
+            return false;
+        }
+        pos = make_float3(pos.x + dir.x * dt, pos.y + dir.y * dt, pos.z + dir.z * dt);
+        dist += dt;
+        // Σ step/c_grp + dt/c_grp of photon_prop_cuda.jl:473,481 taken at
+        // once: the same sum up to FP32 rounding of the partial sums
+        time += dist / c_grp;
+        mid = p.module_ids[hit_t];
+        hit = true;
+        return true;
+    };
+    const int max_steps = m.max_steps;
+    PhiloxRun run;
+    run.init(rng.k0, rng.k1, rng.plo, rng.phi, rng.src);
+    int k = 0;
+    for (uint32_t blk = 1u; k < max_steps; blk += 3u) {
+        const uint4 b0 = run.block(blk);
+        if (scatter_step(b0.x, b0.y, b0.z) || ++k >= max_steps) break;
+        const uint4 b1 = run.block(blk + 1u);
+        if (scatter_step(b0.w, b1.x, b1.y) || ++k >= max_steps) break;
+        const uint4 b2 = run.block(blk + 2u);
+        if (scatter_step(b1.z, b1.w, b2.x) || ++k >= max_steps) break;
+        if (scatter_step(b2.y, b2.z, b2.w) || ++k >= max_steps) break;
+    }
+    // every step tested all strings (or all targets): k steps without
+    // a hit, one more if the photon hit
+    o.steps = (uint32_t)k + (hit ? 1u : 0u);
+    o.tests = tests;
+    o.pos = pos;
+    o.dir = dir;
+    o.idir = idir;
+    o.wl = wl;
+    o.time = time;
+    o.dist = dist;
+    o.mid = mid;
+    o.hit = hit;
+}
+
+// Write this lane's hit at its compacted position (mask: the wavefront's
+// hits, staged: hits already in the stage).
+__device__ __forceinline__ void stage_write(uint32_t* st, int staged, uint64_t mask, const PhotonOut& o,
+                                            uint32_t sidx, uint64_t pid) {
+    const int at = staged + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+    float* f = reinterpret_cast<float*>(st);
+    f[0 * kStageSlots + at] = o.pos.x;
+    f[1 * kStageSlots + at] = o.pos.y;
+    f[2 * kStageSlots + at] = o.pos.z;
+    f[3 * kStageSlots + at] = o.dir.x;
+    f[4 * kStageSlots + at] = o.dir.y;
+    f[5 * kStageSlots + at] = o.dir.z;
+    f[6 * kStageSlots + at] = o.idir.x;
+    f[7 * kStageSlots + at] = o.idir.y;
+    f[8 * kStageSlots + at] = o.idir.z;
+    f[9 * kStageSlots + at] = o.wl;
+    f[10 * kStageSlots + at] = o.time;
+    f[11 * kStageSlots + at] = o.dist;
+    st[12 * kStageSlots + at] = o.mid;
+    st[13 * kStageSlots + at] = sidx;
+    st[14 * kStageSlots + at] = (uint32_t)pid;
+    st[15 * kStageSlots + at] = (uint32_t)(pid >> 32);
+}
 
 // KIND >= 0: every source of the launch has that kind (smaller code, fewer
 // registers); KIND < 0: mixed kinds, dispatched per photon.  DIRM: the
@@ -451,7 +623,6 @@ __global__ __launch_bounds__(kThreads, NTP_MIN_WAVES_PER_SIMD) void propagate_ke
     uint32_t tests = 0, ctests = 0;     // per lane, since the last commit
     bool commit_due = false;            // wave-uniform: a lane's count reached 2^31
 
-    const Medium& m = p.medium;
     const int64_t stride = (int64_t)gridDim.x * kThreads;
     int src = 0;
     int64_t src_lo = 0, src_hi = 0;     // global index range of `src`
@@ -474,124 +645,27 @@ __global__ __launch_bounds__(kThreads, NTP_MIN_WAVES_PER_SIMD) void propagate_ke
         }
         const int64_t pid_local = base - src_lo + lane;
         const bool active = pid_local < S->photons;
-        bool hit = false;
-        float3 pos, dir, idir;
-        float wl = 0.f, time = 0.f, dist = 0.f;
-        uint32_t mid = 0;
         const uint32_t sidx = S->rng_index;
         const uint64_t pid = S->photon_base + (uint64_t)pid_local;
+        PhotonOut o;
+        o.wl = o.time = o.dist = 0.0f;
+        o.mid = o.steps = o.tests = 0;
+        o.hit = false;
         if (active) {
-            const Medium mi = load_medium(kp);      // initialisation-only fields
-            const Rng rng{p.seed_lo ^ S->key_lo, p.seed_hi ^ S->key_hi, (uint32_t)pid, (uint32_t)(pid >> 32),
-                          sidx};
-
-            // initialize_photon_state, photon_prop_cuda.jl:133-188
-            const uint4 r0 = rng.block(0u);
-            wl = sample_spectrum(load_spectrum(kp), s_knots, u01(r0.x));
-            pos = make_float3(S->pos[0], S->pos[1], S->pos[2]);
-            const float3 sdir = make_float3(S->dir[0], S->dir[1], S->dir[2]);
-            time = S->t0;
-            switch (KIND < 0 ? S->kind : KIND) {
-                case NT_SRC_POINTLIKE_ISOTROPIC:
-                    dir = iso_dir(u01(r0.y), u01(r0.z));
-                    break;
-                case NT_SRC_POINTLIKE_TIME_RANGE:
-                    dir = iso_dir(u01(r0.y), u01(r0.z));
-                    time = S->t0 + u01(r0.w) * (S->t1 - S->t0);
-                    break;
-                case NT_SRC_AXICONE:
-                    dir = rot_from_ez_mode(sdir, sph_to_cart(S->angle, 0.0f + u01(r0.z) * kTwoPi), DIRM);
-                    break;
-                case NT_SRC_PENCIL:
-                    dir = sdir;
-                    break;
-                case NT_SRC_EXTENDED_CHERENKOV: {
-                    const float lp = rand_gamma(S->gamma_d, S->gamma_c, S->gamma_scale, rng, nullptr) * S->lrad;
-                    pos = make_float3(pos.x + lp * sdir.x, pos.y + lp * sdir.y, pos.z + lp * sdir.z);
-                    time = S->t0 + lp / kCVacMNs;
-                    dir = DIRM == NT_DIR_FP32_REF ? cherenkov_dir32(wl, u01(r0.y), u01(r0.z), u01(r0.w), sdir, mi)
-                                                  : cherenkov_dir(wl, u01(r0.y), u01(r0.z), u01(r0.w), sdir, mi);
-                    break;
-                }
-                default:  // NT_SRC_POINTLIKE_CHERENKOV
-                    dir = DIRM == NT_DIR_FP32_REF ? cherenkov_dir32(wl, u01(r0.y), u01(r0.z), u01(r0.w), sdir, mi)
-                                                  : cherenkov_dir(wl, u01(r0.y), u01(r0.z), u01(r0.w), sdir, mi);
-                    break;
-            }
-            idir = dir;
-            const float sca_len = scattering_length_fit(wl, &kp->sca_poly, s_poly, mi);
-            const float c_grp = group_velocity(wl, mi);
-
-            // Scattering steps read the Philox stream of blocks 1, 2, ... three
-            // words per step (four steps per three blocks): step, HG/SL uniform
-            // and azimuth from the words' top 24 bits, the mixture uniform from
-            // their low bytes.  The loop is unrolled by four so that every
-            // step's words are fixed registers; the block index is wave-uniform.
-            // A step returns true when the photon hit a target (state at the hit).
-            // −ln u · L_sca = log2 u · (−ln 2 · L_sca): the hardware log2 of u ∈ (0, 1]
-            // times a per-photon constant (≤ 2 ulp, like −ln u · L_sca itself)
-            const float step_scale = -0.693147182f * sca_len;
-            auto scatter_step = [&](uint32_t wx, uint32_t wy, uint32_t wz) -> bool {
-                const float step = __builtin_amdgcn_logf(u01_oc(wx)) * step_scale;
-                // first listed target hit within this step (photon_prop_cuda.jl:457-468)
-                float dt = 0.0f;
-                const int hit_t = first_hit(p, pos, dir, step, dt, tests);
-                if (hit_t < 0) {
-                    pos = make_float3(pos.x + dir.x * step, pos.y + dir.y * step, pos.z + dir.z * step);
-                    dist += step;
-                    // the mixture uniform matters only with a Simplified-Liu share
-                    // (a wave-uniform medium field): with none, u = 1 never selects it
-                    float u_mix = 1.0f;
-                    if (m.sl_fraction > 0.0f)
-                        u_mix = u01(((wx & 0xFFu) << 8) | ((wy & 0xFFu) << 16) | ((wz & 0xFFu) << 24));
-                    if (DIRM == NT_DIR_FP32_REF) {
-                        dir = update_direction32(dir, u01(wy), u01(wz), u_mix, m);
-                    } else {
-                        const const_kparams_ptr kh = kargs();
-                        const HgConst hg{kh->medium.hg.omg, kh->medium.hg.twog, kh->medium.hg.omg2, kh->medium.hg.k0,
-                                         kh->medium.hg.inv2g};
-                        dir = update_direction(dir, u01(wy), u01(wz), u_mix, m, hg);
-                    }
-                    return false;
-                }
-                pos = make_float3(pos.x + dir.x * dt, pos.y + dir.y * dt, pos.z + dir.z * dt);
-                dist += dt;
-                // Σ step/c_grp + dt/c_grp of photon_prop_cuda.jl:473,481 taken at
-                // once: the same sum up to FP32 rounding of the partial sums
-                time += dist / c_grp;
-                mid = p.module_ids[hit_t];
-                hit = true;
-                return true;
-            };
-            const int max_steps = m.max_steps;
-            PhiloxRun run;
-            run.init(rng.k0, rng.k1, rng.plo, rng.phi, rng.src);
-            int k = 0;
-            for (uint32_t blk = 1u; k < max_steps; blk += 3u) {
-                const uint4 b0 = run.block(blk);
-                if (scatter_step(b0.x, b0.y, b0.z) || ++k >= max_steps) break;
-                const uint4 b1 = run.block(blk + 1u);
-                if (scatter_step(b0.w, b1.x, b1.y) || ++k >= max_steps) break;
-                const uint4 b2 = run.block(blk + 2u);
-                if (scatter_step(b1.z, b1.w, b2.x) || ++k >= max_steps) break;
-                if (scatter_step(b2.y, b2.z, b2.w) || ++k >= max_steps) break;
-            }
-            // every step tested all strings (or all targets): k steps without
-            // a hit, one more if the photon hit
-            const uint32_t steps_taken = (uint32_t)k + (hit ? 1u : 0u);
+            simulate_photon<KIND, DIRM>(p, kp, S, s_knots, s_poly, pid, sidx, o);
+            tests += o.tests;
             if (p.n_clusters)
-                ctests += steps_taken * (uint32_t)p.n_clusters;
+                ctests += o.steps * (uint32_t)p.n_clusters;
             else
-                tests += steps_taken * (uint32_t)p.n_targets;
+                tests += o.steps * (uint32_t)p.n_targets;
         }
         simulated += __popcll(__ballot(active));
         // The per-lane test counts are 32-bit: one photon adds less than
         // (NT_MAX_STEPS + 1)·NT_MAX_TARGETS = 2^31, so they are committed at
         // the next photon once any lane has reached 2^31 and never wrap.
         commit_due = __ballot(((tests | ctests) >> 31) != 0u) != 0;
-
         // wavefront compaction of this round's hits into the LDS stage
-        const uint64_t mask = __ballot(hit);
+        const uint64_t mask = __ballot(o.hit);
         if (mask) {
             const int cnt = __popcll(mask);
             if (staged > 0 && (staged + cnt > kStageSlots || slot != stage_slot)) {
@@ -599,33 +673,363 @@ __global__ __launch_bounds__(kThreads, NTP_MIN_WAVES_PER_SIMD) void propagate_ke
                 staged = 0;
             }
             stage_slot = slot;
-            if (hit) {
-                const int at = staged + (int)__builtin_amdgcn_mbcnt_hi(
-                                            (uint32_t)(mask >> 32),
-                                            __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-                float* f = reinterpret_cast<float*>(st);
-                f[0 * kStageSlots + at] = pos.x;
-                f[1 * kStageSlots + at] = pos.y;
-                f[2 * kStageSlots + at] = pos.z;
-                f[3 * kStageSlots + at] = dir.x;
-                f[4 * kStageSlots + at] = dir.y;
-                f[5 * kStageSlots + at] = dir.z;
-                f[6 * kStageSlots + at] = idir.x;
-                f[7 * kStageSlots + at] = idir.y;
-                f[8 * kStageSlots + at] = idir.z;
-                f[9 * kStageSlots + at] = wl;
-                f[10 * kStageSlots + at] = time;
-                f[11 * kStageSlots + at] = dist;
-                st[12 * kStageSlots + at] = mid;
-                st[13 * kStageSlots + at] = sidx;
-                st[14 * kStageSlots + at] = (uint32_t)pid;
-                st[15 * kStageSlots + at] = (uint32_t)(pid >> 32);
-            }
+            if (o.hit) stage_write(st, staged, mask, o, sidx, pid);
             staged += cnt;
         }
     }
     if (staged) flush_stage(st, staged, lane, stage_slot);
     if (count_slot >= 0) commit_counts(count_slot, simulated, tests, ctests, lane);
+}
+
+// ---------------------------------------------------------------------------
+// Serving from a device work list (nt_server_*, NT_SERVER_WORKLIST)
+// ---------------------------------------------------------------------------
+// The host writes work items (photon ranges of one source of one request:
+// DevSource records with `photons` ≤ 2^30 and `slot` = the request's slot)
+// and slot records into rings in fine-grained host memory and advances
+// `n_pub` in the control block there; no copy engine or kernel launch is
+// needed to hand work over.  One wavefront at a time (a device lock) copies
+// newly published items into a device copy of the ring and refreshes the
+// device copies of n_pub (`n_ready`), `stop` and `epoch`: when a wavefront
+// finds nothing to claim (at most every kIdleFetchTicks) and, while all are
+// busy, every kBusyFetchTicks.  Claims read device memory only, so the host
+// memory traffic does not grow with the number of wavefronts.  Wavefronts of
+// a persistent kernel claim tickets of the oldest item that still has
+// unclaimed photons: the cursor names that item (only below n_ready, so no
+// claim is ever left unprocessed) and a claim is one fetch_add on the item's
+// `claimed` count, which cannot fail; an add past the item's end is ignored
+// and the cursor moves on with one CAS per item.  Small items come first
+// (see claim_ticket).  Tickets are the item's
+// photons over the number of wavefronts (multiples of 64, between 64 and
+// `ticket`), so one request spreads over every wavefront.  They propagate
+// them exactly as
+// propagate_kernel propagates a source, write their hits straight into the
+// request's pinned host columns, and add the ticket to the slot's finished
+// photons; the wavefront that completes a request copies its counters to
+// host memory and then writes its sequence number into done[slot].  Every
+// ring entry fills whole cache lines and is not reused while a kernel
+// instance runs, so no cache can hold a stale copy of it.  A wavefront
+// without work sleeps and polls.  The instance leaves as a whole: when the
+// host sets `stop`, or when one wavefront has seen neither work nor a new
+// `epoch` for `idle_ticks` (100 MHz ticks) and raises `exit_gen` to the
+// instance's launch generation; every wavefront then leaves at its next
+// claim (claimed tickets are always finished), so work published meanwhile
+// waits for a fresh full instance instead of a few remaining wavefronts.
+constexpr int kWorkSlots = 1 << 14;        // slot ring (requests in flight per kernel instance)
+constexpr int kWorkItems = 1 << 16;        // item ring
+
+struct WorkCtrl {
+    uint32_t n_pub;                        // items published (absolute count)
+    uint32_t stop;                         // 1: wavefronts exit at their next claim
+    uint32_t epoch;                        // advanced by the host while requests are pending
+    uint32_t pad;
+    uint32_t done[kWorkSlots];             // per slot: sequence number of the completed request
+};
+
+struct alignas(64) SlotExtra {
+    int64_t total;                         // photons of the request
+    int64_t* host_counters;                // where the request's final counters go (host memory)
+    float ref[3];                          // time-residual reference position (epilogue)
+    int32_t epilogue;                      // 1: fill pmt_id, total_weight, tres
+    uint32_t seq;                          // written to done[slot] on completion
+};
+
+struct alignas(128) WorkItem {
+    DevSource src;
+    unsigned long long claimed;            // device ring: photons claimed (0 when published)
+};
+static_assert(sizeof(WorkItem) == 128 && offsetof(WorkItem, claimed) == sizeof(DevSource), "WorkItem layout");
+
+// Device copy of the host's control words (see fetch_items).
+struct alignas(128) DevWork {
+    uint32_t n_ready;                      // items copied to the device ring (absolute count)
+    uint32_t stop;
+    uint32_t epoch;
+    uint32_t lock;                         // 1 while a wavefront fetches
+    uint64_t last_fetch;                   // s_memrealtime of the last fetch
+    uint32_t exit_gen;                     // instances of this generation and older leave
+};
+constexpr uint32_t kFetchMax = 1024;       // items copied per fetch
+constexpr int64_t kSmallItem = int64_t(1) << 21;   // items claimed first (photons)
+constexpr uint64_t kIdleFetchTicks = 1000;     // 10 µs
+constexpr uint64_t kBusyFetchTicks = 10000;    // 100 µs
+
+struct SrvParams {
+    const WorkItem* items;                 // item ring (kWorkItems), host memory
+    WorkItem* ditems;                      // its device copy
+    DevWork* dw;
+    const SlotExtra* extra;                // per slot (kWorkSlots), host memory
+    int64_t* slot_done;                    // photons finished per slot
+    uint32_t* cursor;                      // per class (small, large): oldest item that may have unclaimed photons
+    unsigned long long* busy_ticks;        // Σ ticket durations (100 MHz ticks)
+    WorkCtrl* ctrl;
+    Epilogue ep;
+    const int32_t* target_of_module;       // module id → target list index (epilogue)
+    uint32_t ticket;                       // largest ticket (multiple of 64)
+    uint32_t n_waves;                      // wavefronts of the kernel
+    uint32_t idle_ticks;
+    uint32_t gen;                          // launch generation of this instance (from 1)
+};
+
+// Copy the items published since the last fetch (at most kFetchMax) from
+// the host ring to the device ring, then n_ready, stop and epoch (whole
+// wavefront; returns without work unless it takes the lock and the last fetch
+// is at least `min_ticks` old).  Returns the number of items copied.
+__device__ uint32_t fetch_items(const SrvParams& sp, int lane, uint64_t min_ticks) {
+    uint32_t got = 0;
+    if (lane == 0) {
+        const uint64_t now = __builtin_amdgcn_s_memrealtime();
+        const uint64_t last = __hip_atomic_load(&sp.dw->last_fetch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t free_lock = 0;
+        if (now - last >= min_ticks &&
+            __hip_atomic_load(&sp.dw->lock, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u &&
+            __hip_atomic_compare_exchange_strong(&sp.dw->lock, &free_lock, 1u, __ATOMIC_ACQUIRE, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT))
+            got = 1;
+    }
+    if (!__builtin_amdgcn_readfirstlane(got)) return 0;
+    uint32_t npub = 0, stop = 0, epoch = 0, have = 0;
+    if (lane == 0) {
+        stop = __hip_atomic_load(&sp.ctrl->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        epoch = __hip_atomic_load(&sp.ctrl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        npub = __hip_atomic_load(&sp.ctrl->n_pub, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+        have = __hip_atomic_load(&sp.dw->n_ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    npub = __builtin_amdgcn_readfirstlane(npub);
+    stop = __builtin_amdgcn_readfirstlane(stop);
+    epoch = __builtin_amdgcn_readfirstlane(epoch);
+    have = __builtin_amdgcn_readfirstlane(have);
+    const int32_t pending = (int32_t)(npub - have);
+    const uint32_t n = pending > 0 ? min((uint32_t)pending, kFetchMax) : 0u;
+    // the 7 16-byte words of the source and the (zero) claimed count in each
+    // 128-byte entry
+    constexpr int kWords = (sizeof(DevSource) + 16) / 16, kEntry = sizeof(WorkItem) / 16;
+    const uint4* src = reinterpret_cast<const uint4*>(sp.items);
+    uint4* dst = reinterpret_cast<uint4*>(sp.ditems);
+    const uint32_t total = n * kWords;
+    for (uint32_t w = lane; w < total; w += 64) {
+        const uint32_t at = ((have + w / kWords) & (kWorkItems - 1)) * kEntry + w % kWords;
+        dst[at] = src[at];
+    }
+    // the copied entries are visible device-wide before n_ready
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (lane == 0) {
+        __hip_atomic_store(&sp.dw->stop, stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&sp.dw->epoch, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&sp.dw->n_ready, have + n, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&sp.dw->last_fetch, __builtin_amdgcn_s_memrealtime(), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&sp.dw->lock, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return n;
+}
+
+// Claim a ticket (one lane): 0 claimed, 1 nothing to claim now, 2 stop.
+// Items of at most kSmallItem photons are claimed first: each class has its
+// own cursor over the ring and steps over the other class's items, so a small
+// request does not wait behind the rest of a large one.
+__device__ int claim_ticket(const SrvParams& sp, uint32_t& item, uint32_t& off, uint32_t& take) {
+    int cls = 0;
+    for (int tries = 0; tries < 64; ++tries) {
+        if (__hip_atomic_load(&sp.dw->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+            (int32_t)(__hip_atomic_load(&sp.dw->exit_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - sp.gen) >= 0)
+            return 2;
+        const uint32_t nready = __hip_atomic_load(&sp.dw->n_ready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t k = __hip_atomic_load(sp.cursor + cls, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((int32_t)(nready - k) <= 0) {
+            if (cls == 1) return 1;
+            cls = 1;                       // no small item left: the large ones
+            continue;
+        }
+        WorkItem& it = sp.ditems[k & (kWorkItems - 1)];
+        const int64_t ph = it.src.photons;
+        if ((ph > kSmallItem) == (cls == 0)) {
+            // the other class's item: step over it
+            __hip_atomic_compare_exchange_strong(sp.cursor + cls, &k, k + 1u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+            continue;
+        }
+        const int64_t want = min((int64_t)sp.ticket, max((int64_t)64, (ph / sp.n_waves + 63) & ~(int64_t)63));
+        const int64_t o = (int64_t)__hip_atomic_fetch_add(&it.claimed, (unsigned long long)want, __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT);
+        if (o < ph) {
+            item = k;
+            off = (uint32_t)o;
+            take = (uint32_t)min(want, ph - o);
+            return 0;
+        }
+        // item exhausted: advance the cursor (whoever wins)
+        __hip_atomic_compare_exchange_strong(sp.cursor + cls, &k, k + 1u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return 1;
+}
+
+// Write the staged hits of this wavefront to the columns of `slot`, with the
+// epilogue columns (pmt_id, total_weight, tres) when the request asked for them.
+__device__ __forceinline__ void flush_stage_srv(const uint32_t* __restrict__ st, int n, int slot,
+                                                const SrvParams& sp) {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const int lane = (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+    int64_t* counters;
+    const HitCols h = slot_cols(kargs(), slot, counters);
+    const SlotExtra& x = sp.extra[slot];
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd((unsigned long long*)&counters[0], (unsigned long long)n);
+    base = __shfl(base, 0);
+    const float* f = reinterpret_cast<const float*>(st);
+    for (int i = lane; i < n; i += 64) {
+        const int64_t idx = (int64_t)base + i;
+        if (idx >= h.capacity) continue;
+        const float3 pos = make_float3(f[0 * kStageSlots + i], f[1 * kStageSlots + i], f[2 * kStageSlots + i]);
+        const float3 dir = make_float3(f[3 * kStageSlots + i], f[4 * kStageSlots + i], f[5 * kStageSlots + i]);
+        const float wl = f[9 * kStageSlots + i], time = f[10 * kStageSlots + i], dist = f[11 * kStageSlots + i];
+        const uint32_t mid = st[12 * kStageSlots + i];
+        h.pos[3 * idx + 0] = pos.x;
+        h.pos[3 * idx + 1] = pos.y;
+        h.pos[3 * idx + 2] = pos.z;
+        h.dir[3 * idx + 0] = dir.x;
+        h.dir[3 * idx + 1] = dir.y;
+        h.dir[3 * idx + 2] = dir.z;
+        h.idir[3 * idx + 0] = f[6 * kStageSlots + i];
+        h.idir[3 * idx + 1] = f[7 * kStageSlots + i];
+        h.idir[3 * idx + 2] = f[8 * kStageSlots + i];
+        h.wl[idx] = wl;
+        h.time[idx] = time;
+        h.dist[idx] = dist;
+        h.module_id[idx] = (uint16_t)mid;
+        h.source_index[idx] = st[13 * kStageSlots + i];
+        h.photon_id[idx] = (uint64_t)st[14 * kStageSlots + i] | ((uint64_t)st[15 * kStageSlots + i] << 32);
+        if (x.epilogue) {
+            const int ti = sp.target_of_module[mid & 0xFFFFu];
+            uint8_t pmt = 0;
+            float w = 0.0f, tres = 0.0f;
+            if (ti >= 0) hit_epilogue(sp.ep, ti, pos, dir, wl, dist, time, pmt, w, tres, x.ref);
+            h.pmt_id[idx] = pmt;
+            h.total_weight[idx] = w;
+            h.tres[idx] = tres;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <int DIRM>
+__global__ __launch_bounds__(kThreads, NTP_SERVE_WAVES_PER_SIMD) void serve_kernel(KParams p, SrvParams sp) {
+    extern __shared__ float lds[];
+    float* s_knots = lds;
+    uint32_t* s_stage = reinterpret_cast<uint32_t*>(lds + ((p.spectrum.n + 3) & ~3));
+    __shared__ double s_poly[kScaPolyCoefs];
+    for (int i = threadIdx.x; i < p.spectrum.n; i += kThreads) s_knots[i] = p.knots[i];
+    if (threadIdx.x < kScaPolyCoefs) s_poly[threadIdx.x] = p.sca_poly.c[threadIdx.x];
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    uint32_t* st = s_stage + wave * (kStageFields * kStageSlots);
+    __shared__ __align__(16) uint32_t s_src[(kThreads / 64) * kSrcWords];
+    uint32_t last_epoch = 0;
+    uint64_t idle_since = 0;
+    bool idle = false;
+    for (;;) {
+        uint32_t item = 0, off = 0, take = 0;
+        int state = 0;
+        if (lane == 0) state = claim_ticket(sp, item, off, take);
+        state = __builtin_amdgcn_readfirstlane(state);
+        if (state == 2) break;
+        if (state == 1) {
+            if (fetch_items(sp, lane, kIdleFetchTicks)) continue;
+            uint32_t e = 0;
+            if (lane == 0) e = __hip_atomic_load(&sp.dw->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            e = __builtin_amdgcn_readfirstlane(e);
+            const uint64_t now = __builtin_amdgcn_s_memrealtime();
+            if (!idle || e != last_epoch) {
+                idle = true;
+                last_epoch = e;
+                idle_since = now;
+            } else if (now - idle_since > sp.idle_ticks) {
+                if (lane == 0)
+                    __hip_atomic_fetch_max(&sp.dw->exit_gen, sp.gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(127);
+            continue;
+        }
+        idle = false;
+        const uint64_t t_start = __builtin_amdgcn_s_memrealtime();
+        item = __builtin_amdgcn_readfirstlane(item);
+        off = __builtin_amdgcn_readfirstlane(off);
+        take = __builtin_amdgcn_readfirstlane(take);
+        const const_kparams_ptr kp = kargs();
+        // The item's source record is copied once per ticket (one word per lane)
+        // into this wavefront's LDS slot, which the photon loop reads.  Earlier
+        // reads of the slot are ordered before the rewrite.
+        const uint32_t* sw = reinterpret_cast<const uint32_t*>(&sp.ditems[item & (kWorkItems - 1)].src);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (lane < kSrcWords) s_src[wave * kSrcWords + lane] = sw[lane];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const DevSource* S = reinterpret_cast<const DevSource*>(s_src + wave * kSrcWords);
+        const int slot = S->slot;
+        const uint32_t sidx = S->rng_index;
+        int staged = 0;
+        int64_t simulated = 0;
+        uint32_t tests = 0, ctests = 0;
+        for (uint32_t g = 0; g < take; g += 64) {
+            const int64_t pid_local = (int64_t)off + g + lane;
+            const bool active = pid_local < (int64_t)off + take;
+            const uint64_t pid = S->photon_base + (uint64_t)pid_local;
+            PhotonOut o;
+            o.wl = o.time = o.dist = 0.0f;
+            o.mid = o.steps = o.tests = 0;
+            o.hit = false;
+            if (active) {
+                simulate_photon<-1, DIRM>(p, kp, S, s_knots, s_poly, pid, sidx, o);
+                tests += o.tests;
+                if (p.n_clusters)
+                    ctests += o.steps * (uint32_t)p.n_clusters;
+                else
+                    tests += o.steps * (uint32_t)p.n_targets;
+            }
+            simulated += __popcll(__ballot(active));
+            if (__ballot(((tests | ctests) >> 31) != 0u)) {      // 32-bit per-lane counts never wrap
+                commit_counts(slot, simulated, tests, ctests, lane);
+                simulated = tests = ctests = 0;
+            }
+            const uint64_t mask = __ballot(o.hit);
+            if (mask) {
+                const int cnt = __popcll(mask);
+                if (staged + cnt > kStageSlots) {
+                    flush_stage_srv(st, staged, slot, sp);
+                    staged = 0;
+                }
+                if (o.hit) stage_write(st, staged, mask, o, sidx, pid);
+                staged += cnt;
+            }
+        }
+        if (staged) flush_stage_srv(st, staged, slot, sp);
+        commit_counts(slot, simulated, tests, ctests, lane);
+        // every lane's hit stores are visible device- and host-wide before the count
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+        if (lane == 0) {
+            const SlotExtra& x = sp.extra[slot];
+            const int64_t done =
+                __hip_atomic_fetch_add(sp.slot_done + slot, (int64_t)take, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_SYSTEM) +
+                (int64_t)take;
+            if (done == x.total) {
+                // the request's counters (device atomics) to its host result, then the mark
+                int64_t* counters;
+                (void)slot_cols(kp, slot, counters);
+                for (int q = 0; q < 4; ++q)
+                    x.host_counters[q] = __hip_atomic_load(counters + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&sp.ctrl->done[slot], x.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+            atomicAdd(sp.busy_ticks, (unsigned long long)(__builtin_amdgcn_s_memrealtime() - t_start));
+        }
+        (void)fetch_items(sp, lane, kBusyFetchTicks);
+    }
 }
 
 typedef void (*PropagateFn)(KParams);
@@ -646,6 +1050,11 @@ PropagateFn propagate_variant_m(int v) {
 
 PropagateFn propagate_variant(int v, int dir_mode) {
     return dir_mode == NT_DIR_FP32_REF ? propagate_variant_m<NT_DIR_FP32_REF>(v) : propagate_variant_m<NT_DIR_FP64>(v);
+}
+
+typedef void (*ServeFn)(KParams, SrvParams);
+ServeFn serve_variant(int dir_mode) {
+    return dir_mode == NT_DIR_FP32_REF ? serve_kernel<NT_DIR_FP32_REF> : serve_kernel<NT_DIR_FP64>;
 }
 
 __global__ void clear_counters(int64_t* c) {
@@ -1039,6 +1448,7 @@ int32_t rebuild_epilogue(nt_ctx* c) {
                     dz = c->epi_source[2] - t.position[2];
         const float dist = sqrtf((dx * dx + dy * dy) + dz * dz);
         e.tgeo = (double)(dist - t.radius) / c_med;
+        e.radius = t.radius;
     }
     int32_t st = grow(&c->d_epi_targets, (int64_t)et.size(), &c->epi_targets_cap);
     if (st) return st;
@@ -1076,6 +1486,7 @@ int32_t rebuild_epilogue(nt_ctx* c) {
     d.n_ang = (int)ng;
     d.abs_len_scale = c->abs_len_scale;
     d.enabled = 1;
+    d.c_med = c_med;
     return NT_OK;
 }
 
@@ -1928,8 +2339,9 @@ class Arena {
 
 // Column layout of one request's hits inside an arena block (device and pinned host alike).
 struct ReqLayout {
-    size_t counters, pos, dir, idir, wl, time, dist, mid, sidx, pid, bytes;
-    explicit ReqLayout(int64_t cap) {
+    size_t counters, pos, dir, idir, wl, time, dist, mid, sidx, pid, pmt = 0, weight = 0, tres = 0, bytes;
+    bool epi;
+    explicit ReqLayout(int64_t cap, bool epilogue = false) : epi(epilogue) {
         size_t o = 0;
         auto take = [&](size_t n) { size_t at = o; o += align_up(n); return at; };
         counters = take(4 * sizeof(int64_t));
@@ -1942,6 +2354,11 @@ struct ReqLayout {
         mid = take(2 * cap);
         sidx = take(4 * cap);
         pid = take(8 * cap);
+        if (epi) {
+            pmt = take(1 * cap);
+            weight = take(4 * cap);
+            tres = take(4 * cap);
+        }
         bytes = o;
     }
     HitCols cols(char* base, int64_t cap) const {
@@ -1950,6 +2367,9 @@ struct ReqLayout {
         h.wl = (float*)(base + wl); h.time = (float*)(base + time); h.dist = (float*)(base + dist);
         h.module_id = (uint16_t*)(base + mid); h.source_index = (uint32_t*)(base + sidx);
         h.photon_id = (uint64_t*)(base + pid); h.capacity = cap;
+        if (epi) {
+            h.pmt_id = (uint8_t*)(base + pmt); h.total_weight = (float*)(base + weight); h.tres = (float*)(base + tres);
+        }
         return h;
     }
 };
@@ -1969,6 +2389,10 @@ struct Request {
     int64_t n_found = 0, n_sim = 0;
     hipEvent_t ev = nullptr;           // completion of the pending copy
     int32_t status = NT_OK;
+    bool epilogue = false;             // work-list mode: fill pmt_id, total_weight, tres
+    float ref[3] = {0.0f, 0.0f, 0.0f}; // their time-residual reference position
+    int slot = -1;                     // work-list mode: slot ring position
+    uint32_t seq = 0;                  // its completion mark
     std::chrono::steady_clock::time_point t_submit, t_done;
 };
 
@@ -2006,6 +2430,29 @@ struct nt_server {
     int32_t fatal = NT_OK;
     nt_server_stats stats{};
     std::thread worker;
+    // work-list mode (see serve_kernel)
+    int32_t mode = NT_SERVER_ROUNDS;
+    WorkCtrl* ctrl = nullptr;                          // fine-grained pinned host memory
+    WorkItem* h_items = nullptr;                       // rings in fine-grained host memory
+    SlotDesc* h_slots = nullptr;
+    SlotExtra* h_extra = nullptr;
+    int64_t* d_slot_done = nullptr;                    // device: finished photons per slot
+    int64_t* d_slot_cnt = nullptr;                     // device: the 4 counters per slot
+    unsigned long long* d_claim = nullptr;             // [0] item cursors (small, large), [1] busy ticks
+    DevWork* d_work = nullptr;                         // device copies of the control words
+    WorkItem* d_items = nullptr;                       // device copy of the item ring
+    hipStream_t upload = nullptr;
+    hipStream_t kstream[2] = {nullptr, nullptr};
+    hipEvent_t kdone[2] = {nullptr, nullptr};
+    bool klaunched[2] = {false, false};
+    uint32_t items_pub = 0;                            // items published (absolute, = ctrl->n_pub)
+    uint64_t slots_pub = 0;                            // slots handed out (absolute)
+    uint32_t win_items0 = 0;                           // first item / slot of the running window
+    uint64_t win_slots0 = 0;
+    bool window_full = false;
+    int srv_blocks = 0;
+    uint32_t launch_gen = 0;                           // generation of the last launched instance
+    std::chrono::steady_clock::time_point last_bump{};
 };
 
 namespace {
@@ -2036,8 +2483,8 @@ void admit(nt_server* sv) {
         r.dev_off = doff;
         r.host_off = hoff;
         r.state = RQ_ACTIVE;
-        if (hipMemsetAsync(sv->d_arena + doff + ReqLayout(r.capacity).counters, 0, 4 * sizeof(int64_t),
-                           sv->stream) != hipSuccess)
+        if (hipMemsetAsync(sv->d_arena + doff + ReqLayout(r.capacity, r.epilogue).counters, 0, 4 * sizeof(int64_t),
+                           sv->mode == NT_SERVER_WORKLIST ? sv->upload : sv->stream) != hipSuccess)
             sv->fatal = NT_ERR_DEVICE;
         sv->active.push_back(r.id);
         sv->queue.pop_front();
@@ -2143,17 +2590,19 @@ bool progress(nt_server* sv) {
                                                                                   : hipErrorNotReady;
         if (q != hipSuccess && q != hipErrorNotReady) { sv->fatal = NT_ERR_DEVICE; return true; }
         if (q == hipSuccess && r.state == RQ_COPY_COUNTS) {
-            const ReqLayout L(r.capacity);
+            const ReqLayout L(r.capacity, r.epilogue);
             const int64_t* cnt = (const int64_t*)(sv->h_arena + r.host_off + L.counters);
             r.n_found = cnt[0];
             r.n_sim = cnt[1];
             const int64_t n = std::min(r.n_found, r.capacity);
             char* d = sv->d_arena + r.dev_off;
             char* h = sv->h_arena + r.host_off;
-            const size_t off[9] = {L.pos, L.dir, L.idir, L.wl, L.time, L.dist, L.mid, L.sidx, L.pid};
-            const size_t w[9] = {12, 12, 12, 4, 4, 4, 2, 4, 8};
+            const size_t off[12] = {L.pos, L.dir, L.idir, L.wl, L.time, L.dist, L.mid, L.sidx, L.pid,
+                                    L.pmt, L.weight, L.tres};
+            const size_t w[12] = {12, 12, 12, 4, 4, 4, 2, 4, 8, 1, 4, 4};
+            const int ncol = r.epilogue ? 12 : 9;
             bool ok = true;
-            for (int k = 0; k < 9 && n > 0; ++k)
+            for (int k = 0; k < ncol && n > 0; ++k)
                 ok &= hipMemcpyAsync(h + off[k], d + off[k], w[k] * n, hipMemcpyDeviceToHost, sv->copy_stream) ==
                       hipSuccess;
             ok &= hipEventRecord(r.ev, sv->copy_stream) == hipSuccess;
@@ -2176,9 +2625,207 @@ bool progress(nt_server* sv) {
     return changed;
 }
 
+constexpr int64_t kItemMax = int64_t(1) << 30;   // photons per work item
+constexpr uint32_t kTicket = 4096;               // largest ticket (photons per claim)
+constexpr uint32_t kIdleTicks = 200000;          // 2 ms of the 100 MHz real-time counter
+
+int64_t items_of(const Request& r) {
+    int64_t n = 0;
+    for (const DevSource& d : r.src)
+        if (d.photons > 0) n += (d.photons + kItemMax - 1) / kItemMax;
+    return n;
+}
+
+// Publish the admitted requests that fit the running window: their slot and
+// item records go straight into the host rings, then n_pub.
+void worklist_publish(nt_server* sv) {
+    uint32_t k = sv->items_pub;
+    bool any = false;
+    for (uint64_t id : sv->active) {
+        Request& r = *sv->reqs[id];
+        if (r.state != RQ_ACTIVE) continue;
+        const int64_t need = items_of(r);
+        if ((int64_t)(k - sv->win_items0) + need > kWorkItems ||
+            sv->slots_pub + 1 - sv->win_slots0 > (uint64_t)kWorkSlots) {
+            sv->window_full = true;
+            break;
+        }
+        const int slot = (int)(sv->slots_pub % kWorkSlots);
+        r.slot = slot;
+        r.seq = (uint32_t)(sv->slots_pub + 1);
+        sv->slots_pub += 1;
+        const ReqLayout L(r.capacity, r.epilogue);
+        SlotDesc& sd = sv->h_slots[slot];
+        sd.cols = L.cols(sv->h_arena + r.host_off, r.capacity);
+        sd.counters = sv->d_slot_cnt + 4 * slot;
+        SlotExtra& x = sv->h_extra[slot];
+        x.total = r.photons;
+        x.host_counters = (int64_t*)(sv->h_arena + r.host_off + L.counters);
+        for (int q = 0; q < 3; ++q) x.ref[q] = r.ref[q];
+        x.epilogue = r.epilogue ? 1 : 0;
+        x.seq = r.seq;
+        for (size_t j = 0; j < r.src.size(); ++j) {
+            for (int64_t at = 0; at < r.src[j].photons; at += kItemMax) {
+                DevSource& d = sv->h_items[k % kWorkItems].src;
+                d = r.src[j];
+                d.photons = std::min(kItemMax, r.src[j].photons - at);
+                d.photon_base = (uint64_t)at;
+                d.key_lo = (uint32_t)r.seed;
+                d.key_hi = (uint32_t)(r.seed >> 32);
+                d.rng_index = (uint32_t)j;
+                d.slot = slot;
+                d.offset = 0;
+                sv->h_items[k % kWorkItems].claimed = 0;
+                ++k;
+            }
+        }
+        r.state = RQ_SCHEDULED;
+        any = true;
+    }
+    if (!any) return;
+    sv->items_pub = k;
+    __atomic_store_n(&sv->ctrl->n_pub, k, __ATOMIC_RELEASE);
+}
+
+int32_t worklist_launch(nt_server* sv, int i) {
+    nt_ctx* c = sv->ctx;
+    KParams kp{};
+    kp.slots = sv->h_slots;
+    kp.targets = c->d_targets;
+    kp.module_ids = c->d_module_ids;
+    kp.clusters = c->d_clusters;
+    kp.members = c->d_members;
+    kp.member_idx = c->d_member_idx;
+    kp.n_clusters = c->culling ? c->n_clusters : 0;
+    kp.cluster_reach = c->cluster_reach;
+    kp.knots = c->d_knots;
+    kp.medium = c->medium;
+    kp.spectrum = c->spectrum;
+    kp.sca_poly = c->sca_poly;
+    kp.n_targets = c->n_targets;
+    SrvParams sp{};
+    sp.items = sv->h_items;
+    sp.ditems = sv->d_items;
+    sp.dw = sv->d_work;
+    sp.extra = sv->h_extra;
+    sp.slot_done = sv->d_slot_done;
+    sp.cursor = reinterpret_cast<uint32_t*>(sv->d_claim);
+    sp.busy_ticks = sv->d_claim + 1;
+    sp.ctrl = sv->ctrl;
+    sp.ep = c->epi;
+    sp.target_of_module = c->d_target_of_module;
+    sp.ticket = kTicket;
+    sp.n_waves = (uint32_t)sv->srv_blocks * (kThreads / 64);
+    sp.gen = ++sv->launch_gen;
+    sp.idle_ticks = kIdleTicks;
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        hipLaunchKernelGGL(serve_variant(c->medium.dir_mode), dim3(sv->srv_blocks), dim3(kThreads),
+                           lds_bytes(c->spectrum.n), sv->kstream[i], kp, sp);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipEventRecord(sv->kdone[i], sv->kstream[i]));
+    sv->klaunched[i] = true;
+    return NT_OK;
+}
+
+// Keep a kernel instance running while published requests are pending:
+// advance the epoch, launch when none runs (or a second one when the host
+// was away long enough for wavefronts to start leaving).  A full window is
+// reset once it has drained and no instance runs: the device counts are
+// zeroed while nothing reads them.
+int32_t worklist_kernels(nt_server* sv, bool pending) {
+    for (int i = 0; i < 2; ++i) {
+        if (!sv->klaunched[i]) continue;
+        const hipError_t q = hipEventQuery(sv->kdone[i]);
+        if (q == hipSuccess) sv->klaunched[i] = false;
+        else if (q != hipErrorNotReady) return NT_ERR_DEVICE;
+    }
+    const auto now = std::chrono::steady_clock::now();
+    if (pending) {
+        const bool stale = now - sv->last_bump > std::chrono::microseconds(kIdleTicks / 200);   // half the idle limit
+        __atomic_fetch_add(&sv->ctrl->epoch, 1u, __ATOMIC_RELEASE);
+        sv->last_bump = now;
+        const int live = (int)sv->klaunched[0] + (int)sv->klaunched[1];
+        if (live == 0 || (stale && live < 2)) {
+            const int32_t st = worklist_launch(sv, sv->klaunched[0] ? 1 : 0);
+            if (st) return st;
+            sv->stats.rounds += 1;
+        }
+    } else if (sv->window_full && !sv->klaunched[0] && !sv->klaunched[1]) {
+        HIP_TRY(hipMemsetAsync(sv->d_slot_done, 0, sizeof(int64_t) * kWorkSlots, sv->upload));
+        HIP_TRY(hipMemsetAsync(sv->d_slot_cnt, 0, 4 * sizeof(int64_t) * kWorkSlots, sv->upload));
+        HIP_TRY(hipStreamSynchronize(sv->upload));
+        sv->win_items0 = sv->items_pub;
+        sv->win_slots0 = sv->slots_pub;
+        sv->window_full = false;
+    }
+    return NT_OK;
+}
+
+// Requests whose completion mark arrived are done: the kernel wrote their
+// hits and counters into their pinned host block.
+bool worklist_poll(nt_server* sv) {
+    bool changed = false;
+    for (size_t i = 0; i < sv->active.size();) {
+        Request& r = *sv->reqs[sv->active[i]];
+        if (r.state != RQ_SCHEDULED || __atomic_load_n(&sv->ctrl->done[r.slot], __ATOMIC_ACQUIRE) != r.seq) {
+            ++i;
+            continue;
+        }
+        const ReqLayout L(r.capacity, r.epilogue);
+        const volatile int64_t* cnt = (const volatile int64_t*)(sv->h_arena + r.host_off + L.counters);
+        r.n_found = cnt[0];
+        r.n_sim = cnt[1];
+        r.t_done = std::chrono::steady_clock::now();
+        r.state = RQ_DONE;
+        r.status = r.n_found > r.capacity ? NT_ERR_HIT_OVERFLOW : NT_OK;
+        sv->active.erase(sv->active.begin() + i);
+        sv->stats.completed += 1;
+        sv->stats.photons += r.photons;
+        changed = true;
+    }
+    if (changed) sv->cv_done.notify_all();
+    return changed;
+}
+
+// Admission for the work list: pinned host memory only (the kernel writes
+// the results there directly).
+void worklist_admit(nt_server* sv) {
+    while (!sv->queue.empty()) {
+        Request& r = *sv->reqs[sv->queue.front()];
+        size_t hoff;
+        if (!sv->host_alloc.alloc(r.bytes, hoff)) return;
+        r.host_off = hoff;
+        r.state = RQ_ACTIVE;
+        sv->active.push_back(r.id);
+        sv->queue.pop_front();
+    }
+}
+
+void worklist_loop(nt_server* sv, std::unique_lock<std::mutex>& lk) {
+    while (!sv->stop && sv->fatal == NT_OK) {
+        worklist_admit(sv);
+        worklist_publish(sv);
+        bool pending = false;
+        for (uint64_t id : sv->active) pending |= sv->reqs[id]->state == RQ_SCHEDULED;
+        const int32_t st = worklist_kernels(sv, pending);
+        if (st) {
+            sv->fatal = st;
+            break;
+        }
+        if (!worklist_poll(sv)) {
+            const bool idle = sv->queue.empty() && sv->active.empty();
+            if (idle) sv->cv_work.wait_for(lk, std::chrono::milliseconds(5));
+            else sv->cv_work.wait_for(lk, std::chrono::microseconds(20));
+        }
+    }
+}
+
 void server_loop(nt_server* sv) {
     (void)hipSetDevice(sv->ctx->device);
     std::unique_lock<std::mutex> lk(sv->mu);
+    if (sv->mode == NT_SERVER_WORKLIST) worklist_loop(sv, lk);
     while (!sv->stop && sv->fatal == NT_OK) {
         admit(sv);
         bool launched = false;
@@ -2216,6 +2863,20 @@ void free_server(nt_server* sv) {
     if (sv->h_arena) (void)hipHostFree(sv->h_arena);
     if (sv->stream) (void)hipStreamDestroy(sv->stream);
     if (sv->copy_stream) (void)hipStreamDestroy(sv->copy_stream);
+    if (sv->ctrl) (void)hipHostFree(sv->ctrl);
+    if (sv->h_items) (void)hipHostFree(sv->h_items);
+    if (sv->h_slots) (void)hipHostFree(sv->h_slots);
+    if (sv->h_extra) (void)hipHostFree(sv->h_extra);
+    if (sv->d_slot_done) (void)hipFree(sv->d_slot_done);
+    if (sv->d_slot_cnt) (void)hipFree(sv->d_slot_cnt);
+    if (sv->d_claim) (void)hipFree(sv->d_claim);
+    if (sv->d_work) (void)hipFree(sv->d_work);
+    if (sv->d_items) (void)hipFree(sv->d_items);
+    if (sv->upload) (void)hipStreamDestroy(sv->upload);
+    for (int i = 0; i < 2; ++i) {
+        if (sv->kstream[i]) (void)hipStreamDestroy(sv->kstream[i]);
+        if (sv->kdone[i]) (void)hipEventDestroy(sv->kdone[i]);
+    }
     delete sv;
 }
 
@@ -2225,8 +2886,10 @@ extern "C" {
 
 int32_t nt_server_create(nt_ctx* c, const nt_server_config* cfg, nt_server** out) {
     if (!c || !cfg || !out || cfg->device_budget_bytes < kAlign || cfg->host_budget_bytes < kAlign ||
-        cfg->round_photons < 64 || cfg->max_inflight_rounds < 1 || cfg->max_inflight_rounds > 16) {
-        snprintf(g_err, sizeof g_err, "server config: budgets >= 256 B, round_photons >= 64, 1..16 rounds in flight");
+        cfg->round_photons < 64 || cfg->max_inflight_rounds < 1 || cfg->max_inflight_rounds > 16 ||
+        (cfg->mode != NT_SERVER_ROUNDS && cfg->mode != NT_SERVER_WORKLIST)) {
+        snprintf(g_err, sizeof g_err,
+                 "server config: budgets >= 256 B, round_photons >= 64, 1..16 rounds in flight, a known mode");
         return NT_ERR_ARG;
     }
     if (!c->has_medium || !c->has_spectrum || !c->has_targets) {
@@ -2242,8 +2905,13 @@ int32_t nt_server_create(nt_ctx* c, const nt_server_config* cfg, nt_server** out
     hipError_t e;
     if ((e = hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking)) != hipSuccess) return fail(e, "stream");
     if ((e = hipStreamCreateWithFlags(&sv->copy_stream, hipStreamNonBlocking)) != hipSuccess) return fail(e, "stream");
-    if ((e = hipMalloc((void**)&sv->d_arena, cfg->device_budget_bytes)) != hipSuccess) return fail(e, "device budget");
-    if ((e = hipHostMalloc((void**)&sv->h_arena, cfg->host_budget_bytes)) != hipSuccess) return fail(e, "host budget");
+    const bool wl = cfg->mode == NT_SERVER_WORKLIST;
+    // work list: the kernel writes results straight into fine-grained pinned memory
+    if (!wl && (e = hipMalloc((void**)&sv->d_arena, cfg->device_budget_bytes)) != hipSuccess)
+        return fail(e, "device budget");
+    if ((e = hipHostMalloc((void**)&sv->h_arena, cfg->host_budget_bytes, wl ? hipHostMallocCoherent : 0)) !=
+        hipSuccess)
+        return fail(e, "host budget");
     sv->dev_alloc.init(cfg->device_budget_bytes);
     sv->host_alloc.init(cfg->host_budget_bytes);
     sv->staging.resize(cfg->max_inflight_rounds);
@@ -2251,14 +2919,61 @@ int32_t nt_server_create(nt_ctx* c, const nt_server_config* cfg, nt_server** out
         if ((e = hipEventCreate(&s.start)) != hipSuccess) return fail(e, "event");
         if ((e = hipEventCreate(&s.done)) != hipSuccess) return fail(e, "event");
     }
+    sv->mode = cfg->mode;
+    if (sv->mode == NT_SERVER_WORKLIST) {
+        if ((e = hipHostMalloc((void**)&sv->ctrl, sizeof(WorkCtrl), hipHostMallocCoherent)) != hipSuccess)
+            return fail(e, "control block");
+        memset(sv->ctrl, 0, sizeof(WorkCtrl));
+        if ((e = hipHostMalloc((void**)&sv->h_items, sizeof(WorkItem) * kWorkItems, hipHostMallocCoherent)) !=
+                hipSuccess ||
+            (e = hipHostMalloc((void**)&sv->h_slots, sizeof(SlotDesc) * kWorkSlots, hipHostMallocCoherent)) !=
+                hipSuccess ||
+            (e = hipHostMalloc((void**)&sv->h_extra, sizeof(SlotExtra) * kWorkSlots, hipHostMallocCoherent)) !=
+                hipSuccess)
+            return fail(e, "work-list rings");
+        if ((e = hipMalloc((void**)&sv->d_slot_done, sizeof(int64_t) * kWorkSlots)) != hipSuccess ||
+            (e = hipMalloc((void**)&sv->d_slot_cnt, 4 * sizeof(int64_t) * kWorkSlots)) != hipSuccess ||
+            (e = hipMalloc((void**)&sv->d_claim, 2 * sizeof(unsigned long long))) != hipSuccess ||
+            (e = hipMalloc((void**)&sv->d_work, sizeof(DevWork))) != hipSuccess ||
+            (e = hipMalloc((void**)&sv->d_items, sizeof(WorkItem) * kWorkItems)) != hipSuccess)
+            return fail(e, "work-list counters");
+        if ((e = hipMemset(sv->d_claim, 0, 2 * sizeof(unsigned long long))) != hipSuccess ||
+            (e = hipMemset(sv->d_work, 0, sizeof(DevWork))) != hipSuccess ||
+            (e = hipMemset(sv->d_slot_done, 0, sizeof(int64_t) * kWorkSlots)) != hipSuccess ||
+            (e = hipMemset(sv->d_slot_cnt, 0, 4 * sizeof(int64_t) * kWorkSlots)) != hipSuccess)
+            return fail(e, "work-list counters");
+        if ((e = hipStreamCreateWithFlags(&sv->upload, hipStreamNonBlocking)) != hipSuccess) return fail(e, "stream");
+        for (int i = 0; i < 2; ++i) {
+            if ((e = hipStreamCreateWithFlags(&sv->kstream[i], hipStreamNonBlocking)) != hipSuccess)
+                return fail(e, "stream");
+            if ((e = hipEventCreateWithFlags(&sv->kdone[i], hipEventDisableTiming)) != hipSuccess)
+                return fail(e, "event");
+        }
+        int per_cu = 0;
+        if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, serve_variant(c->medium.dir_mode), kThreads,
+                                                             lds_bytes(NT_MAX_KNOTS))) != hipSuccess)
+            return fail(e, "occupancy");
+        sv->srv_blocks = std::max(1, per_cu) * c->num_cus;
+        if ((e = hipDeviceSynchronize()) != hipSuccess) return fail(e, "work-list rings");
+    }
     sv->worker = std::thread(server_loop, sv);
     *out = sv;
     return NT_OK;
 }
 
-int32_t nt_server_submit(nt_server* sv, const nt_source* s, int32_t n, uint64_t seed, int64_t cap, uint64_t* id) {
+static int32_t server_submit(nt_server* sv, const nt_source* s, int32_t n, uint64_t seed, int64_t cap,
+                             const float* ref, uint64_t* id) {
     if (!sv || n < 1 || !s || cap < 0 || !id) return NT_ERR_ARG;
     auto r = std::make_unique<Request>();
+    if (ref) {
+        if (sv->mode != NT_SERVER_WORKLIST || !sv->ctx->epi_enabled) {
+            snprintf(g_err, sizeof g_err, "epilogue columns need a work-list server and nt_set_epilogue");
+            return NT_ERR_STATE;
+        }
+        if (!std::isfinite(ref[0]) || !std::isfinite(ref[1]) || !std::isfinite(ref[2])) return NT_ERR_ARG;
+        r->epilogue = true;
+        for (int q = 0; q < 3; ++q) r->ref[q] = ref[q];
+    }
     r->src.resize(n);
     r->next.assign(n, 0);
     for (int i = 0; i < n; ++i) {
@@ -2268,8 +2983,9 @@ int32_t nt_server_submit(nt_server* sv, const nt_source* s, int32_t n, uint64_t 
     }
     r->seed = seed;
     r->capacity = cap;
-    r->bytes = ReqLayout(cap).bytes;
-    if (r->bytes > sv->cfg.device_budget_bytes || r->bytes > sv->cfg.host_budget_bytes) {
+    r->bytes = ReqLayout(cap, r->epilogue).bytes;
+    if ((sv->mode != NT_SERVER_WORKLIST && r->bytes > sv->cfg.device_budget_bytes) ||
+        r->bytes > sv->cfg.host_budget_bytes) {
         snprintf(g_err, sizeof g_err, "request needs %zu bytes of hit buffers, above the budget", r->bytes);
         return NT_ERR_BUDGET;
     }
@@ -2291,6 +3007,16 @@ int32_t nt_server_submit(nt_server* sv, const nt_source* s, int32_t n, uint64_t 
     sv->stats.submitted += 1;
     sv->cv_work.notify_one();
     return NT_OK;
+}
+
+int32_t nt_server_submit(nt_server* sv, const nt_source* s, int32_t n, uint64_t seed, int64_t cap, uint64_t* id) {
+    return server_submit(sv, s, n, seed, cap, nullptr, id);
+}
+
+int32_t nt_server_submit_epilogue(nt_server* sv, const nt_source* s, int32_t n, uint64_t seed, int64_t cap,
+                                  const float* ref_position, uint64_t* id) {
+    if (!ref_position) return NT_ERR_ARG;
+    return server_submit(sv, s, n, seed, cap, ref_position, id);
 }
 
 int32_t nt_server_poll(nt_server* sv, uint64_t id) {
@@ -2333,6 +3059,12 @@ int32_t nt_server_wait(nt_server* sv, uint64_t id, const nt_hits* out, int64_t* 
         memcpy(out->module_id, h + L.mid, 2 * n);
         if (out->source_index) memcpy(out->source_index, h + L.sidx, 4 * n);
         if (out->photon_id) memcpy(out->photon_id, h + L.pid, 8 * n);
+        if (r->epilogue) {
+            const ReqLayout Le(r->capacity, true);
+            if (out->pmt_id) memcpy(out->pmt_id, h + Le.pmt, n);
+            if (out->total_weight) memcpy(out->total_weight, h + Le.weight, 4 * n);
+            if (out->tres) memcpy(out->tres, h + Le.tres, 4 * n);
+        }
     }
     if (n_hits) *n_hits = status == NT_OK ? n : 0;
     if (n_sim) *n_sim = r->n_sim;
@@ -2354,6 +3086,16 @@ int32_t nt_server_stats_get(nt_server* sv, nt_server_stats* out) {
     *out = sv->stats;
     out->queued = (int64_t)sv->queue.size();
     out->active = (int64_t)sv->active.size();
+    if (sv->mode == NT_SERVER_WORKLIST) {
+        // Σ ticket durations over all wavefronts ÷ wavefronts of one instance: the
+        // time the whole device spent propagating
+        unsigned long long ticks = 0;
+        if (hipSetDevice(sv->ctx->device) != hipSuccess ||
+            hipMemcpyAsync(&ticks, sv->d_claim + 1, sizeof ticks, hipMemcpyDeviceToHost, sv->upload) != hipSuccess ||
+            hipStreamSynchronize(sv->upload) != hipSuccess)
+            return NT_ERR_DEVICE;
+        out->busy_ms = (double)ticks * 1e-5 / (double)((int64_t)sv->srv_blocks * (kThreads / 64));
+    }
     return sv->fatal;
 }
 
@@ -2362,12 +3104,16 @@ void nt_server_destroy(nt_server* sv) {
     {
         std::lock_guard<std::mutex> lk(sv->mu);
         sv->stop = true;
+        if (sv->ctrl) __atomic_store_n(&sv->ctrl->stop, 1u, __ATOMIC_RELEASE);   // wavefronts leave at their next claim
         sv->cv_work.notify_all();
     }
     if (sv->worker.joinable()) sv->worker.join();
     (void)hipSetDevice(sv->ctx->device);
     (void)hipStreamSynchronize(sv->stream);
     (void)hipStreamSynchronize(sv->copy_stream);
+    for (int i = 0; i < 2; ++i)
+        if (sv->kstream[i]) (void)hipStreamSynchronize(sv->kstream[i]);
+    if (sv->upload) (void)hipStreamSynchronize(sv->upload);
     free_server(sv);
 }
 

@@ -26,13 +26,22 @@ from .propagation import PhotonHits, PropagationEngine
 
 
 class PhotonServer:
+    """mode "worklist": a persistent kernel pulls photon tickets of the admitted
+    requests from a device work list (requests join at once); "rounds": the
+    scheduler thread packs the requests into one launch per round of at most
+    `round_photons` photons (`max_inflight_rounds` queued ahead)."""
+
     def __init__(self, engine: PropagationEngine, device_budget: int = 8 << 30,
                  host_budget: int = 4 << 30, round_photons: int = 20_000_000,
-                 max_inflight_rounds: int = 2):
+                 max_inflight_rounds: int = 2, mode: str = "worklist"):
+        if mode not in ("worklist", "rounds"):
+            raise ValueError("mode must be 'worklist' or 'rounds'")
         self.engine = engine
         self.lib = engine.lib
+        self.mode = mode
         cfg = _abi.nt_server_config(int(device_budget), int(host_budget), int(round_photons),
-                                    int(max_inflight_rounds), 0)
+                                    int(max_inflight_rounds),
+                                    _abi.NT_SERVER_WORKLIST if mode == "worklist" else _abi.NT_SERVER_ROUNDS)
         h = C.c_void_p()
         _abi.check(self.lib.nt_server_create(engine.handle, C.byref(cfg), C.byref(h)), "nt_server_create")
         self.handle = h
@@ -55,12 +64,21 @@ class PhotonServer:
         except Exception:
             pass
 
-    def submit(self, sources, seed: int, capacity: int) -> int:
+    def submit(self, sources, seed: int, capacity: int, epilogue_position=None) -> int:
+        """`epilogue_position` (work-list mode, after `engine.set_epilogue`): the
+        hits also get pmt_id, total_weight and tres, the time residual measured
+        from this position."""
         arr = _abi.as_c_array(_abi.nt_source, [s.to_abi() for s in sources])
         rid = C.c_uint64()
-        _abi.check(self.lib.nt_server_submit(self.handle, arr, len(sources), C.c_uint64(seed & (2**64 - 1)),
-                                             int(capacity), C.byref(rid)), "nt_server_submit")
-        self._capacity[rid.value] = int(capacity)
+        if epilogue_position is None:
+            _abi.check(self.lib.nt_server_submit(self.handle, arr, len(sources), C.c_uint64(seed & (2**64 - 1)),
+                                                 int(capacity), C.byref(rid)), "nt_server_submit")
+        else:
+            ref = (C.c_float * 3)(*[float(v) for v in np.asarray(epilogue_position, np.float32)])
+            _abi.check(self.lib.nt_server_submit_epilogue(self.handle, arr, len(sources),
+                                                          C.c_uint64(seed & (2**64 - 1)), int(capacity), ref,
+                                                          C.byref(rid)), "nt_server_submit_epilogue")
+        self._capacity[rid.value] = (int(capacity), epilogue_position is not None)
         return rid.value
 
     def poll(self, rid: int) -> bool:
@@ -71,8 +89,8 @@ class PhotonServer:
         return True
 
     def wait(self, rid: int, raise_on_overflow: bool = True):
-        cap = self._capacity.pop(rid)
-        hits = PhotonHits.empty(max(cap, 1), epilogue=False)
+        cap, epi = self._capacity.pop(rid)
+        hits = PhotonHits.empty(max(cap, 1), epilogue=epi)
         h = hits.to_abi()
         nh, ns, nd, lat = C.c_int64(), C.c_int64(), C.c_int64(), C.c_double()
         st = self.lib.nt_server_wait(self.handle, C.c_uint64(rid), C.byref(h), C.byref(nh), C.byref(ns),

@@ -1,6 +1,8 @@
-"""Serving (continuous batching): each request's hits are identical to the
-same sources propagated alone with the same seed, whatever else shares its
-rounds; admission waits for memory; requests that cannot fit are refused."""
+"""Serving (continuous batching), in both modes (the persistent kernel over a
+device work list, and host-scheduled rounds): each request's hits are
+identical to the same sources propagated alone with the same seed, whatever
+else is being served; admission waits for memory; requests that cannot fit
+are refused; served requests can carry the hit epilogue columns."""
 import math
 
 import numpy as np
@@ -30,6 +32,11 @@ def serving_engine():
     eng.close()
 
 
+@pytest.fixture(params=["worklist", "rounds"])
+def mode(request):
+    return request.param
+
+
 def alone(eng, sources, seed, cap):
     eng.set_sources(sources)
     return eng.propagate(seed=seed, capacity=cap).sorted()
@@ -41,7 +48,7 @@ def assert_same(a, b):
         assert np.array_equal(getattr(a, c), getattr(b, c)), c
 
 
-def test_results_identical_alone_and_in_mix(serving_engine):
+def test_results_identical_alone_and_in_mix(serving_engine, mode):
     eng, med = serving_engine
     rng = np.random.default_rng(5)
     reqs = cascade_requests(10, med, rng, radius=60.0, height=200.0, e_range=(1e2, 2e4))
@@ -52,11 +59,13 @@ def test_results_identical_alone_and_in_mix(serving_engine):
                                          np.array([0.0, 0.6, 0.8], np.float32), 0.0, 300_001, lp),
              nt.PointlikeIsotropicEmitter(np.zeros(3, np.float32), 0.0, 0)]
     jobs = [(r.sources, r.seed, r.capacity) for r in reqs] + [(multi, 77, 200_000)]
-    with PhotonServer(eng, device_budget=2 << 30, host_budget=1 << 30, round_photons=400_000) as srv:
+    with PhotonServer(eng, device_budget=2 << 30, host_budget=1 << 30, round_photons=400_000, mode=mode) as srv:
         ids = [srv.submit(s, seed, cap) for s, seed, cap in jobs]
         got = [srv.wait(i) for i in ids]
         stats = srv.stats()
-    assert stats["completed"] == len(jobs) and stats["rounds"] > 1
+    assert stats["completed"] == len(jobs) and stats["rounds"] >= 1 and stats["busy_ms"] > 0
+    if mode == "rounds":
+        assert stats["rounds"] > 1
     total_hits = 0
     for (sources, seed, cap), (h, lat) in zip(jobs, got):
         assert lat > 0
@@ -67,23 +76,23 @@ def test_results_identical_alone_and_in_mix(serving_engine):
     assert total_hits > 100
 
 
-def test_small_budget_admits_in_turn(serving_engine):
+def test_small_budget_admits_in_turn(serving_engine, mode):
     eng, med = serving_engine
     rng = np.random.default_rng(6)
     reqs = cascade_requests(6, med, rng, radius=60.0, height=200.0, e_range=(1e3, 1e4))
     cap = 50_000
     one = 4096 + cap * 62 + 10 * 256                      # one request's hit buffers, rounded up
-    with PhotonServer(eng, device_budget=2 * one, host_budget=2 * one, round_photons=200_000) as srv:
+    with PhotonServer(eng, device_budget=2 * one, host_budget=2 * one, round_photons=200_000, mode=mode) as srv:
         ids = [srv.submit(r.sources, r.seed, cap) for r in reqs]
         out = [srv.wait(i)[0] for i in ids]
     for r, h in zip(reqs, out):
         assert_same(h.sorted(), alone(eng, r.sources, r.seed, cap))
 
 
-def test_refusal_and_overflow(serving_engine):
+def test_refusal_and_overflow(serving_engine, mode):
     eng, med = serving_engine
     src = [nt.PointlikeIsotropicEmitter(np.array([0.0, 0.0, 0.0], np.float32), 0.0, 400_000)]
-    with PhotonServer(eng, device_budget=1 << 20, host_budget=1 << 20, round_photons=100_000) as srv:
+    with PhotonServer(eng, device_budget=1 << 20, host_budget=1 << 20, round_photons=100_000, mode=mode) as srv:
         with pytest.raises(_abi.NTError) as e:
             srv.submit(src, 1, 10_000_000)                  # 620 MB of hit buffers, budget 1 MiB
         assert e.value.status == _abi.NT_ERR_BUDGET
@@ -97,14 +106,51 @@ def test_refusal_and_overflow(serving_engine):
         assert e.value.status == _abi.NT_ERR_HIT_OVERFLOW
 
 
-def test_poisson_stream_latency(serving_engine):
+def test_poisson_stream_latency(serving_engine, mode):
     eng, med = serving_engine
     rng = np.random.default_rng(7)
     reqs = cascade_requests(40, med, rng, e_range=(1e2, 1e4))
-    with PhotonServer(eng, device_budget=4 << 30, host_budget=2 << 30, round_photons=5_000_000) as srv:
+    with PhotonServer(eng, device_budget=4 << 30, host_budget=2 << 30, round_photons=5_000_000, mode=mode) as srv:
         res = run_poisson_stream(srv, reqs, rate_hz=400.0, rng=rng)
     assert res["requests"] == 40 and np.all(res["latency_ms"] > 0)
     assert res["p99_ms"] >= res["p50_ms"] > 0
+
+
+def test_served_epilogue_columns_equal_alone(serving_engine):
+    """Work-list requests with the epilogue: PMT id, total weight and time
+    residual (from each request's own reference position) equal the same
+    sources propagated alone with the context's epilogue at that position."""
+    from scipy.spatial.transform import Rotation
+    eng, med = serving_engine
+    rng = np.random.default_rng(9)
+    reqs = cascade_requests(5, med, rng, radius=60.0, height=200.0, e_range=(1e3, 2e4))
+    orient = Rotation.random(random_state=4).as_matrix()
+    # PMTs of 0.5 m^2 on the 2 m spheres cover about 16 % of each module, so a
+    # few hundred hits land on some of them
+    eng.set_targets(cluster_targets(radius=2.0, pmt_area=0.5))
+    eng.set_epilogue(med, orient, (0.0, 0.0, 0.0))
+    try:
+        with PhotonServer(eng, host_budget=1 << 30, mode="worklist") as srv:
+            ids = [srv.submit(r.sources, r.seed, 20 * r.capacity, epilogue_position=r.sources[0].position)
+                   for r in reqs]
+            got = [srv.wait(i)[0] for i in ids]
+        with pytest.raises(_abi.NTError):
+            with PhotonServer(eng, host_budget=1 << 30, mode="rounds") as srv:
+                srv.submit(reqs[0].sources, 1, 100, epilogue_position=(0, 0, 0))
+        total = on_pmt = 0
+        for r, h in zip(reqs, got):
+            eng.set_epilogue(med, orient, r.sources[0].position)
+            ref = alone(eng, r.sources, r.seed, 20 * r.capacity)
+            hs = h.sorted()
+            assert_same(hs, ref)
+            for c in ("pmt_id", "total_weight", "tres"):
+                assert np.array_equal(getattr(hs, c), getattr(ref, c)), c
+            total += len(ref)
+            on_pmt += int(np.sum(ref.pmt_id > 0))
+        assert total > 100 and 0 < on_pmt < total // 2
+    finally:
+        eng.set_epilogue(enabled=False)
+        eng.set_targets(cluster_targets(radius=2.0))
 
 
 def test_full_size_serving_configuration():
