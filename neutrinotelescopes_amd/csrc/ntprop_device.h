@@ -245,9 +245,13 @@ __device__ __forceinline__ float3 sph_to_cart_cos(float ct, float u_phi) {
 // FP32, where √(1 − cos²θ) near cos θ = ±1, the rotation coefficients near
 // a_z = −1 and the track-direction sampler lose up to thousands of FP32 ulps
 // against an FP64 evaluation of the same formulas.  The kernel evaluates the
-// formulas from the FP32 inputs in FP64 (an FP64 fma issues at the rate of an
-// FP32 one on gfx950) and rounds once to FP32; the remaining error is the
-// hardware sine/cosine of the azimuth (≤ 1.3e-7) and that final rounding.
+// formulas from the FP32 inputs in FP64 and rounds once to FP32; the
+// remaining error is the hardware sine/cosine of the azimuth (≤ 1.3e-7) and
+// that final rounding.  Those two already take 1.6 of the 2 ulp(1) allowed,
+// which leaves no room for FP32 roundings in the cosine, sine or rotation.
+// The price: on gfx950 an FP64 fma issues at half the rate of an FP32 one
+// and v_rcp_f64 / v_rsq_f64 at about an eighth, so these functions cost the
+// kernel about a quarter of its time against the FP32 formulas.
 // ---------------------------------------------------------------------------
 // An FP64 constant materialised where it is used (two scalar moves): the
 // persistent kernel's loop would otherwise hoist every FP64 literal into a
