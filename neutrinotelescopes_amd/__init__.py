@@ -22,5 +22,7 @@ from .propagation import (PhotonPropSetup, PhotonHits, PropagationEngine, propag
                           make_hits_from_photons, get_engine)
 
 from . import refflow  # noqa: E402  (the reference flow's input layout and model file)
+from . import pmt_frontend  # noqa: E402  (PMT front end: waveforms and unfolding on the device)
+from .pmt_frontend import PMTConfig, PMTFrontEnd, make_reco_pulses, std_pmt_config  # noqa: E402
 
 __all__ = [n for n in dir() if not n.startswith("_")]

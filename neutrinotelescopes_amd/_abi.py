@@ -195,6 +195,23 @@ class nt_server_stats(C.Structure):
     ]
 
 
+class ntpmt_config(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "tt_shape", "tt_scale", "tt_subtract", "spe_expon_weight", "spe_expon_scale", "spe_norm_mu",
+        "spe_norm_sigma", "spe_trunc_low", "pulse_mu", "pulse_beta", "pulse_lo", "pulse_hi", "pulse_amplitude",
+        "noise_amp", "sampling_freq", "adc_freq", "unf_pulse_res", "time_range", "lead_time", "min_charge",
+        "iir_b0", "iir_b1", "iir_a1", "nnls_tol")] + [
+        ("nnls_max_iter", C.c_int64),
+        ("decim_taps", C.POINTER(C.c_double)), ("n_decim_taps", C.c_int32),
+        ("filt_pulse", C.POINTER(C.c_double)), ("n_filt_pulse", C.c_int32),
+        ("filt_pulse_t0", C.c_double), ("filt_pulse_dt", C.c_double)]
+
+
+class ntpmt_pulses(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in
+                ("module_id", "pmt_id", "n_hits", "n_pulses", "status", "t0", "time", "charge", "waveform")]
+
+
 # exported symbol -> (restype, argtypes)
 SIGNATURES = {
     "nt_abi_version": (C.c_int32, []),
@@ -232,6 +249,14 @@ SIGNATURES = {
                                         C.c_void_p]),
     "nt_build_cascade_sources": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(nt_cascade_model),
                                              C.c_uint64, C.c_void_p, C.c_void_p]),
+    # include/ntpmt.h (PMT front end)
+    "ntpmt_create": (C.c_int32, [C.c_int32, C.POINTER(ntpmt_config), C.POINTER(C.c_void_p)]),
+    "ntpmt_destroy": (None, [C.c_void_p]),
+    "ntpmt_sizes": (C.c_int32, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "ntpmt_make_reco_pulses": (C.c_int32, [C.c_void_p, C.POINTER(nt_hits), C.c_void_p, C.c_int64, C.c_int32,
+                                           C.c_uint64, C.c_int64, C.POINTER(ntpmt_pulses), C.c_void_p, C.c_void_p]),
+    "ntpmt_synchronize": (C.c_int32, [C.c_void_p]),
+    "ntpmt_last_error": (C.c_char_p, []),
     # include/ntflow.h (flow surrogate)
     "ntf_create": (C.c_int32, [C.c_int32, C.POINTER(C.c_void_p)]),
     "ntf_destroy": (None, [C.c_void_p]),
@@ -301,7 +326,10 @@ def load_library(path: Path | None = None) -> C.CDLL:
 class NTError(RuntimeError):
     def __init__(self, status: int, where: str):
         lib = load_library()
-        msg = lib.nt_status_string(status).decode(errors="replace")
+        if where.startswith("ntpmt_") and status != NT_OK:
+            msg = lib.ntpmt_last_error().decode(errors="replace")
+        else:
+            msg = lib.nt_status_string(status).decode(errors="replace")
         super().__init__(f"{where}: status {status} ({msg})")
         self.status = status
 

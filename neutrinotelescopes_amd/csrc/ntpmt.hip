@@ -1,0 +1,904 @@
+// ntpmt.hip — the PMT front end on the device (C ABI: include/ntpmt.h).
+//
+// make_reco_pulses (src/pmt_frontend/pipeline.jl:146-160) for every
+// (module, PMT) channel of an event in one call:
+//   1. hit_draws_kernel: per selected hit, the transit-time Gamma draw and the
+//      SPE charge (FP64), and a sort key (channel, FP32 time);
+//   2. one radix sort of the keys, a run-length encoding of the channels and
+//      an exclusive scan of the run lengths (hipCUB): every channel's hits are
+//      contiguous and in time order;
+//   3. channel_kernel: one 64-lane workgroup per channel, taken from an atomic
+//      work counter (bright channels cost more).  In LDS: the waveform
+//      (n_wave FP64 samples, each a sum over the hits within the pulse's
+//      support, found by binary search in the sorted keys, plus noise), the
+//      first-order low-pass as a parallel scan of affine maps over the lanes,
+//      the decimation by R (one FIR phase, FP64), then the unfolding.
+// The unfolding is the fnnls active-set method (Bro & de Jong 1997) in FP64
+// on the implicit matrix A[k, i] = filtered pulse at t_k − τ_i: with
+// τ_i = t0 + i·res and Q = (1/adc_freq)/res an integer, column i = Q·q + p is
+// pattern p shifted down by q rows, so A is J·Q numbers (a 16 × 40 table for
+// the standard configuration) kept in LDS.  The active set's Gram matrix
+// A_Pᵀ A_P is kept as a Cholesky factor (packed, LDS; the waveform's space)
+// that grows by one row when a column enters and is recomputed from the
+// first removed position when columns leave; solves are lane-parallel
+// column sweeps.  Per outer iteration the gradient Aᵀ(b − A x) over all n_pulse
+// columns is one pass of J fused multiply-adds per column.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "ntpmt.h"
+#include "ntprop_device.h"
+
+struct ntpmt_ctx;
+
+namespace ntpmt {
+namespace {
+
+using ntp::philox;
+using ntp::u53;
+
+constexpr uint32_t kTtWord = 0x72000000u;
+constexpr uint32_t kSpeWord = 0x73000000u;
+constexpr uint32_t kNoiseWord = 0x74000000u;
+constexpr int kTtTrials = 64;
+constexpr int kSpeTrials = 16;
+constexpr uint32_t kNoChannel = (1u << 21) - 1;      // key of hits that are not selected
+constexpr int kLanes = 64;
+constexpr int kMaxPos = 2 * kLanes;                  // active-set positions per workgroup (≥ NTPMT_MAX_ADC)
+constexpr double kDegenerate = 1e-14;                // a column with d² ≤ this · ‖a‖² is numerically dependent
+
+struct DevCfg {
+    double tt_d, tt_c, tt_scale, tt_subtract;
+    double spe_w, spe_scale, spe_mu, spe_sigma, spe_low;
+    double p_mu, p_inv_beta, p_lo, p_hi, p_norm;
+    double noise_amp, dt_wave, dt_adc, res, lead, min_charge;
+    double b0, b1, a1, tol;
+    int64_t max_iter;
+    int n_wave, n_adc, n_pulse, R, Q, J, m0, n_taps, chunk, max_outer;
+    const double* taps;      // device [n_taps]
+    const double* table;     // device [J][Q]
+};
+
+struct HitIn {
+    const float* time;
+    const uint16_t* module_id;
+    const uint8_t* pmt_id;
+    const uint32_t* source_index;
+    const uint64_t* photon_id;
+    const uint8_t* select;
+    int64_t n;
+    uint64_t seed;
+};
+
+struct ChanArgs {
+    const uint64_t* key;       // sorted keys
+    const int32_t* idx;        // hit index of each sorted key
+    const double* tq;          // [n][2] (t', charge) by hit index
+    const uint8_t* hflag;      // per hit status bits
+    const uint32_t* unique;    // channel of each run
+    const int32_t* counts;     // run lengths
+    const int32_t* offsets;    // run starts
+    const int32_t* n_runs;
+    int32_t* next;             // work counter
+    int64_t capacity;
+    uint64_t seed;
+    ntpmt_pulses out;
+};
+
+// ---------------------------------------------------------------- RNG ---
+__device__ __forceinline__ void normals(uint4 r, double& n0, double& n1) {
+    const double u1 = 1.0 - u53(r.x, r.y);          // (0, 1]
+    const double u2 = u53(r.z, r.w);
+    const double rad = sqrt(-2.0 * log(u1));
+    double s, c;
+    sincospi(2.0 * u2, &s, &c);
+    n0 = rad * c;
+    n1 = rad * s;
+}
+
+// Gamma(α, θ): Marsaglia & Tsang (2000), no squeeze; d = α − 1/3, c = 1/√(9d)
+__device__ double tt_draw(const DevCfg& c, uint32_t k0, uint32_t k1, uint32_t plo, uint32_t phi, uint32_t src,
+                          uint8_t& flag) {
+    for (int t = 0; t < kTtTrials; ++t) {
+        double x, unused;
+        normals(philox(make_uint4(kTtWord + 2u * t, plo, phi, src), k0, k1), x, unused);
+        double v = 1.0 + c.tt_c * x;
+        if (!(v > 0.0)) continue;
+        v = v * v * v;
+        const uint4 a = philox(make_uint4(kTtWord + 2u * t + 1u, plo, phi, src), k0, k1);
+        const double u = 1.0 - u53(a.x, a.y);
+        if (log(u) < 0.5 * x * x + c.tt_d - c.tt_d * v + c.tt_d * log(v)) return c.tt_d * v * c.tt_scale;
+    }
+    flag |= NTPMT_ST_TT_GIVEUP;
+    return (c.tt_d + 1.0 / 3.0) * c.tt_scale;
+}
+
+// SPE charge: exponential with probability w, else Normal(mu, sigma) truncated to [low, ∞)
+__device__ double spe_draw(const DevCfg& c, uint32_t k0, uint32_t k1, uint32_t plo, uint32_t phi, uint32_t src,
+                           uint8_t& flag) {
+    const uint4 r = philox(make_uint4(kSpeWord, plo, phi, src), k0, k1);
+    if (u53(r.x, r.y) < c.spe_w) return -log(1.0 - u53(r.z, r.w)) * c.spe_scale;
+    for (int t = 0; t < kSpeTrials; ++t) {
+        double n0, n1;
+        normals(philox(make_uint4(kSpeWord + 1u + t, plo, phi, src), k0, k1), n0, n1);
+        const double v0 = c.spe_mu + c.spe_sigma * n0;
+        if (v0 >= c.spe_low) return v0;
+        const double v1 = c.spe_mu + c.spe_sigma * n1;
+        if (v1 >= c.spe_low) return v1;
+    }
+    flag |= NTPMT_ST_SPE_GIVEUP;
+    return c.spe_mu;
+}
+
+// ascending order of a float as ascending order of a uint32
+__device__ __forceinline__ uint32_t ordered32(float f) {
+    const uint32_t b = __float_as_uint(f);
+    return (b >> 31) ? ~b : (b | 0x80000000u);
+}
+
+__global__ __launch_bounds__(256) void hit_draws_kernel(HitIn h, DevCfg c, uint64_t* key, int32_t* idx, double* tq,
+                                                        uint8_t* hflag) {
+    const uint32_t k0 = (uint32_t)h.seed, k1 = (uint32_t)(h.seed >> 32);
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < h.n; i += (int64_t)gridDim.x * 256) {
+        uint64_t k = ((uint64_t)kNoChannel << 32) | 0xFFFFFFFFull;
+        if (!h.select || h.select[i]) {
+            const uint64_t pid = h.photon_id[i];
+            const uint32_t plo = (uint32_t)pid, phi = (uint32_t)(pid >> 32), src = h.source_index[i];
+            uint8_t f = 0;
+            const double tt = tt_draw(c, k0, k1, plo, phi, src, f);
+            const double tp = ((double)h.time[i] + tt) - c.tt_subtract;     // apply_tt!, subtract_mean_tt!
+            const double q = spe_draw(c, k0, k1, plo, phi, src, f);
+            const uint32_t ch = (uint32_t)h.module_id[i] * 17u + (uint32_t)min((int)h.pmt_id[i], 16);
+            k = ((uint64_t)ch << 32) | ordered32((float)tp);
+            tq[2 * i] = tp;
+            tq[2 * i + 1] = q;
+            hflag[i] = f;
+        }
+        key[i] = k;
+        idx[i] = (int32_t)i;
+    }
+}
+
+__global__ void channel_ids_kernel(const uint64_t* __restrict__ key, int64_t n, uint32_t* ch) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        ch[i] = (uint32_t)(key[i] >> 32);
+}
+
+// count of channels (the run of unselected hits, if any, is last) and the work counter
+__global__ void finish_count_kernel(const int32_t* n_runs, const uint32_t* unique, int64_t* n_channels,
+                                    int32_t* next) {
+    const int32_t r = *n_runs;
+    const int64_t nc = (r > 0 && unique[r - 1] == kNoChannel) ? r - 1 : r;
+    if (n_channels) *n_channels = nc;
+    *next = 0;
+}
+
+__global__ void zero_count_kernel(int64_t* n_channels) {
+    if (n_channels) *n_channels = 0;
+}
+
+// ------------------------------------------------------------- wave ops ---
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+    for (int o = 32; o; o >>= 1) v += __shfl_xor(v, o);
+    return v;                                 // a butterfly: every lane holds the same sum
+}
+__device__ __forceinline__ double wmin(double v) {
+#pragma unroll
+    for (int o = 32; o; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ bool wany(bool b) { return __ballot(b) != 0; }
+// value of position j (uniform) from the two per-lane registers
+__device__ __forceinline__ double getp(double a, double b, int j) {
+    return j < kLanes ? __shfl(a, j) : __shfl(b, j - kLanes);
+}
+__device__ __forceinline__ int getpi(int a, int b, int j) {
+    return j < kLanes ? __shfl(a, j) : __shfl(b, j - kLanes);
+}
+__device__ __forceinline__ int tri(int i) { return i * (i + 1) / 2; }
+
+struct Lds {
+    double* T;          // [J][Q]
+    double* buf;        // waveform [n_wave], then the Cholesky factor (packed rows)
+    double* b;          // [n_adc] digitised waveform
+    double* r;          // [n_adc] residual b − A x
+    uint32_t* inP;      // [n_pulse / 32] active set
+    uint32_t* excl;     // [n_pulse / 32] columns skipped as dependent
+    int* mv_col;        // [kMaxPos] compaction staging
+    double* mv_x;       // [kMaxPos]
+    double* mv_atb;     // [kMaxPos]
+};
+
+__device__ __forceinline__ double acol(const Lds& s, const DevCfg& c, int col, int k) {
+    const int q = col / c.Q, p = col - q * c.Q, j = k - q - c.m0;
+    return (j >= 0 && j < c.J) ? s.T[j * c.Q + p] : 0.0;
+}
+
+// a_colᵀ v over the rows the column reaches (k ascending)
+__device__ __forceinline__ double adot(const Lds& s, const DevCfg& c, int col, const double* v) {
+    const int q = col / c.Q, p = col - q * c.Q;
+    double acc = 0.0;
+    for (int j = 0; j < c.J; ++j) {
+        const int k = q + c.m0 + j;
+        if (k >= 0 && k < c.n_adc) acc += s.T[j * c.Q + p] * v[k];
+    }
+    return acc;
+}
+
+// a_ciᵀ a_cj
+__device__ __forceinline__ double gram(const Lds& s, const DevCfg& c, int ci, int cj) {
+    const int qi = ci / c.Q, pi = ci - qi * c.Q, qj = cj / c.Q, pj = cj - qj * c.Q;
+    double acc = 0.0;
+    for (int ji = 0; ji < c.J; ++ji) {
+        const int k = qi + c.m0 + ji, jj = ji + qi - qj;
+        if (k >= 0 && k < c.n_adc && jj >= 0 && jj < c.J) acc += s.T[ji * c.Q + pi] * s.T[jj * c.Q + pj];
+    }
+    return acc;
+}
+
+__device__ __forceinline__ bool bit(const uint32_t* m, int i) { return (m[i >> 5] >> (i & 31)) & 1u; }
+
+// Active set: positions 0 … P−1, position j in lane j mod 64, register A (j < 64) or B.
+struct Active {
+    int P = 0;
+    int colA = 0, colB = 0;
+    double xA = 0, xB = 0, sA = 0, sB = 0, zA = 0, zB = 0, atbA = 0, atbB = 0;
+};
+
+// Append column `col` as position t = a.P − (already counted) : extends the
+// Cholesky factor by row t and z by z_t.  Returns false (nothing written) if
+// the column is numerically dependent on positions 0 … t−1.
+__device__ bool chol_append(const Lds& s, const DevCfg& c, Active& a, int t, int col, double atb_col, int lane) {
+    double gA = (lane < t) ? gram(s, c, a.colA, col) : 0.0;
+    double gB = (lane + kLanes < t) ? gram(s, c, a.colB, col) : 0.0;
+    double lA = 0.0, lB = 0.0;
+    for (int j = 0; j < t; ++j) {
+        const double ljj = s.buf[tri(j) + j];
+        const double lj = getp(gA, gB, j) / ljj;
+        if (j < kLanes) {
+            if (lane == j) lA = lj;
+        } else if (lane == j - kLanes) {
+            lB = lj;
+        }
+        if (lane > j && lane < t) gA -= s.buf[tri(lane) + j] * lj;
+        if (lane + kLanes > j && lane + kLanes < t) gB -= s.buf[tri(lane + kLanes) + j] * lj;
+    }
+    const double gcc = gram(s, c, col, col);
+    const double d2 = gcc - wsum(lA * lA + lB * lB);
+    if (!(d2 > kDegenerate * gcc)) return false;
+    const double d = sqrt(d2);
+    const double zt = (atb_col - wsum((lane < t ? lA * a.zA : 0.0) + (lane + kLanes < t ? lB * a.zB : 0.0))) / d;
+    if (lane < t) s.buf[tri(t) + lane] = lA;
+    if (lane + kLanes < t) s.buf[tri(t) + lane + kLanes] = lB;
+    if (lane == 0) s.buf[tri(t) + t] = d;
+    if (t < kLanes) {
+        if (lane == t) { a.colA = col; a.zA = zt; a.atbA = atb_col; }
+    } else if (lane == t - kLanes) {
+        a.colB = col; a.zB = zt; a.atbB = atb_col;
+    }
+    __syncthreads();
+    return true;
+}
+
+// s = L⁻ᵀ z over positions 0 … P−1 (0 beyond)
+__device__ void back_solve(const Lds& s, Active& a, int lane) {
+    double zA = a.zA, zB = a.zB;
+    a.sA = 0.0;
+    a.sB = 0.0;
+    for (int j = a.P - 1; j >= 0; --j) {
+        const double sj = getp(zA, zB, j) / s.buf[tri(j) + j];
+        if (j < kLanes) {
+            if (lane == j) a.sA = sj;
+        } else if (lane == j - kLanes) {
+            a.sB = sj;
+        }
+        if (lane < j) zA -= s.buf[tri(j) + lane] * sj;
+        if (lane + kLanes < j) zB -= s.buf[tri(j) + lane + kLanes] * sj;
+    }
+}
+
+// r = b − A x (rows in uniform blocks of 64: the shuffles run on every lane)
+__device__ void residual(const Lds& s, const DevCfg& c, const Active& a, int lane) {
+    for (int k0 = 0; k0 < c.n_adc; k0 += kLanes) {
+        const int k = k0 + lane;
+        const bool row = k < c.n_adc;
+        double acc = row ? s.b[k] : 0.0;
+        for (int j = 0; j < a.P; ++j) {
+            const int col = getpi(a.colA, a.colB, j);
+            const double xj = getp(a.xA, a.xB, j);
+            if (row) acc -= acol(s, c, col, k) * xj;
+        }
+        if (row) s.r[k] = acc;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(64) void channel_kernel(ChanArgs g, DevCfg c) {
+    extern __shared__ double smem[];
+    const int lane = threadIdx.x;
+    const int nw32 = (c.n_pulse + 31) / 32;
+    Lds s;
+    s.T = smem;
+    s.buf = s.T + c.J * c.Q;
+    const int nbuf = max(c.n_wave, tri(c.n_adc));
+    s.b = s.buf + nbuf;
+    s.r = s.b + c.n_adc;
+    s.mv_x = s.r + c.n_adc;
+    s.mv_atb = s.mv_x + kMaxPos;
+    s.inP = (uint32_t*)(s.mv_atb + kMaxPos);
+    s.excl = s.inP + nw32;
+    s.mv_col = (int*)(s.excl + nw32);
+    for (int i = lane; i < c.J * c.Q; i += kLanes) s.T[i] = c.table[i];
+    __syncthreads();
+    const uint32_t k0 = (uint32_t)g.seed, k1 = (uint32_t)(g.seed >> 32);
+    const int n_runs = *g.n_runs;
+
+    for (;;) {
+        int ci = 0;
+        if (lane == 0) ci = atomicAdd(g.next, 1);
+        ci = __shfl(ci, 0);
+        if (ci >= n_runs) break;
+        const uint32_t ch = g.unique[ci];
+        if (ch == kNoChannel || ci >= g.capacity) continue;
+        const int beg = g.offsets[ci], n = g.counts[ci];
+        int status = 0;
+
+        // ---- first pulse time, status of the hit draws
+        double tmin = INFINITY;
+        for (int p = lane; p < n; p += kLanes) {
+            const int h = g.idx[beg + p];
+            tmin = fmin(tmin, g.tq[2 * h]);
+            status |= g.hflag[h];
+        }
+        tmin = wmin(tmin);
+        for (int o = 32; o; o >>= 1) status |= __shfl_xor(status, o);
+        const double t0 = tmin - c.lead;
+
+        // ---- make_waveform: Σ_j q_j · pulse(t_i − t'_j) + noise
+        for (int g0 = 0; g0 < c.n_wave; g0 += kLanes) {
+            const int i = g0 + lane;
+            const double ts = t0 + (double)i * c.dt_wave;
+            // hits with t' in [t_first − p_hi, t_last − p_lo] of this group's samples
+            const double lo = (t0 + (double)g0 * c.dt_wave) - c.p_hi;
+            const double hi = (t0 + (double)(g0 + kLanes - 1) * c.dt_wave) - c.p_lo;
+            const uint32_t olo = ordered32((float)lo), ohi = ordered32((float)hi);
+            int a = 0, bnd = n;                            // first key ≥ lo
+            while (a < bnd) {
+                const int m = (a + bnd) >> 1;
+                if ((uint32_t)g.key[beg + m] < olo) a = m + 1; else bnd = m;
+            }
+            int e = a, e1 = n;                             // first key > hi
+            while (e < e1) {
+                const int m = (e + e1) >> 1;
+                if ((uint32_t)g.key[beg + m] <= ohi) e = m + 1; else e1 = m;
+            }
+            double acc = 0.0;
+            for (int p = a; p < e; ++p) {
+                const int h = g.idx[beg + p];
+                const double x = ts - g.tq[2 * h];
+                if (x >= c.p_lo && x <= c.p_hi) {
+                    const double z = (x - c.p_mu) * c.p_inv_beta;
+                    acc += c.p_norm * exp(-(z + exp(-z))) * g.tq[2 * h + 1];
+                }
+            }
+            if (c.noise_amp > 0.0) {
+                double n0, n1;
+                normals(philox(make_uint4(kNoiseWord, ch, (uint32_t)(i >> 1), 0u), k0, k1), n0, n1);
+                acc += ((i & 1) ? n1 : n0) * c.noise_amp;
+            }
+            if (i < c.n_wave) s.buf[i] = acc;
+        }
+        __syncthreads();
+
+        // ---- low-pass y_i = b0 v_i + (b1 v_{i−1} − a1 y_{i−1}): chunk per lane, affine-map scan
+        {
+            const int st = lane * c.chunk;
+            const int len = max(0, min(c.chunk, c.n_wave - st));
+            const double xprev = (st > 0 && st <= c.n_wave) ? s.buf[st - 1] : 0.0;
+            double y = 0.0, xp = xprev, A = 1.0;
+            for (int k = 0; k < len; ++k) {
+                const double x = s.buf[st + k];
+                y = c.b0 * x + (c.b1 * xp - c.a1 * y);
+                xp = x;
+                A *= -c.a1;
+            }
+            double B = y;
+            for (int o = 1; o < kLanes; o <<= 1) {          // inclusive scan of (A, B) ∘ …
+                const double Ao = __shfl_up(A, o), Bo = __shfl_up(B, o);
+                if (lane >= o) {
+                    B = A * Bo + B;
+                    A = A * Ao;
+                }
+            }
+            double Y = __shfl_up(B, 1);
+            if (lane == 0) Y = 0.0;
+            __syncthreads();                               // every lane has read its x_{st−1}
+            y = Y;
+            xp = xprev;
+            for (int k = 0; k < len; ++k) {
+                const double x = s.buf[st + k];
+                y = c.b0 * x + (c.b1 * xp - c.a1 * y);
+                xp = x;
+                s.buf[st + k] = y;
+            }
+        }
+        __syncthreads();
+
+        // ---- resample by 1/R: digi_k = Σ_m h_m y_{R·k + half − m}
+        {
+            const int half = (c.n_taps - 1) / 2;
+            for (int k = lane; k < c.n_adc; k += kLanes) {
+                double acc = 0.0;
+                for (int m = 0; m < c.n_taps; ++m) {
+                    const int j = c.R * k + half - m;
+                    if (j >= 0 && j < c.n_wave) acc += c.taps[m] * s.buf[j];
+                }
+                s.b[k] = acc;
+                s.r[k] = acc;
+                if (g.out.waveform) g.out.waveform[(int64_t)ci * c.n_adc + k] = acc;
+            }
+            for (int i = lane; i < 2 * nw32; i += kLanes) s.inP[i] = 0u;   // inP and excl
+        }
+        __syncthreads();
+
+        // ---- fnnls
+        Active a;
+        int64_t iter = 0;
+        int outer = 0;
+        for (;;) {
+            double best = -INFINITY;
+            int bi = 0x7fffffff;
+            for (int col = lane; col < c.n_pulse; col += kLanes) {
+                if (bit(s.inP, col) || bit(s.excl, col)) continue;
+                const double w = adot(s, c, col, s.r);
+                if (w > best) { best = w; bi = col; }
+            }
+            for (int o = 32; o; o >>= 1) {                 // max, first index on ties
+                const double bo = __shfl_xor(best, o);
+                const int io = __shfl_xor(bi, o);
+                if (bo > best || (bo == best && io < bi)) { best = bo; bi = io; }
+            }
+            if (!(best > c.tol) || iter >= c.max_iter || a.P >= c.n_pulse) break;
+            if (++outer > c.max_outer) { status |= NTPMT_ST_ITER_CAP; break; }
+            if (a.P >= c.n_adc) { status |= NTPMT_ST_RANK_CAP; break; }
+            const double atb_col = adot(s, c, bi, s.b);
+            if (!chol_append(s, c, a, a.P, bi, atb_col, lane)) {
+                if (lane == 0) s.excl[bi >> 5] |= 1u << (bi & 31);
+                status |= NTPMT_ST_DEGENERATE;
+                __syncthreads();
+                continue;
+            }
+            if (a.P < kLanes) { if (lane == a.P) a.xA = 0.0; } else if (lane == a.P - kLanes) a.xB = 0.0;
+            if (lane == 0) s.inP[bi >> 5] |= 1u << (bi & 31);
+            a.P += 1;
+            __syncthreads();
+            back_solve(s, a, lane);
+            for (;;) {
+                const bool vA = lane < a.P, vB = lane + kLanes < a.P;
+                if (!wany((vA && a.sA <= c.tol) || (vB && a.sB <= c.tol))) break;
+                ++iter;
+                double al = INFINITY;
+                if (vA && a.sA <= c.tol) al = fmin(al, a.xA / (a.xA - a.sA));
+                if (vB && a.sB <= c.tol) al = fmin(al, a.xB / (a.xB - a.sB));
+                al = wmin(al);
+                if (!(al < INFINITY)) al = 0.0;            // only 0/0 quotients: keep x, drop x ≤ tol
+                if (vA) a.xA += al * (a.sA - a.xA);
+                if (vB) a.xB += al * (a.sB - a.xB);
+                // drop positions with x ≤ tol, keep the order of the rest
+                const bool kA = vA && a.xA > c.tol, kB = vB && a.xB > c.tol;
+                const uint64_t mA = __ballot(kA), mB = __ballot(kB);
+                const uint64_t vmA = __ballot(vA), vmB = __ballot(vB);
+                const int nA = __popcll(mA);
+                const int newP = nA + __popcll(mB);
+                if (newP == a.P) { status |= NTPMT_ST_ITER_CAP; break; }    // no progress: cannot happen exactly
+                // first dropped position: factor rows before it stay valid
+                const uint64_t dA = vmA & ~mA, dB = vmB & ~mB;
+                const int first = dA ? __builtin_ctzll(dA) : kLanes + __builtin_ctzll(dB);
+                if (vA && !kA) atomicAnd(&s.inP[a.colA >> 5], ~(1u << (a.colA & 31)));
+                if (vB && !kB) atomicAnd(&s.inP[a.colB >> 5], ~(1u << (a.colB & 31)));
+                const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+                if (kA) {
+                    const int np = __popcll(mA & below);
+                    s.mv_col[np] = a.colA; s.mv_x[np] = a.xA; s.mv_atb[np] = a.atbA;
+                }
+                if (kB) {
+                    const int np = nA + __popcll(mB & below);
+                    s.mv_col[np] = a.colB; s.mv_x[np] = a.xB; s.mv_atb[np] = a.atbB;
+                }
+                __syncthreads();
+                a.P = newP;
+                a.colA = lane < newP ? s.mv_col[lane] : 0;
+                a.xA = lane < newP ? s.mv_x[lane] : 0.0;
+                a.atbA = lane < newP ? s.mv_atb[lane] : 0.0;
+                a.colB = lane + kLanes < newP ? s.mv_col[lane + kLanes] : 0;
+                a.xB = lane + kLanes < newP ? s.mv_x[lane + kLanes] : 0.0;
+                a.atbB = lane + kLanes < newP ? s.mv_atb[lane + kLanes] : 0.0;
+                __syncthreads();
+                // rows 0 … first−1 of the factor and z are unchanged; rebuild the rest
+                int t = first;
+                for (; t < newP; ++t) {
+                    const int col = getpi(a.colA, a.colB, t);
+                    const double atb = getp(a.atbA, a.atbB, t);
+                    if (!chol_append(s, c, a, t, col, atb, lane)) break;
+                }
+                if (t < newP) {                            // a kept column became dependent: drop it and the rest
+                    status |= NTPMT_ST_DEGENERATE;
+                    if (lane >= t && lane < newP) { atomicAnd(&s.inP[a.colA >> 5], ~(1u << (a.colA & 31))); a.xA = 0.0; }
+                    if (lane + kLanes >= t && lane + kLanes < newP) {
+                        atomicAnd(&s.inP[a.colB >> 5], ~(1u << (a.colB & 31)));
+                        a.xB = 0.0;
+                    }
+                    __syncthreads();
+                    a.P = t;
+                }
+                back_solve(s, a, lane);
+            }
+            // x = s
+            a.xA = lane < a.P ? a.sA : 0.0;
+            a.xB = lane + kLanes < a.P ? a.sB : 0.0;
+            residual(s, c, a, lane);
+        }
+
+        // ---- pulses with charge > min_charge, in time (column) order
+        {
+            const bool oA = lane < a.P && a.xA > c.min_charge, oB = lane + kLanes < a.P && a.xB > c.min_charge;
+            int rA = 0, rB = 0;
+            for (int j = 0; j < a.P; ++j) {
+                const int col = getpi(a.colA, a.colB, j);
+                const double xj = getp(a.xA, a.xB, j);
+                if (xj > c.min_charge) {
+                    rA += col < a.colA;
+                    rB += col < a.colB;
+                }
+            }
+            const int64_t base = (int64_t)ci * c.n_adc;
+            if (oA) {
+                g.out.time[base + rA] = t0 + (double)a.colA * c.res;
+                g.out.charge[base + rA] = a.xA;
+            }
+            if (oB) {
+                g.out.time[base + rB] = t0 + (double)a.colB * c.res;
+                g.out.charge[base + rB] = a.xB;
+            }
+            const int npulse = __popcll(__ballot(oA)) + __popcll(__ballot(oB));
+            if (lane == 0) {
+                g.out.module_id[ci] = (uint16_t)(ch / 17u);
+                g.out.pmt_id[ci] = (uint8_t)(ch % 17u);
+                g.out.n_hits[ci] = n;
+                g.out.n_pulses[ci] = npulse;
+                g.out.status[ci] = status;
+                g.out.t0[ci] = t0;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+thread_local char g_err[256];
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace
+}  // namespace ntpmt
+
+using namespace ntpmt;
+
+struct ntpmt_ctx {
+    int device = 0;
+    int num_cus = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    DevCfg cfg{};
+    double* d_taps = nullptr;
+    double* d_table = nullptr;
+    size_t lds_bytes = 0;
+    int blocks = 0;
+    char* d_ws = nullptr;
+    size_t ws_cap = 0;
+    hipEvent_t last = nullptr;
+};
+
+namespace {
+
+int32_t fail(hipError_t e, const char* what) {
+    snprintf(ntpmt::g_err, sizeof ntpmt::g_err, "%s: %s", what, hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? NT_ERR_OOM : NT_ERR_DEVICE;
+}
+
+#define PMT_TRY(x)                                      \
+    do {                                                \
+        hipError_t _e = (x);                            \
+        if (_e != hipSuccess) return fail(_e, #x);      \
+    } while (0)
+
+bool near_int(double v, int64_t* out) {
+    const double r = std::nearbyint(v);
+    if (!(std::fabs(v - r) <= 1e-9 * std::fmax(1.0, std::fabs(v))) || r < 1.0) return false;
+    *out = (int64_t)r;
+    return true;
+}
+
+int32_t arg_error(const char* msg) {
+    snprintf(ntpmt::g_err, sizeof ntpmt::g_err, "ntpmt: %s", msg);
+    return NT_ERR_ARG;
+}
+
+// linear interpolation of the filtered pulse, 0 outside (the oracle's formula)
+double filt_pulse_at(const ntpmt_config* c, double s) {
+    const int n = c->n_filt_pulse;
+    const double u = (s - c->filt_pulse_t0) / c->filt_pulse_dt;
+    if (!(u >= 0.0 && u <= (double)(n - 1))) return 0.0;
+    const int i = (int)std::floor(u);
+    if (i >= n - 1) return c->filt_pulse[n - 1];
+    const double f = u - i;
+    return c->filt_pulse[i] * (1.0 - f) + c->filt_pulse[i + 1] * f;
+}
+
+struct WsLayout {
+    size_t key_a, key_b, idx_a, idx_b, tq, hflag, ch, unique, counts, offsets, n_runs, next, cub, cub_bytes, total;
+};
+
+WsLayout ws_layout(int64_t n) {
+    WsLayout L{};
+    const size_t nn = (size_t)std::max<int64_t>(n, 1);
+    size_t o = 0;
+    auto take = [&](size_t bytes) { const size_t at = o; o += align256(bytes); return at; };
+    L.key_a = take(8 * nn);
+    L.key_b = take(8 * nn);
+    L.idx_a = take(4 * nn);
+    L.idx_b = take(4 * nn);
+    L.tq = take(16 * nn);
+    L.hflag = take(nn);
+    L.ch = take(4 * nn);
+    L.unique = take(4 * nn);
+    L.counts = take(4 * nn);
+    L.offsets = take(4 * nn);
+    L.n_runs = take(8);
+    L.next = take(8);
+    size_t t1 = 0, t2 = 0, t3 = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, t1, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                             (const int32_t*)nullptr, (int32_t*)nullptr, (int)nn, 0, 53);
+    (void)hipcub::DeviceRunLengthEncode::Encode(nullptr, t2, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                (int32_t*)nullptr, (int32_t*)nullptr, (int)nn);
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, t3, (const int32_t*)nullptr, (int32_t*)nullptr, (int)nn);
+    L.cub_bytes = std::max(t1, std::max(t2, t3));
+    L.cub = take(L.cub_bytes);
+    L.total = o;
+    return L;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t ntpmt_create(int32_t device, const ntpmt_config* c, ntpmt_ctx** out) {
+    if (!out) return arg_error("null output handle");
+    *out = nullptr;
+    if (!c) return arg_error("null configuration");
+    const double vals[] = {c->tt_shape, c->tt_scale, c->tt_subtract, c->spe_expon_weight, c->spe_expon_scale,
+                           c->spe_norm_mu, c->spe_norm_sigma, c->spe_trunc_low, c->pulse_mu, c->pulse_beta,
+                           c->pulse_lo, c->pulse_hi, c->pulse_amplitude, c->noise_amp, c->sampling_freq,
+                           c->adc_freq, c->unf_pulse_res, c->time_range, c->lead_time, c->min_charge, c->iir_b0,
+                           c->iir_b1, c->iir_a1, c->nnls_tol, c->filt_pulse_t0, c->filt_pulse_dt};
+    for (double v : vals)
+        if (!std::isfinite(v)) return arg_error("configuration values must be finite");
+    if (!(c->tt_shape >= 1.0) || !(c->tt_scale > 0.0)) return arg_error("transit-time Gamma needs shape >= 1, scale > 0");
+    if (!(c->spe_expon_weight >= 0.0 && c->spe_expon_weight <= 1.0) || !(c->spe_expon_scale > 0.0) ||
+        !(c->spe_norm_sigma > 0.0))
+        return arg_error("SPE weight in [0, 1], positive exponential scale and sigma");
+    if (!(c->pulse_beta > 0.0) || !(c->pulse_hi > c->pulse_lo)) return arg_error("pulse beta > 0 and lo < hi");
+    const double F = std::exp(-std::exp(-(c->pulse_hi - c->pulse_mu) / c->pulse_beta)) -
+                     std::exp(-std::exp(-(c->pulse_lo - c->pulse_mu) / c->pulse_beta));
+    if (!(F > 0.0)) return arg_error("the pulse's truncation interval holds no probability");
+    if (!(c->noise_amp >= 0.0) || !(c->sampling_freq > 0.0) || !(c->adc_freq > 0.0) || !(c->unf_pulse_res > 0.0) ||
+        !(c->time_range > 0.0) || !(c->nnls_tol >= 0.0) || !(c->filt_pulse_dt > 0.0) || c->nnls_max_iter < 0)
+        return arg_error("positive frequencies, resolution, time range, table step; tolerance >= 0");
+    int64_t R = 0, Q = 0;
+    if (!near_int(c->sampling_freq / c->adc_freq, &R)) return arg_error("sampling_freq/adc_freq must be an integer");
+    if (!near_int((1.0 / c->adc_freq) / c->unf_pulse_res, &Q) || Q > 1024)
+        return arg_error("(1/adc_freq)/unf_pulse_res must be an integer <= 1024");
+    const int64_t n_wave = (int64_t)std::floor(c->time_range * c->sampling_freq + 1e-9) + 1;
+    const int64_t n_adc = (int64_t)std::floor(c->time_range * c->adc_freq + 1e-9) + 1;
+    const int64_t n_pulse = (int64_t)std::floor(c->time_range / c->unf_pulse_res + 1e-9) + 1;
+    if (n_wave > NTPMT_MAX_WAVE || n_adc > NTPMT_MAX_ADC || n_pulse > NTPMT_MAX_PULSE)
+        return arg_error("waveform, digitised or unfolding grid longer than the NTPMT_MAX_* limits");
+    if (n_adc != (n_wave + R - 1) / R) return arg_error("the digitised grid and the resampled length differ");
+    if (!c->decim_taps || c->n_decim_taps < 1 || c->n_decim_taps > NTPMT_MAX_TAPS || c->n_decim_taps % 2 == 0)
+        return arg_error("decim_taps: an odd number of taps up to NTPMT_MAX_TAPS");
+    if (!c->filt_pulse || c->n_filt_pulse < 2 || c->n_filt_pulse > NTPMT_MAX_FILT_PULSE)
+        return arg_error("filt_pulse: 2 to NTPMT_MAX_FILT_PULSE samples");
+    for (int i = 0; i < c->n_decim_taps; ++i)
+        if (!std::isfinite(c->decim_taps[i])) return arg_error("decim_taps must be finite");
+    for (int i = 0; i < c->n_filt_pulse; ++i)
+        if (!std::isfinite(c->filt_pulse[i])) return arg_error("filt_pulse must be finite");
+
+    // pattern table: T[j][p] = filtered pulse at dt_adc·(j + m0) − res·p
+    const double dt_adc = 1.0 / c->adc_freq;
+    const double fp_end = c->filt_pulse_t0 + (c->n_filt_pulse - 1) * c->filt_pulse_dt;
+    const int m0 = (int)std::ceil(c->filt_pulse_t0 / dt_adc - 1e-9);
+    const int m1 = (int)std::floor((fp_end + c->unf_pulse_res * (double)(Q - 1)) / dt_adc + 1e-9);
+    const int J = m1 - m0 + 1;
+    if (J < 1 || J > 256) return arg_error("the filtered pulse spans more than 256 digitiser samples");
+    std::vector<double> table((size_t)J * Q);
+    for (int j = 0; j < J; ++j)
+        for (int p = 0; p < Q; ++p)
+            table[(size_t)j * Q + p] = filt_pulse_at(c, dt_adc * (double)(j + m0) - c->unf_pulse_res * (double)p);
+
+    ntpmt_ctx* x = new ntpmt_ctx();
+    x->device = device;
+    auto cleanup = [&](int32_t st) {
+        ntpmt_destroy(x);
+        return st;
+    };
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return cleanup(fail(e, "hipSetDevice"));
+    hipDeviceProp_t prop;
+    if ((e = hipGetDeviceProperties(&prop, device)) != hipSuccess) return cleanup(fail(e, "hipGetDeviceProperties"));
+    x->num_cus = prop.multiProcessorCount;
+    if ((e = hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking)) != hipSuccess)
+        return cleanup(fail(e, "hipStreamCreate"));
+    if ((e = hipEventCreateWithFlags(&x->last, hipEventDisableTiming)) != hipSuccess)
+        return cleanup(fail(e, "hipEventCreate"));
+    if ((e = hipMalloc(&x->d_taps, sizeof(double) * c->n_decim_taps)) != hipSuccess) return cleanup(fail(e, "hipMalloc"));
+    if ((e = hipMalloc(&x->d_table, sizeof(double) * table.size())) != hipSuccess) return cleanup(fail(e, "hipMalloc"));
+    if ((e = hipMemcpy(x->d_taps, c->decim_taps, sizeof(double) * c->n_decim_taps, hipMemcpyHostToDevice)) !=
+        hipSuccess)
+        return cleanup(fail(e, "hipMemcpy"));
+    if ((e = hipMemcpy(x->d_table, table.data(), sizeof(double) * table.size(), hipMemcpyHostToDevice)) != hipSuccess)
+        return cleanup(fail(e, "hipMemcpy"));
+
+    DevCfg& d = x->cfg;
+    d.tt_d = c->tt_shape - 1.0 / 3.0;
+    d.tt_c = 1.0 / std::sqrt(9.0 * d.tt_d);
+    d.tt_scale = c->tt_scale;
+    d.tt_subtract = c->tt_subtract;
+    d.spe_w = c->spe_expon_weight;
+    d.spe_scale = c->spe_expon_scale;
+    d.spe_mu = c->spe_norm_mu;
+    d.spe_sigma = c->spe_norm_sigma;
+    d.spe_low = c->spe_trunc_low;
+    d.p_mu = c->pulse_mu;
+    d.p_inv_beta = 1.0 / c->pulse_beta;
+    d.p_lo = c->pulse_lo;
+    d.p_hi = c->pulse_hi;
+    d.p_norm = c->pulse_amplitude / c->pulse_beta / F;
+    d.noise_amp = c->noise_amp;
+    d.dt_wave = 1.0 / c->sampling_freq;
+    d.dt_adc = dt_adc;
+    d.res = c->unf_pulse_res;
+    d.lead = c->lead_time;
+    d.min_charge = c->min_charge;
+    d.b0 = c->iir_b0;
+    d.b1 = c->iir_b1;
+    d.a1 = c->iir_a1;
+    d.tol = c->nnls_tol;
+    d.max_iter = c->nnls_max_iter;
+    d.n_wave = (int)n_wave;
+    d.n_adc = (int)n_adc;
+    d.n_pulse = (int)n_pulse;
+    d.R = (int)R;
+    d.Q = (int)Q;
+    d.J = J;
+    d.m0 = m0;
+    d.n_taps = c->n_decim_taps;
+    d.chunk = (int)((n_wave + kLanes - 1) / kLanes);
+    d.max_outer = 8 * (int)n_adc + 64;
+    d.taps = x->d_taps;
+    d.table = x->d_table;
+    const size_t nw32 = (size_t)(n_pulse + 31) / 32;
+    x->lds_bytes = sizeof(double) * ((size_t)J * Q + std::max<size_t>(n_wave, (size_t)n_adc * (n_adc + 1) / 2) +
+                                     2 * (size_t)n_adc + 2 * kMaxPos) +
+                   sizeof(uint32_t) * 2 * nw32 + sizeof(int) * kMaxPos;
+    if (x->lds_bytes > 64 * 1024) return cleanup(arg_error("configuration needs more than 64 KB of LDS per channel"));
+    int per_cu = 0;
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, channel_kernel, 64, x->lds_bytes)) != hipSuccess)
+        return cleanup(fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor"));
+    x->blocks = std::max(1, per_cu) * x->num_cus;
+    *out = x;
+    return NT_OK;
+}
+
+void ntpmt_destroy(ntpmt_ctx* x) {
+    if (!x) return;
+    (void)hipSetDevice(x->device);
+    if (x->last) (void)hipEventSynchronize(x->last);
+    if (x->d_ws) (void)hipFree(x->d_ws);
+    if (x->d_taps) (void)hipFree(x->d_taps);
+    if (x->d_table) (void)hipFree(x->d_table);
+    if (x->last) (void)hipEventDestroy(x->last);
+    if (x->stream) (void)hipStreamDestroy(x->stream);
+    delete x;
+}
+
+int32_t ntpmt_sizes(const ntpmt_ctx* x, int32_t* n_wave, int32_t* n_adc, int32_t* n_pulse) {
+    if (!x) return arg_error("null context");
+    if (n_wave) *n_wave = x->cfg.n_wave;
+    if (n_adc) *n_adc = x->cfg.n_adc;
+    if (n_pulse) *n_pulse = x->cfg.n_pulse;
+    return NT_OK;
+}
+
+int32_t ntpmt_make_reco_pulses(ntpmt_ctx* x, const nt_hits* h, const uint8_t* d_select, int64_t n, int32_t time_column,
+                               uint64_t seed, int64_t capacity, const ntpmt_pulses* out, int64_t* d_n_channels,
+                               void* stream) {
+    if (!x || !h || !out) return arg_error("null context, hits or output");
+    if (n < 0 || n > INT32_MAX) return arg_error("0 <= n_hits <= 2^31 - 1");
+    if (capacity < 0) return arg_error("capacity >= 0");
+    if (time_column != 0 && time_column != 1) return arg_error("time_column 0 (time) or 1 (tres)");
+    const float* tcol = time_column == 0 ? h->time : h->tres;
+    if (n > 0 && (!tcol || !h->module_id || !h->pmt_id || !h->source_index || !h->photon_id))
+        return arg_error("the time column, module_id, pmt_id, source_index and photon_id are required");
+    if (capacity > 0 && (!out->module_id || !out->pmt_id || !out->n_hits || !out->n_pulses || !out->status ||
+                         !out->t0 || !out->time || !out->charge))
+        return arg_error("every output column except waveform is required");
+    std::lock_guard<std::mutex> lk(x->mu);
+    PMT_TRY(hipSetDevice(x->device));
+    hipStream_t s = stream ? (hipStream_t)stream : x->stream;
+    // the workspace is the context's: the previous call's kernels finish first
+    PMT_TRY(hipEventSynchronize(x->last));
+    if (n == 0) {
+        zero_count_kernel<<<1, 1, 0, s>>>(d_n_channels);
+        PMT_TRY(hipGetLastError());
+        PMT_TRY(hipEventRecord(x->last, s));
+        return NT_OK;
+    }
+    const WsLayout L = ws_layout(n);
+    if (L.total > x->ws_cap) {
+        if (x->d_ws) (void)hipFree(x->d_ws);
+        x->d_ws = nullptr;
+        x->ws_cap = 0;
+        PMT_TRY(hipMalloc(&x->d_ws, L.total));
+        x->ws_cap = L.total;
+    }
+    char* b = x->d_ws;
+    auto* key_a = (uint64_t*)(b + L.key_a);
+    auto* key_b = (uint64_t*)(b + L.key_b);
+    auto* idx_a = (int32_t*)(b + L.idx_a);
+    auto* idx_b = (int32_t*)(b + L.idx_b);
+    auto* tq = (double*)(b + L.tq);
+    auto* hflag = (uint8_t*)(b + L.hflag);
+    auto* ch = (uint32_t*)(b + L.ch);
+    auto* uniq = (uint32_t*)(b + L.unique);
+    auto* counts = (int32_t*)(b + L.counts);
+    auto* offs = (int32_t*)(b + L.offsets);
+    auto* n_runs = (int32_t*)(b + L.n_runs);
+    auto* next = (int32_t*)(b + L.next);
+    size_t cub = L.cub_bytes;
+
+    HitIn hi{tcol, h->module_id, h->pmt_id, h->source_index, h->photon_id, d_select, n, seed};
+    const int grid = (int)std::min<int64_t>((n + 255) / 256, (int64_t)x->num_cus * 16);
+    hit_draws_kernel<<<grid, 256, 0, s>>>(hi, x->cfg, key_a, idx_a, tq, hflag);
+    PMT_TRY(hipGetLastError());
+    PMT_TRY(hipcub::DeviceRadixSort::SortPairs(b + L.cub, cub, key_a, key_b, idx_a, idx_b, (int)n, 0, 53, s));
+    channel_ids_kernel<<<grid, 256, 0, s>>>(key_b, n, ch);
+    PMT_TRY(hipGetLastError());
+    cub = L.cub_bytes;
+    PMT_TRY(hipcub::DeviceRunLengthEncode::Encode(b + L.cub, cub, ch, uniq, counts, n_runs, (int)n, s));
+    cub = L.cub_bytes;
+    PMT_TRY(hipcub::DeviceScan::ExclusiveSum(b + L.cub, cub, counts, offs, (int)n, s));
+    finish_count_kernel<<<1, 1, 0, s>>>(n_runs, uniq, d_n_channels, next);
+    PMT_TRY(hipGetLastError());
+    ChanArgs a{key_b, idx_b, tq, hflag, uniq, counts, offs, n_runs, next, capacity, seed, *out};
+    channel_kernel<<<x->blocks, 64, x->lds_bytes, s>>>(a, x->cfg);
+    PMT_TRY(hipGetLastError());
+    PMT_TRY(hipEventRecord(x->last, s));
+    return NT_OK;
+}
+
+const char* ntpmt_last_error(void) { return ntpmt::g_err; }
+
+int32_t ntpmt_synchronize(ntpmt_ctx* x) {
+    if (!x) return arg_error("null context");
+    PMT_TRY(hipSetDevice(x->device));
+    PMT_TRY(hipEventSynchronize(x->last));
+    return NT_OK;
+}
+
+}  // extern "C"

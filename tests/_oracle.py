@@ -92,6 +92,19 @@ def _bind(lib):
         "ntor_fast_linear_interp": (d, [d, dp, dp, i]),
         "ntor_check_pmt_hit": (i, [dp, dp, i, d]),
         "ntor_pom_pmt_coordinates": (None, [dp]),
+        # PMT front end (oracle/ntpmt_oracle.c)
+        "ntor_pmt_normals": (None, [C.POINTER(C.c_uint32), dp]),
+        "ntor_pmt_hit_draws": (None, [C.POINTER(_abi.ntpmt_config), C.c_int64, fp, C.POINTER(C.c_uint32),
+                                      C.POINTER(C.c_uint64), u64, dp, dp, C.POINTER(C.c_int32)]),
+        "ntor_pmt_pulse": (d, [C.POINTER(_abi.ntpmt_config), d]),
+        "ntor_pmt_filt_pulse": (d, [C.POINTER(_abi.ntpmt_config), d]),
+        "ntor_pmt_iir": (None, [C.POINTER(_abi.ntpmt_config), C.c_int32, dp, dp]),
+        "ntor_pmt_decimate": (None, [C.POINTER(_abi.ntpmt_config), C.c_int32, dp, C.c_int32, C.c_int32, dp]),
+        "ntor_fnnls": (C.c_int32, [C.c_int32, C.c_int32, dp, dp, d, C.c_int64, dp, C.POINTER(C.c_int64),
+                                   C.POINTER(C.c_int32)]),
+        "ntor_pmt_sizes": (C.c_int32, [C.POINTER(_abi.ntpmt_config)] + [C.POINTER(C.c_int32)] * 4),
+        "ntor_pmt_channel": (C.c_int32, [C.POINTER(_abi.ntpmt_config), C.c_int32, dp, dp, u32, u64, dp, dp, dp, dp,
+                                         dp, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -165,3 +178,57 @@ def propagate(setup, seed=0, capacity=1 << 20, threads=None, lib=None) -> Photon
     out = hits.truncated(nh.value)
     out.n_simulated = ns.value
     return out
+
+
+# ---------------------------------------------------------------- PMT front end
+def pmt_hit_draws(cfg_abi, time, source_index, photon_id, seed):
+    """Per hit: (t', charge, status bits) of the transit-time and SPE draws."""
+    lib = load()
+    n = len(time)
+    t = np.ascontiguousarray(time, dtype=np.float32)
+    src = np.ascontiguousarray(source_index, dtype=np.uint32)
+    pid = np.ascontiguousarray(photon_id, dtype=np.uint64)
+    tp, q = np.zeros(n), np.zeros(n)
+    fl = np.zeros(n, np.int32)
+    lib.ntor_pmt_hit_draws(C.byref(cfg_abi), n, t.ctypes.data_as(C.POINTER(C.c_float)),
+                           src.ctypes.data_as(C.POINTER(C.c_uint32)), pid.ctypes.data_as(C.POINTER(C.c_uint64)),
+                           C.c_uint64(seed), tp.ctypes.data_as(C.POINTER(C.c_double)),
+                           q.ctypes.data_as(C.POINTER(C.c_double)), fl.ctypes.data_as(C.POINTER(C.c_int32)))
+    return tp, q, fl
+
+
+def pmt_sizes(cfg_abi):
+    lib = load()
+    v = [C.c_int32() for _ in range(4)]
+    lib.ntor_pmt_sizes(C.byref(cfg_abi), *[C.byref(x) for x in v])
+    return tuple(x.value for x in v)          # n_wave, n_adc, n_pulse, R
+
+
+def pmt_channel(cfg_abi, t, q, channel, seed, unfold=True):
+    """One channel's chain: dict with t0, wave, filt, digi, x (charges on the
+    unfolding grid), status, iter, outer."""
+    lib = load()
+    n_wave, n_adc, n_pulse, _ = pmt_sizes(cfg_abi)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    t0 = C.c_double()
+    wave, filt, digi, x = np.zeros(n_wave), np.zeros(n_wave), np.zeros(n_adc), np.zeros(n_pulse)
+    it, outer = C.c_int64(), C.c_int32()
+    P = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    st = lib.ntor_pmt_channel(C.byref(cfg_abi), len(t), P(t), P(q), int(channel), C.c_uint64(seed), C.byref(t0),
+                              P(wave), P(filt), P(digi), P(x) if unfold else None, C.byref(it), C.byref(outer))
+    return {"t0": t0.value, "wave": wave, "filt": filt, "digi": digi, "x": x if unfold else None, "status": st,
+            "iter": it.value, "outer": outer.value}
+
+
+def fnnls(A, b, tol=1e-8, max_iter=None):
+    lib = load()
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    m, n = A.shape
+    x = np.zeros(n)
+    it, outer = C.c_int64(), C.c_int32()
+    P = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    st = lib.ntor_fnnls(m, n, P(A), P(b), tol, 30 * n if max_iter is None else max_iter, P(x), C.byref(it),
+                        C.byref(outer))
+    return x, st

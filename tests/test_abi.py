@@ -17,14 +17,14 @@ def declared_functions():
     names = set()
     for header in sorted((ROOT / "include").glob("*.h")):
         text = re.sub(r"/\*.*?\*/", "", header.read_text(), flags=re.S)
-        names |= set(re.findall(r"^\s*(?:int32_t|void|const char\*)\s+\**(ntf?_\w+)\s*\(", text, re.M))
+        names |= set(re.findall(r"^\s*(?:int32_t|void|const char\*)\s+\**(nt(?:f|pmt)?_\w+)\s*\(", text, re.M))
     return sorted(names)
 
 
 def test_header_declares_entry_points():
     fns = declared_functions()
     for name in ("nt_create", "nt_destroy", "nt_set_medium", "nt_set_spectrum", "nt_set_targets",
-                 "nt_set_sources", "nt_propagate", "nt_launch", "nt_eval"):
+                 "nt_set_sources", "nt_propagate", "nt_launch", "nt_eval", "ntpmt_create", "ntpmt_make_reco_pulses"):
         assert name in fns
 
 
@@ -87,8 +87,9 @@ def test_struct_layouts_match_c_compiler(tmp_path):
     """sizeof and field offsets of every ABI struct as a C compiler lays them
     out from the header, against the ctypes mirror."""
     structs = ["nt_source", "nt_medium", "nt_target", "nt_spectrum", "nt_hits", "nt_particle", "nt_cascade_model",
-               "nt_injector"]
-    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ntprop.h"', "int main(void) {"]
+               "nt_injector", "ntpmt_config", "ntpmt_pulses"]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ntprop.h"', '#include "ntpmt.h"',
+             "int main(void) {"]
     for st in structs:
         lines.append(f'printf("{st} %zu\\n", sizeof({st}));')
         for name, *_ in getattr(_abi, st)._fields_:
