@@ -416,6 +416,94 @@ def flow_kernels_benchmark(steps: int = 2000, llh_reps: int = 10):
             "llh_pairs": int(1024 * len(modules) * 16), "llh_hits": int(d.hit_t.shape[0]), "llh_ms": llh_ms}
 
 
+def pmt_event_hits(events: int, rng):
+    """Detected hits of `events` events on the 10-string × 20-module × 16-PMT
+    array (3200 channels each; module ids 200·e + m so every event's channels
+    are distinct): per channel Poisson(λ) hits, λ log-uniform in [0.1, 300]
+    (near and far modules), times a prompt peak (N(t_p, 1.5 ns), 60 %) plus a
+    scattered tail (t_p + Exp(25 ns)), t_p uniform in [20, 200] ns."""
+    n_ch = events * 200 * 16
+    lam = np.exp(rng.uniform(np.log(0.1), np.log(300.0), n_ch))
+    k = rng.poisson(lam)
+    ch = np.repeat(np.arange(n_ch), k)
+    tp = rng.uniform(20, 200, n_ch)[ch]
+    prompt = rng.random(ch.size) < 0.6
+    t = np.where(prompt, tp + rng.normal(0, 1.5, ch.size), tp + rng.exponential(25.0, ch.size))
+    return {"module_id": (ch // 16).astype(np.uint16), "pmt_id": (ch % 16 + 1).astype(np.uint8),
+            "time": t.astype(np.float32), "source_index": np.zeros(ch.size, np.uint32),
+            "photon_id": np.arange(ch.size, dtype=np.uint64)}
+
+
+def pmt_frontend_benchmark(events: int = 16, reps: int = 10, cpu_target_s: float = 10.0, cpu: bool = True):
+    """PMT front end (make_reco_pulses for every channel in one call): `events`
+    events of pmt_event_hits on the device, `reps` timed calls after 2 warm-up
+    calls (HIP events on the launch stream), and the CPU oracle (hit draws and
+    the per-channel chain, FP64, one channel per thread over all host threads)
+    on a bounded sample of the same channels."""
+    import torch
+    from concurrent.futures import ThreadPoolExecutor
+
+    from neutrinotelescopes_amd import pmt_frontend as pf
+    sys.path.insert(0, str(ROOT / "tests"))
+    import _oracle
+    rng = np.random.default_rng(0)
+    h = pmt_event_hits(events, rng)
+    n = h["time"].size
+    fe = pf.PMTFrontEnd(pf.std_pmt_config(), 0)
+    dev = torch.device("cuda:0")
+    up = lambda a: torch.from_numpy(a.view({np.uint16: np.int16, np.uint32: np.int32, np.uint64: np.int64}.get(
+        a.dtype.type, a.dtype))).to(dev)
+    cols = {k: up(v) for k, v in h.items()}
+    cap = events * 200 * 17
+    for _ in range(2):
+        out, n_ch, _ = fe.make_reco_pulses_device(cols, n, seed=1, capacity=cap)
+    torch.cuda.synchronize(dev)
+    stream = torch.cuda.current_stream(dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for r in range(reps):
+        out, n_ch, _ = fe.make_reco_pulses_device(cols, n, seed=1 + r, capacity=cap)
+    e1.record(stream)
+    torch.cuda.synchronize(dev)
+    ms = e0.elapsed_time(e1) / reps
+    channels = int(n_ch.item())
+    pulses = int(out["n_pulses"][:channels].sum().item())
+    st = int((out["status"][:channels] != 0).sum().item())
+    fe.close()
+    res = {"config": f"{events} events x 3200 channels (10 strings x 20 modules x 16 PMTs), Poisson hits "
+                     f"log-uniform 0.1-300 per channel; STD_PMT_CONFIG (601-sample waveform, 76 ADC samples, "
+                     f"3001-column FNNLS)",
+           "hits": n, "channels": channels, "pulses": pulses, "channels_with_status": st, "ms_per_call": ms,
+           "channels_per_s": channels / (ms * 1e-3), "events_per_s": events / (ms * 1e-3)}
+    if not cpu:
+        return res
+    # CPU oracle on a sample of the same channels (all host threads, one channel each)
+    cfg = pf.std_pmt_config()
+    ca, keep = cfg.to_abi()
+    ch_all = h["module_id"].astype(np.int64) * 17 + h["pmt_id"]
+    order = np.argsort(ch_all, kind="stable")
+    bounds = np.r_[0, np.flatnonzero(np.diff(ch_all[order])) + 1, order.size]
+    threads = host_cpu_info()["cpu_share"]
+
+    def one(c):
+        idx = order[bounds[c]:bounds[c + 1]]
+        tp, q, _ = _oracle.pmt_hit_draws(ca, h["time"][idx], h["source_index"][idx], h["photon_id"][idx], 1)
+        _oracle.pmt_channel(ca, tp, q, int(ch_all[idx[0]]), 1)
+
+    sample = rng.permutation(len(bounds) - 1)
+    done, t0 = 0, time.perf_counter()
+    with ThreadPoolExecutor(threads) as pool:
+        while time.perf_counter() - t0 < cpu_target_s and done < sample.size:
+            chunk = sample[done:done + 4 * threads]
+            list(pool.map(one, chunk))
+            done += chunk.size
+    cpu_dt = time.perf_counter() - t0
+    cpu_rate = done / cpu_dt
+    return dict(res, cpu_oracle_channels_per_s=cpu_rate, cpu_threads=threads,
+                cpu_sample=f"{done} channels in {cpu_dt:.1f} s on {threads} threads (FP64 C oracle)",
+                gpu_vs_cpu=channels / (ms * 1e-3) / cpu_rate)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -440,6 +528,9 @@ def main():
                     help="test every sphere on every step instead of string bounds first")
     ap.add_argument("--flow-kernels", action="store_true",
                     help="only time the flow kernels on synthetic data (training steps, event likelihood)")
+    ap.add_argument("--no-pmt-frontend", action="store_true", help="skip the PMT front-end measurement")
+    ap.add_argument("--pmt-frontend-only", action="store_true",
+                    help="only time the PMT front end (waveforms and unfolding of 16 array events)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU per process; multi-GPU runs are not "
@@ -457,6 +548,9 @@ def main():
         return
     if args.flow_kernels:
         print(json.dumps({"flow_kernels": flow_kernels_benchmark()}), flush=True)
+        return
+    if args.pmt_frontend_only:
+        print(json.dumps({"pmt_frontend": pmt_frontend_benchmark(cpu=not args.no_cpu_baseline)}), flush=True)
         return
     if args.copy_calibration:
         a = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
@@ -496,6 +590,9 @@ def main():
     if not args.no_surrogate and args.workload == "cascade":
         out["surrogate"] = surrogate_benchmark()
         progress("surrogate done")
+    if not args.no_pmt_frontend and args.workload == "cascade":
+        out["pmt_frontend"] = pmt_frontend_benchmark()
+        progress("pmt front end done")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload)
         out["gpu_vs_cpu"] = value / out["cpu_baseline"]["value"]

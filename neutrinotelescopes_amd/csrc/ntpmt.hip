@@ -19,9 +19,9 @@
 // pattern p shifted down by q rows, so A is J·Q numbers (a 16 × 40 table for
 // the standard configuration) kept in LDS.  The active set's Gram matrix
 // A_Pᵀ A_P is kept as a Cholesky factor (packed, LDS; the waveform's space)
-// that grows by one row when a column enters and is recomputed from the
-// first removed position when columns leave; solves are lane-parallel
-// column sweeps.  Per outer iteration the gradient Aᵀ(b − A x) over all n_pulse
+// that grows by one row when a column enters and loses one by Givens
+// rotations when a column leaves; solves are lane-parallel column sweeps
+// with the reciprocal diagonal kept beside the factor.  Per outer iteration the gradient Aᵀ(b − A x) over all n_pulse
 // columns is one pass of J fused multiply-adds per column.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
@@ -52,6 +52,8 @@ constexpr uint32_t kNoChannel = (1u << 21) - 1;      // key of hits that are not
 constexpr int kLanes = 64;
 constexpr int kMaxPos = 2 * kLanes;                  // active-set positions per workgroup (≥ NTPMT_MAX_ADC)
 constexpr double kDegenerate = 1e-14;                // a column with d² ≤ this · ‖a‖² is numerically dependent
+constexpr int kTaps = 16;                            // rows a column reaches in the register-window gradient pass
+constexpr int kPadLo = 16, kPadHi = 48;              // zero rows around the residual (padded LDS array)
 
 struct DevCfg {
     double tt_d, tt_c, tt_scale, tt_subtract;
@@ -195,12 +197,25 @@ __device__ __forceinline__ double wmin(double v) {
     return v;
 }
 __device__ __forceinline__ bool wany(bool b) { return __ballot(b) != 0; }
+// lane l's value (l uniform) by v_readlane: a scalar broadcast, no LDS round trip
+__device__ __forceinline__ double readlane(double v, int l) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
 // value of position j (uniform) from the two per-lane registers
 __device__ __forceinline__ double getp(double a, double b, int j) {
-    return j < kLanes ? __shfl(a, j) : __shfl(b, j - kLanes);
+    return j < kLanes ? readlane(a, j) : readlane(b, j - kLanes);
 }
 __device__ __forceinline__ int getpi(int a, int b, int j) {
-    return j < kLanes ? __shfl(a, j) : __shfl(b, j - kLanes);
+    return j < kLanes ? __builtin_amdgcn_readlane(a, j) : __builtin_amdgcn_readlane(b, j - kLanes);
+}
+// set position j (uniform) of a register pair (selects on both registers:
+// no choice between the two addresses, which would put them in memory)
+__device__ __forceinline__ void setp(double& a, double& b, int j, double v, int lane) {
+    const bool in_a = j < kLanes;
+    a = (in_a && lane == j) ? v : a;
+    b = (!in_a && lane == j - kLanes) ? v : b;
 }
 __device__ __forceinline__ int tri(int i) { return i * (i + 1) / 2; }
 
@@ -208,22 +223,13 @@ struct Lds {
     double* T;          // [J][Q]
     double* buf;        // waveform [n_wave], then the Cholesky factor (packed rows)
     double* b;          // [n_adc] digitised waveform
-    double* r;          // [n_adc] residual b − A x
+    double* r;          // [n_adc] residual b − A x, inside kPadLo / kPadHi zeros
     uint32_t* inP;      // [n_pulse / 32] active set
     uint32_t* excl;     // [n_pulse / 32] columns skipped as dependent
-    int* mv_col;        // [kMaxPos] compaction staging
-    double* mv_x;       // [kMaxPos]
-    double* mv_atb;     // [kMaxPos]
 };
 
-__device__ __forceinline__ double acol(const Lds& s, const DevCfg& c, int col, int k) {
-    const int q = col / c.Q, p = col - q * c.Q, j = k - q - c.m0;
-    return (j >= 0 && j < c.J) ? s.T[j * c.Q + p] : 0.0;
-}
-
 // a_colᵀ v over the rows the column reaches (k ascending)
-__device__ __forceinline__ double adot(const Lds& s, const DevCfg& c, int col, const double* v) {
-    const int q = col / c.Q, p = col - q * c.Q;
+__device__ __forceinline__ double adot(const Lds& s, const DevCfg& c, int q, int p, const double* v) {
     double acc = 0.0;
     for (int j = 0; j < c.J; ++j) {
         const int k = q + c.m0 + j;
@@ -232,9 +238,8 @@ __device__ __forceinline__ double adot(const Lds& s, const DevCfg& c, int col, c
     return acc;
 }
 
-// a_ciᵀ a_cj
-__device__ __forceinline__ double gram(const Lds& s, const DevCfg& c, int ci, int cj) {
-    const int qi = ci / c.Q, pi = ci - qi * c.Q, qj = cj / c.Q, pj = cj - qj * c.Q;
+// a_iᵀ a_j for columns i = (qi, pi), j = (qj, pj)
+__device__ __forceinline__ double gram(const Lds& s, const DevCfg& c, int qi, int pi, int qj, int pj) {
     double acc = 0.0;
     for (int ji = 0; ji < c.J; ++ji) {
         const int k = qi + c.m0 + ji, jj = ji + qi - qj;
@@ -248,79 +253,184 @@ __device__ __forceinline__ bool bit(const uint32_t* m, int i) { return (m[i >> 5
 // Active set: positions 0 … P−1, position j in lane j mod 64, register A (j < 64) or B.
 struct Active {
     int P = 0;
-    int colA = 0, colB = 0;
+    int colA = 0, colB = 0;      // column q·Q + p
+    int qA = 0, qB = 0, pA = 0, pB = 0;
     double xA = 0, xB = 0, sA = 0, sB = 0, zA = 0, zB = 0, atbA = 0, atbB = 0;
+    double dA = 0, dB = 0;       // reciprocal diagonal of the factor's row j
 };
 
-// Append column `col` as position t = a.P − (already counted) : extends the
-// Cholesky factor by row t and z by z_t.  Returns false (nothing written) if
-// the column is numerically dependent on positions 0 … t−1.
-__device__ bool chol_append(const Lds& s, const DevCfg& c, Active& a, int t, int col, double atb_col, int lane) {
-    double gA = (lane < t) ? gram(s, c, a.colA, col) : 0.0;
-    double gB = (lane + kLanes < t) ? gram(s, c, a.colB, col) : 0.0;
+// Append column `col` as position t: extends the Cholesky factor by row t
+// (forward substitution with the Gram row of the active columns) and z by z_t.
+// Returns false (nothing written) if the column is numerically dependent on
+// positions 0 … t−1.
+__device__ __forceinline__ bool chol_append(const Lds& s, const DevCfg& c, Active& a, int t, int col, double atb_col,
+                                            int lane) {
+    const int cq = col / c.Q, cp = col - cq * c.Q;
+    double gA = (lane < t) ? gram(s, c, a.qA, a.pA, cq, cp) : 0.0;
+    double gB = (lane + kLanes < t) ? gram(s, c, a.qB, a.pB, cq, cp) : 0.0;
     double lA = 0.0, lB = 0.0;
     for (int j = 0; j < t; ++j) {
-        const double ljj = s.buf[tri(j) + j];
-        const double lj = getp(gA, gB, j) / ljj;
-        if (j < kLanes) {
-            if (lane == j) lA = lj;
-        } else if (lane == j - kLanes) {
-            lB = lj;
-        }
+        const double lj = getp(gA, gB, j) * getp(a.dA, a.dB, j);
+        setp(lA, lB, j, lj, lane);
         if (lane > j && lane < t) gA -= s.buf[tri(lane) + j] * lj;
         if (lane + kLanes > j && lane + kLanes < t) gB -= s.buf[tri(lane + kLanes) + j] * lj;
     }
-    const double gcc = gram(s, c, col, col);
+    const double gcc = gram(s, c, cq, cp, cq, cp);
     const double d2 = gcc - wsum(lA * lA + lB * lB);
     if (!(d2 > kDegenerate * gcc)) return false;
-    const double d = sqrt(d2);
-    const double zt = (atb_col - wsum((lane < t ? lA * a.zA : 0.0) + (lane + kLanes < t ? lB * a.zB : 0.0))) / d;
+    const double d = sqrt(d2), id = 1.0 / d;
+    const double zt = (atb_col - wsum((lane < t ? lA * a.zA : 0.0) + (lane + kLanes < t ? lB * a.zB : 0.0))) * id;
     if (lane < t) s.buf[tri(t) + lane] = lA;
     if (lane + kLanes < t) s.buf[tri(t) + lane + kLanes] = lB;
     if (lane == 0) s.buf[tri(t) + t] = d;
-    if (t < kLanes) {
-        if (lane == t) { a.colA = col; a.zA = zt; a.atbA = atb_col; }
-    } else if (lane == t - kLanes) {
-        a.colB = col; a.zB = zt; a.atbB = atb_col;
-    }
+    const bool own_a = t < kLanes && lane == t, own_b = t >= kLanes && lane == t - kLanes;
+    a.colA = own_a ? col : a.colA;
+    a.qA = own_a ? cq : a.qA;
+    a.pA = own_a ? cp : a.pA;
+    a.qB = own_b ? cq : a.qB;
+    a.pB = own_b ? cp : a.pB;
+    a.zA = own_a ? zt : a.zA;
+    a.atbA = own_a ? atb_col : a.atbA;
+    a.dA = own_a ? id : a.dA;
+    a.colB = own_b ? col : a.colB;
+    a.zB = own_b ? zt : a.zB;
+    a.atbB = own_b ? atb_col : a.atbB;
+    a.dB = own_b ? id : a.dB;
     __syncthreads();
     return true;
 }
 
 // s = L⁻ᵀ z over positions 0 … P−1 (0 beyond)
-__device__ void back_solve(const Lds& s, Active& a, int lane) {
+__device__ __forceinline__ void back_solve(const Lds& s, Active& a, int lane) {
     double zA = a.zA, zB = a.zB;
     a.sA = 0.0;
     a.sB = 0.0;
     for (int j = a.P - 1; j >= 0; --j) {
-        const double sj = getp(zA, zB, j) / s.buf[tri(j) + j];
-        if (j < kLanes) {
-            if (lane == j) a.sA = sj;
-        } else if (lane == j - kLanes) {
-            a.sB = sj;
-        }
+        const double sj = getp(zA, zB, j) * getp(a.dA, a.dB, j);
+        setp(a.sA, a.sB, j, sj, lane);
         if (lane < j) zA -= s.buf[tri(j) + lane] * sj;
         if (lane + kLanes < j) zB -= s.buf[tri(j) + lane + kLanes] * sj;
     }
 }
 
-// r = b − A x (rows in uniform blocks of 64: the shuffles run on every lane)
-__device__ void residual(const Lds& s, const DevCfg& c, const Active& a, int lane) {
-    for (int k0 = 0; k0 < c.n_adc; k0 += kLanes) {
-        const int k = k0 + lane;
-        const bool row = k < c.n_adc;
-        double acc = row ? s.b[k] : 0.0;
-        for (int j = 0; j < a.P; ++j) {
-            const int col = getpi(a.colA, a.colB, j);
-            const double xj = getp(a.xA, a.xB, j);
-            if (row) acc -= acol(s, c, col, k) * xj;
+// position j + 1 → j for j ≥ r (the A/B register pair of a per-position value)
+template <typename T>
+__device__ __forceinline__ void shift_down(T& A, T& B, int r, int lane) {
+    const T a1 = __shfl_down(A, 1), b0 = __shfl(B, 0), b1 = __shfl_down(B, 1);
+    const T na = lane < r ? A : (lane == kLanes - 1 ? b0 : a1);
+    const T nb = lane + kLanes < r ? B : b1;
+    A = na;
+    B = nb;
+}
+
+// Remove position r: the factor without row r is lower triangular but for
+// one entry right of the diagonal in each later row; Givens rotations of the
+// column pairs (k, k+1), k = r … P−2, remove them (L Q Qᵀ Lᵀ = L Lᵀ) and the
+// same rotations carry z (L z = Aᵀb stays true).  Then the later rows move up
+// one slot and the per-position registers shift down one position.
+__device__ __forceinline__ void chol_delete(const Lds& s, Active& a, int r, int lane) {
+    const int P = a.P;
+    for (int k = r; k < P - 1; ++k) {
+        const int base = tri(k + 1);                     // old row k + 1
+        const double av = s.buf[base + k], bv = s.buf[base + k + 1];
+        const double rr = sqrt(av * av + bv * bv), ir = 1.0 / rr;
+        const double cs = av * ir, sn = bv * ir;
+        for (int i = k + 2 + lane; i < P; i += kLanes) {
+            const double x = s.buf[tri(i) + k], y = s.buf[tri(i) + k + 1];
+            s.buf[tri(i) + k] = cs * x + sn * y;
+            s.buf[tri(i) + k + 1] = cs * y - sn * x;
         }
-        if (row) s.r[k] = acc;
+        const double zk = getp(a.zA, a.zB, k), zk1 = getp(a.zA, a.zB, k + 1);
+        setp(a.zA, a.zB, k, cs * zk + sn * zk1, lane);
+        setp(a.zA, a.zB, k + 1, cs * zk1 - sn * zk, lane);
+        setp(a.dA, a.dB, k, ir, lane);
+        if (lane == 0) {
+            s.buf[base + k] = rr;
+            s.buf[base + k + 1] = 0.0;
+        }
+        __syncthreads();
     }
+    // old row i (> r), columns 0 … i−1, to slot i − 1: each lane moves the
+    // same columns of every row, so a row's source is read before the next
+    // row's copy overwrites it, by the same lane
+    for (int i = r + 1; i < P; ++i)
+        for (int col = lane; col < i; col += kLanes) s.buf[tri(i - 1) + col] = s.buf[tri(i) + col];
+    __syncthreads();
+    shift_down(a.colA, a.colB, r, lane);
+    shift_down(a.qA, a.qB, r, lane);
+    shift_down(a.pA, a.pB, r, lane);
+    shift_down(a.xA, a.xB, r, lane);
+    shift_down(a.atbA, a.atbB, r, lane);
+    a.P = P - 1;
+    setp(a.zA, a.zB, a.P, 0.0, lane);
+}
+
+// r = b − A x: each lane holds rows lane and lane + 64 and walks the active
+// positions (their q, p and x broadcast by readlane)
+__device__ __forceinline__ void residual(const Lds& s, const DevCfg& c, const Active& a, int lane) {
+    const int k0 = lane, k1 = lane + kLanes;
+    double r0 = k0 < c.n_adc ? s.b[k0] : 0.0, r1 = k1 < c.n_adc ? s.b[k1] : 0.0;
+    for (int j = 0; j < a.P; ++j) {
+        const int qj = getpi(a.qA, a.qB, j), pj = getpi(a.pA, a.pB, j);
+        const double xj = getp(a.xA, a.xB, j);
+        const int l0 = k0 - qj - c.m0, l1 = k1 - qj - c.m0;
+        if (l0 >= 0 && l0 < c.J) r0 -= s.T[l0 * c.Q + pj] * xj;
+        if (l1 >= 0 && l1 < c.J) r1 -= s.T[l1 * c.Q + pj] * xj;
+    }
+    if (k0 < c.n_adc) s.r[k0] = r0;
+    if (k1 < c.n_adc) s.r[k1] = r1;
     __syncthreads();
 }
 
-__global__ __launch_bounds__(64) void channel_kernel(ChanArgs g, DevCfg c) {
+// argmax over the columns outside the active set of w = Aᵀ r (first index
+// on ties).  With J ≤ kTaps and −kPadLo ≤ m0 ≤ 0 (the standard
+// configuration): lane p < Q takes the columns q·Q + p, its pattern's taps in
+// registers and a window of kTaps residual rows rotating through registers
+// (one broadcast LDS read per column); otherwise one column per lane and step.
+__device__ __forceinline__ void w_argmax(const Lds& s, const DevCfg& c, int lane, double& best, int& bi) {
+    best = -INFINITY;
+    bi = 0x7fffffff;
+    if (c.J <= kTaps && c.m0 >= -kPadLo && c.m0 <= 0) {
+        if (lane < c.Q) {
+            const int p = lane;
+            double t[kTaps], win[kTaps];
+#pragma unroll
+            for (int j = 0; j < kTaps; ++j) t[j] = j < c.J ? s.T[j * c.Q + p] : 0.0;
+            const int nq = (c.n_pulse - 1 - p) / c.Q + 1;      // columns of this phase
+            const double* rb = s.r + c.m0;                      // rb[q + j] = r[q + m0 + j]
+#pragma unroll
+            for (int j = 0; j < kTaps; ++j) win[j] = rb[j];
+            for (int q0 = 0; q0 < nq; q0 += kTaps) {
+#pragma unroll
+                for (int u = 0; u < kTaps; ++u) {
+                    double w = 0.0;
+#pragma unroll
+                    for (int j = 0; j < kTaps; ++j) w += t[j] * win[(u + j) & (kTaps - 1)];
+                    win[u] = rb[q0 + u + kTaps];
+                    const int q = q0 + u, col = q * c.Q + p;
+                    if (q < nq && w > best && !bit(s.inP, col) && !bit(s.excl, col)) {
+                        best = w;
+                        bi = col;
+                    }
+                }
+            }
+        }
+    } else {
+        for (int col = lane; col < c.n_pulse; col += kLanes) {
+            if (bit(s.inP, col) || bit(s.excl, col)) continue;
+            const int q = col / c.Q;
+            const double w = adot(s, c, q, col - q * c.Q, s.r);
+            if (w > best) { best = w; bi = col; }
+        }
+    }
+    for (int o = 32; o; o >>= 1) {                 // max, first index on ties
+        const double bo = __shfl_xor(best, o);
+        const int io = __shfl_xor(bi, o);
+        if (bo > best || (bo == best && io < bi)) { best = bo; bi = io; }
+    }
+}
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void channel_kernel(ChanArgs g, DevCfg c) {
     extern __shared__ double smem[];
     const int lane = threadIdx.x;
     const int nw32 = (c.n_pulse + 31) / 32;
@@ -329,12 +439,9 @@ __global__ __launch_bounds__(64) void channel_kernel(ChanArgs g, DevCfg c) {
     s.buf = s.T + c.J * c.Q;
     const int nbuf = max(c.n_wave, tri(c.n_adc));
     s.b = s.buf + nbuf;
-    s.r = s.b + c.n_adc;
-    s.mv_x = s.r + c.n_adc;
-    s.mv_atb = s.mv_x + kMaxPos;
-    s.inP = (uint32_t*)(s.mv_atb + kMaxPos);
+    s.r = s.b + c.n_adc + kPadLo;
+    s.inP = (uint32_t*)(s.r + c.n_adc + kPadHi);
     s.excl = s.inP + nw32;
-    s.mv_col = (int*)(s.excl + nw32);
     for (int i = lane; i < c.J * c.Q; i += kLanes) s.T[i] = c.table[i];
     __syncthreads();
     const uint32_t k0 = (uint32_t)g.seed, k1 = (uint32_t)(g.seed >> 32);
@@ -445,6 +552,8 @@ __global__ __launch_bounds__(64) void channel_kernel(ChanArgs g, DevCfg c) {
                 if (g.out.waveform) g.out.waveform[(int64_t)ci * c.n_adc + k] = acc;
             }
             for (int i = lane; i < 2 * nw32; i += kLanes) s.inP[i] = 0u;   // inP and excl
+            for (int i = lane; i < kPadLo + kPadHi; i += kLanes)
+                (i < kPadLo ? s.r[i - kPadLo] : s.r[c.n_adc + i - kPadLo]) = 0.0;
         }
         __syncthreads();
 
@@ -453,29 +562,20 @@ __global__ __launch_bounds__(64) void channel_kernel(ChanArgs g, DevCfg c) {
         int64_t iter = 0;
         int outer = 0;
         for (;;) {
-            double best = -INFINITY;
-            int bi = 0x7fffffff;
-            for (int col = lane; col < c.n_pulse; col += kLanes) {
-                if (bit(s.inP, col) || bit(s.excl, col)) continue;
-                const double w = adot(s, c, col, s.r);
-                if (w > best) { best = w; bi = col; }
-            }
-            for (int o = 32; o; o >>= 1) {                 // max, first index on ties
-                const double bo = __shfl_xor(best, o);
-                const int io = __shfl_xor(bi, o);
-                if (bo > best || (bo == best && io < bi)) { best = bo; bi = io; }
-            }
+            double best;
+            int bi;
+            w_argmax(s, c, lane, best, bi);
             if (!(best > c.tol) || iter >= c.max_iter || a.P >= c.n_pulse) break;
             if (++outer > c.max_outer) { status |= NTPMT_ST_ITER_CAP; break; }
             if (a.P >= c.n_adc) { status |= NTPMT_ST_RANK_CAP; break; }
-            const double atb_col = adot(s, c, bi, s.b);
+            const double atb_col = adot(s, c, bi / c.Q, bi - (bi / c.Q) * c.Q, s.b);
             if (!chol_append(s, c, a, a.P, bi, atb_col, lane)) {
                 if (lane == 0) s.excl[bi >> 5] |= 1u << (bi & 31);
                 status |= NTPMT_ST_DEGENERATE;
                 __syncthreads();
                 continue;
             }
-            if (a.P < kLanes) { if (lane == a.P) a.xA = 0.0; } else if (lane == a.P - kLanes) a.xB = 0.0;
+            setp(a.xA, a.xB, a.P, 0.0, lane);
             if (lane == 0) s.inP[bi >> 5] |= 1u << (bi & 31);
             a.P += 1;
             __syncthreads();
@@ -491,52 +591,16 @@ __global__ __launch_bounds__(64) void channel_kernel(ChanArgs g, DevCfg c) {
                 if (!(al < INFINITY)) al = 0.0;            // only 0/0 quotients: keep x, drop x ≤ tol
                 if (vA) a.xA += al * (a.sA - a.xA);
                 if (vB) a.xB += al * (a.sB - a.xB);
-                // drop positions with x ≤ tol, keep the order of the rest
+                // drop positions with x ≤ tol (highest first, so lower positions keep their index)
                 const bool kA = vA && a.xA > c.tol, kB = vB && a.xB > c.tol;
-                const uint64_t mA = __ballot(kA), mB = __ballot(kB);
-                const uint64_t vmA = __ballot(vA), vmB = __ballot(vB);
-                const int nA = __popcll(mA);
-                const int newP = nA + __popcll(mB);
-                if (newP == a.P) { status |= NTPMT_ST_ITER_CAP; break; }    // no progress: cannot happen exactly
-                // first dropped position: factor rows before it stay valid
-                const uint64_t dA = vmA & ~mA, dB = vmB & ~mB;
-                const int first = dA ? __builtin_ctzll(dA) : kLanes + __builtin_ctzll(dB);
-                if (vA && !kA) atomicAnd(&s.inP[a.colA >> 5], ~(1u << (a.colA & 31)));
-                if (vB && !kB) atomicAnd(&s.inP[a.colB >> 5], ~(1u << (a.colB & 31)));
-                const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-                if (kA) {
-                    const int np = __popcll(mA & below);
-                    s.mv_col[np] = a.colA; s.mv_x[np] = a.xA; s.mv_atb[np] = a.atbA;
-                }
-                if (kB) {
-                    const int np = nA + __popcll(mB & below);
-                    s.mv_col[np] = a.colB; s.mv_x[np] = a.xB; s.mv_atb[np] = a.atbB;
-                }
-                __syncthreads();
-                a.P = newP;
-                a.colA = lane < newP ? s.mv_col[lane] : 0;
-                a.xA = lane < newP ? s.mv_x[lane] : 0.0;
-                a.atbA = lane < newP ? s.mv_atb[lane] : 0.0;
-                a.colB = lane + kLanes < newP ? s.mv_col[lane + kLanes] : 0;
-                a.xB = lane + kLanes < newP ? s.mv_x[lane + kLanes] : 0.0;
-                a.atbB = lane + kLanes < newP ? s.mv_atb[lane + kLanes] : 0.0;
-                __syncthreads();
-                // rows 0 … first−1 of the factor and z are unchanged; rebuild the rest
-                int t = first;
-                for (; t < newP; ++t) {
-                    const int col = getpi(a.colA, a.colB, t);
-                    const double atb = getp(a.atbA, a.atbB, t);
-                    if (!chol_append(s, c, a, t, col, atb, lane)) break;
-                }
-                if (t < newP) {                            // a kept column became dependent: drop it and the rest
-                    status |= NTPMT_ST_DEGENERATE;
-                    if (lane >= t && lane < newP) { atomicAnd(&s.inP[a.colA >> 5], ~(1u << (a.colA & 31))); a.xA = 0.0; }
-                    if (lane + kLanes >= t && lane + kLanes < newP) {
-                        atomicAnd(&s.inP[a.colB >> 5], ~(1u << (a.colB & 31)));
-                        a.xB = 0.0;
-                    }
-                    __syncthreads();
-                    a.P = t;
+                uint64_t dA = __ballot(vA && !kA), dB = __ballot(vB && !kB);
+                if (!dA && !dB) { status |= NTPMT_ST_ITER_CAP; break; }    // no progress: cannot happen exactly
+                while (dA | dB) {
+                    const int r = dB ? kLanes + 63 - __builtin_clzll(dB) : 63 - __builtin_clzll(dA);
+                    const int col = getpi(a.colA, a.colB, r);
+                    if (lane == 0) s.inP[col >> 5] &= ~(1u << (col & 31));
+                    chol_delete(s, a, r, lane);
+                    if (r < kLanes) dA &= ~(1ull << r); else dB &= ~(1ull << (r - kLanes));
                 }
                 back_solve(s, a, lane);
             }
@@ -794,8 +858,8 @@ int32_t ntpmt_create(int32_t device, const ntpmt_config* c, ntpmt_ctx** out) {
     d.table = x->d_table;
     const size_t nw32 = (size_t)(n_pulse + 31) / 32;
     x->lds_bytes = sizeof(double) * ((size_t)J * Q + std::max<size_t>(n_wave, (size_t)n_adc * (n_adc + 1) / 2) +
-                                     2 * (size_t)n_adc + 2 * kMaxPos) +
-                   sizeof(uint32_t) * 2 * nw32 + sizeof(int) * kMaxPos;
+                                     2 * (size_t)n_adc + kPadLo + kPadHi) +
+                   sizeof(uint32_t) * 2 * nw32;
     if (x->lds_bytes > 64 * 1024) return cleanup(arg_error("configuration needs more than 64 KB of LDS per channel"));
     int per_cu = 0;
     if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, channel_kernel, 64, x->lds_bytes)) != hipSuccess)
