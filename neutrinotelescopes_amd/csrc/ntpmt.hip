@@ -8,7 +8,8 @@
 //      an exclusive scan of the run lengths (hipCUB): every channel's hits are
 //      contiguous and in time order;
 //   3. channel_kernel: one 64-lane workgroup per channel, taken from an atomic
-//      work counter (bright channels cost more).  In LDS: the waveform
+//      work counter over the channels sorted by decreasing hit count (bright
+//      channels cost more: they go first, so none is left alone at the end).  In LDS: the waveform
 //      (n_wave FP64 samples, each a sum over the hits within the pulse's
 //      support, found by binary search in the sorted keys, plus noise), the
 //      first-order low-pass as a parallel scan of affine maps over the lanes,
@@ -87,6 +88,7 @@ struct ChanArgs {
     const int32_t* counts;     // run lengths
     const int32_t* offsets;    // run starts
     const int32_t* n_runs;
+    const int32_t* order;      // runs by decreasing hit count (heaviest first)
     int32_t* next;             // work counter
     int64_t capacity;
     uint64_t seed;
@@ -181,6 +183,17 @@ __global__ void finish_count_kernel(const int32_t* n_runs, const uint32_t* uniqu
     *next = 0;
 }
 
+// sort keys that put the channel runs in decreasing order of their hit count
+// (the run of unselected hits and the unused entries last)
+__global__ void order_keys_kernel(const int32_t* n_runs, const uint32_t* unique, const int32_t* counts, int64_t n,
+                                  uint32_t* key, int32_t* run) {
+    const int32_t r = *n_runs;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        key[i] = (i < r && unique[i] != kNoChannel) ? ~(uint32_t)counts[i] : 0xFFFFFFFFu;
+        run[i] = (int32_t)i;
+    }
+}
+
 __global__ void zero_count_kernel(int64_t* n_channels) {
     if (n_channels) *n_channels = 0;
 }
@@ -224,6 +237,8 @@ struct Lds {
     double* buf;        // waveform [n_wave], then the Cholesky factor (packed rows)
     double* b;          // [n_adc] digitised waveform
     double* r;          // [n_adc] residual b − A x, inside kPadLo / kPadHi zeros
+    int2* pos_qp;       // [kMaxPos] (q, p) of the active positions (residual)
+    double* pos_x;      // [kMaxPos] x of the active positions (residual)
     uint32_t* inP;      // [n_pulse / 32] active set
     uint32_t* excl;     // [n_pulse / 32] columns skipped as dependent
 };
@@ -233,7 +248,7 @@ __device__ __forceinline__ double adot(const Lds& s, const DevCfg& c, int q, int
     double acc = 0.0;
     for (int j = 0; j < c.J; ++j) {
         const int k = q + c.m0 + j;
-        if (k >= 0 && k < c.n_adc) acc += s.T[j * c.Q + p] * v[k];
+        if (k >= 0 && k < c.n_adc) acc = fma(s.T[j * c.Q + p], v[k], acc);
     }
     return acc;
 }
@@ -243,7 +258,7 @@ __device__ __forceinline__ double gram(const Lds& s, const DevCfg& c, int qi, in
     double acc = 0.0;
     for (int ji = 0; ji < c.J; ++ji) {
         const int k = qi + c.m0 + ji, jj = ji + qi - qj;
-        if (k >= 0 && k < c.n_adc && jj >= 0 && jj < c.J) acc += s.T[ji * c.Q + pi] * s.T[jj * c.Q + pj];
+        if (k >= 0 && k < c.n_adc && jj >= 0 && jj < c.J) acc = fma(s.T[ji * c.Q + pi], s.T[jj * c.Q + pj], acc);
     }
     return acc;
 }
@@ -269,11 +284,23 @@ __device__ __forceinline__ bool chol_append(const Lds& s, const DevCfg& c, Activ
     double gA = (lane < t) ? gram(s, c, a.qA, a.pA, cq, cp) : 0.0;
     double gB = (lane + kLanes < t) ? gram(s, c, a.qB, a.pB, cq, cp) : 0.0;
     double lA = 0.0, lB = 0.0;
+    // column-oriented forward substitution; each lane's factor entry of the
+    // next step is loaded a step ahead (the LDS latency stays off the chain
+    // readlane → multiply → fma)
+    const double* rowA = s.buf + tri(lane);
+    const double* rowB = s.buf + tri(lane + kLanes);
+    double eA = (lane > 0 && lane < t) ? rowA[0] : 0.0;
+    double eB = (lane + kLanes < t) ? rowB[0] : 0.0;
     for (int j = 0; j < t; ++j) {
+        const int jn = j + 1;
+        const double nA = (lane > jn && lane < t) ? rowA[jn] : 0.0;
+        const double nB = (lane + kLanes > jn && lane + kLanes < t) ? rowB[jn] : 0.0;
         const double lj = getp(gA, gB, j) * getp(a.dA, a.dB, j);
         setp(lA, lB, j, lj, lane);
-        if (lane > j && lane < t) gA -= s.buf[tri(lane) + j] * lj;
-        if (lane + kLanes > j && lane + kLanes < t) gB -= s.buf[tri(lane + kLanes) + j] * lj;
+        gA = fma(-eA, lj, gA);                     // eA = 0 on the lanes this step leaves alone
+        gB = fma(-eB, lj, gB);
+        eA = nA;
+        eB = nB;
     }
     const double gcc = gram(s, c, cq, cp, cq, cp);
     const double d2 = gcc - wsum(lA * lA + lB * lB);
@@ -300,16 +327,25 @@ __device__ __forceinline__ bool chol_append(const Lds& s, const DevCfg& c, Activ
     return true;
 }
 
-// s = L⁻ᵀ z over positions 0 … P−1 (0 beyond)
+// s = L⁻ᵀ z over positions 0 … P−1 (0 beyond); row j of the factor is
+// loaded one step ahead
 __device__ __forceinline__ void back_solve(const Lds& s, Active& a, int lane) {
     double zA = a.zA, zB = a.zB;
     a.sA = 0.0;
     a.sB = 0.0;
-    for (int j = a.P - 1; j >= 0; --j) {
+    int j = a.P - 1;
+    double eA = (lane < j) ? s.buf[tri(j) + lane] : 0.0;
+    double eB = (lane + kLanes < j) ? s.buf[tri(j) + lane + kLanes] : 0.0;
+    for (; j >= 0; --j) {
+        const int jn = j - 1;
+        const double nA = (lane < jn) ? s.buf[tri(jn) + lane] : 0.0;
+        const double nB = (lane + kLanes < jn) ? s.buf[tri(jn) + lane + kLanes] : 0.0;
         const double sj = getp(zA, zB, j) * getp(a.dA, a.dB, j);
         setp(a.sA, a.sB, j, sj, lane);
-        if (lane < j) zA -= s.buf[tri(j) + lane] * sj;
-        if (lane + kLanes < j) zB -= s.buf[tri(j) + lane + kLanes] * sj;
+        zA = fma(-eA, sj, zA);
+        zB = fma(-eB, sj, zB);
+        eA = nA;
+        eB = nB;
     }
 }
 
@@ -365,17 +401,28 @@ __device__ __forceinline__ void chol_delete(const Lds& s, Active& a, int r, int 
     setp(a.zA, a.zB, a.P, 0.0, lane);
 }
 
-// r = b − A x: each lane holds rows lane and lane + 64 and walks the active
-// positions (their q, p and x broadcast by readlane)
+// r = b − A x: the positions' (q, p, x) are published in LDS, then each lane
+// accumulates its rows lane and lane + 64 over them (loads unrolled ahead of
+// the multiply-adds)
 __device__ __forceinline__ void residual(const Lds& s, const DevCfg& c, const Active& a, int lane) {
+    if (lane < a.P) {
+        s.pos_qp[lane] = make_int2(a.qA, a.pA);
+        s.pos_x[lane] = a.xA;
+    }
+    if (lane + kLanes < a.P) {
+        s.pos_qp[lane + kLanes] = make_int2(a.qB, a.pB);
+        s.pos_x[lane + kLanes] = a.xB;
+    }
+    __syncthreads();
     const int k0 = lane, k1 = lane + kLanes;
     double r0 = k0 < c.n_adc ? s.b[k0] : 0.0, r1 = k1 < c.n_adc ? s.b[k1] : 0.0;
+#pragma unroll 4
     for (int j = 0; j < a.P; ++j) {
-        const int qj = getpi(a.qA, a.qB, j), pj = getpi(a.pA, a.pB, j);
-        const double xj = getp(a.xA, a.xB, j);
-        const int l0 = k0 - qj - c.m0, l1 = k1 - qj - c.m0;
-        if (l0 >= 0 && l0 < c.J) r0 -= s.T[l0 * c.Q + pj] * xj;
-        if (l1 >= 0 && l1 < c.J) r1 -= s.T[l1 * c.Q + pj] * xj;
+        const int2 qp = s.pos_qp[j];
+        const double xj = s.pos_x[j];
+        const int l0 = k0 - qp.x - c.m0, l1 = k1 - qp.x - c.m0;
+        if (l0 >= 0 && l0 < c.J) r0 = fma(-s.T[l0 * c.Q + qp.y], xj, r0);
+        if (l1 >= 0 && l1 < c.J) r1 = fma(-s.T[l1 * c.Q + qp.y], xj, r1);
     }
     if (k0 < c.n_adc) s.r[k0] = r0;
     if (k1 < c.n_adc) s.r[k1] = r1;
@@ -387,10 +434,11 @@ __device__ __forceinline__ void residual(const Lds& s, const DevCfg& c, const Ac
 // configuration): lane p < Q takes the columns q·Q + p, its pattern's taps in
 // registers and a window of kTaps residual rows rotating through registers
 // (one broadcast LDS read per column); otherwise one column per lane and step.
+template <int kJ>
 __device__ __forceinline__ void w_argmax(const Lds& s, const DevCfg& c, int lane, double& best, int& bi) {
     best = -INFINITY;
     bi = 0x7fffffff;
-    if (c.J <= kTaps && c.m0 >= -kPadLo && c.m0 <= 0) {
+    if (c.J <= kJ && c.m0 >= -kPadLo && c.m0 <= 0) {
         if (lane < c.Q) {
             const int p = lane;
             double t[kTaps], win[kTaps];
@@ -405,7 +453,7 @@ __device__ __forceinline__ void w_argmax(const Lds& s, const DevCfg& c, int lane
                 for (int u = 0; u < kTaps; ++u) {
                     double w = 0.0;
 #pragma unroll
-                    for (int j = 0; j < kTaps; ++j) w += t[j] * win[(u + j) & (kTaps - 1)];
+                    for (int j = 0; j < kJ; ++j) w = fma(t[j], win[(u + j) & (kTaps - 1)], w);
                     win[u] = rb[q0 + u + kTaps];
                     const int q = q0 + u, col = q * c.Q + p;
                     if (q < nq && w > best && !bit(s.inP, col) && !bit(s.excl, col)) {
@@ -430,6 +478,7 @@ __device__ __forceinline__ void w_argmax(const Lds& s, const DevCfg& c, int lane
     }
 }
 
+template <int kJ>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void channel_kernel(ChanArgs g, DevCfg c) {
     extern __shared__ double smem[];
     const int lane = threadIdx.x;
@@ -440,7 +489,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
     const int nbuf = max(c.n_wave, tri(c.n_adc));
     s.b = s.buf + nbuf;
     s.r = s.b + c.n_adc + kPadLo;
-    s.inP = (uint32_t*)(s.r + c.n_adc + kPadHi);
+    s.pos_x = s.r + c.n_adc + kPadHi;
+    s.pos_qp = (int2*)(s.pos_x + kMaxPos);
+    s.inP = (uint32_t*)(s.pos_qp + kMaxPos);
     s.excl = s.inP + nw32;
     for (int i = lane; i < c.J * c.Q; i += kLanes) s.T[i] = c.table[i];
     __syncthreads();
@@ -448,10 +499,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
     const int n_runs = *g.n_runs;
 
     for (;;) {
-        int ci = 0;
-        if (lane == 0) ci = atomicAdd(g.next, 1);
-        ci = __shfl(ci, 0);
-        if (ci >= n_runs) break;
+        int wi = 0;
+        if (lane == 0) wi = atomicAdd(g.next, 1);
+        wi = __shfl(wi, 0);
+        if (wi >= n_runs) break;
+        const int ci = g.order[wi];
         const uint32_t ch = g.unique[ci];
         if (ch == kNoChannel || ci >= g.capacity) continue;
         const int beg = g.offsets[ci], n = g.counts[ci];
@@ -564,7 +616,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         for (;;) {
             double best;
             int bi;
-            w_argmax(s, c, lane, best, bi);
+            w_argmax<kJ>(s, c, lane, best, bi);
             if (!(best > c.tol) || iter >= c.max_iter || a.P >= c.n_pulse) break;
             if (++outer > c.max_outer) { status |= NTPMT_ST_ITER_CAP; break; }
             if (a.P >= c.n_adc) { status |= NTPMT_ST_RANK_CAP; break; }
@@ -664,6 +716,7 @@ struct ntpmt_ctx {
     double* d_table = nullptr;
     size_t lds_bytes = 0;
     int blocks = 0;
+    const void* kernel = nullptr;       // channel_kernel instance for the configuration's pattern length
     char* d_ws = nullptr;
     size_t ws_cap = 0;
     hipEvent_t last = nullptr;
@@ -706,7 +759,8 @@ double filt_pulse_at(const ntpmt_config* c, double s) {
 }
 
 struct WsLayout {
-    size_t key_a, key_b, idx_a, idx_b, tq, hflag, ch, unique, counts, offsets, n_runs, next, cub, cub_bytes, total;
+    size_t key_a, key_b, idx_a, idx_b, tq, hflag, ch, unique, counts, offsets, n_runs, next, okey_a, okey_b, orun_a,
+        orun_b, cub, cub_bytes, total;
 };
 
 WsLayout ws_layout(int64_t n) {
@@ -726,13 +780,19 @@ WsLayout ws_layout(int64_t n) {
     L.offsets = take(4 * nn);
     L.n_runs = take(8);
     L.next = take(8);
-    size_t t1 = 0, t2 = 0, t3 = 0;
+    L.okey_a = take(4 * nn);
+    L.okey_b = take(4 * nn);
+    L.orun_a = take(4 * nn);
+    L.orun_b = take(4 * nn);
+    size_t t1 = 0, t2 = 0, t3 = 0, t4 = 0;
     (void)hipcub::DeviceRadixSort::SortPairs(nullptr, t1, (const uint64_t*)nullptr, (uint64_t*)nullptr,
                                              (const int32_t*)nullptr, (int32_t*)nullptr, (int)nn, 0, 53);
     (void)hipcub::DeviceRunLengthEncode::Encode(nullptr, t2, (const uint32_t*)nullptr, (uint32_t*)nullptr,
                                                 (int32_t*)nullptr, (int32_t*)nullptr, (int)nn);
     (void)hipcub::DeviceScan::ExclusiveSum(nullptr, t3, (const int32_t*)nullptr, (int32_t*)nullptr, (int)nn);
-    L.cub_bytes = std::max(t1, std::max(t2, t3));
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, t4, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                             (const int32_t*)nullptr, (int32_t*)nullptr, (int)nn, 0, 32);
+    L.cub_bytes = std::max(std::max(t1, t2), std::max(t3, t4));
     L.cub = take(L.cub_bytes);
     L.total = o;
     return L;
@@ -790,10 +850,21 @@ int32_t ntpmt_create(int32_t device, const ntpmt_config* c, ntpmt_ctx** out) {
     const int m1 = (int)std::floor((fp_end + c->unf_pulse_res * (double)(Q - 1)) / dt_adc + 1e-9);
     const int J = m1 - m0 + 1;
     if (J < 1 || J > 256) return arg_error("the filtered pulse spans more than 256 digitiser samples");
-    std::vector<double> table((size_t)J * Q);
+    std::vector<double> full((size_t)J * Q);
     for (int j = 0; j < J; ++j)
         for (int p = 0; p < Q; ++p)
-            table[(size_t)j * Q + p] = filt_pulse_at(c, dt_adc * (double)(j + m0) - c->unf_pulse_res * (double)p);
+            full[(size_t)j * Q + p] = filt_pulse_at(c, dt_adc * (double)(j + m0) - c->unf_pulse_res * (double)p);
+    // rows that are zero for every pattern contribute nothing: drop them
+    auto zero_row = [&](int j) {
+        for (int p = 0; p < Q; ++p)
+            if (full[(size_t)j * Q + p] != 0.0) return false;
+        return true;
+    };
+    int j_lo = 0, j_hi = J - 1;
+    while (j_lo < j_hi && zero_row(j_lo)) ++j_lo;
+    while (j_hi > j_lo && zero_row(j_hi)) --j_hi;
+    const int m0_trim = m0 + j_lo, J_trim = j_hi - j_lo + 1;
+    std::vector<double> table(full.begin() + (size_t)j_lo * Q, full.begin() + (size_t)(j_hi + 1) * Q);
 
     ntpmt_ctx* x = new ntpmt_ctx();
     x->device = device;
@@ -849,20 +920,21 @@ int32_t ntpmt_create(int32_t device, const ntpmt_config* c, ntpmt_ctx** out) {
     d.n_pulse = (int)n_pulse;
     d.R = (int)R;
     d.Q = (int)Q;
-    d.J = J;
-    d.m0 = m0;
+    d.J = J_trim;
+    d.m0 = m0_trim;
     d.n_taps = c->n_decim_taps;
     d.chunk = (int)((n_wave + kLanes - 1) / kLanes);
     d.max_outer = 8 * (int)n_adc + 64;
     d.taps = x->d_taps;
     d.table = x->d_table;
     const size_t nw32 = (size_t)(n_pulse + 31) / 32;
-    x->lds_bytes = sizeof(double) * ((size_t)J * Q + std::max<size_t>(n_wave, (size_t)n_adc * (n_adc + 1) / 2) +
-                                     2 * (size_t)n_adc + kPadLo + kPadHi) +
+    x->lds_bytes = sizeof(double) * ((size_t)J_trim * Q + std::max<size_t>(n_wave, (size_t)n_adc * (n_adc + 1) / 2) +
+                                     2 * (size_t)n_adc + kPadLo + kPadHi + 2 * kMaxPos) +
                    sizeof(uint32_t) * 2 * nw32;
     if (x->lds_bytes > 64 * 1024) return cleanup(arg_error("configuration needs more than 64 KB of LDS per channel"));
+    x->kernel = J_trim <= 14 ? (const void*)channel_kernel<14> : (const void*)channel_kernel<16>;
     int per_cu = 0;
-    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, channel_kernel, 64, x->lds_bytes)) != hipSuccess)
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, x->kernel, 64, x->lds_bytes)) != hipSuccess)
         return cleanup(fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor"));
     x->blocks = std::max(1, per_cu) * x->num_cus;
     *out = x;
@@ -949,8 +1021,18 @@ int32_t ntpmt_make_reco_pulses(ntpmt_ctx* x, const nt_hits* h, const uint8_t* d_
     PMT_TRY(hipcub::DeviceScan::ExclusiveSum(b + L.cub, cub, counts, offs, (int)n, s));
     finish_count_kernel<<<1, 1, 0, s>>>(n_runs, uniq, d_n_channels, next);
     PMT_TRY(hipGetLastError());
-    ChanArgs a{key_b, idx_b, tq, hflag, uniq, counts, offs, n_runs, next, capacity, seed, *out};
-    channel_kernel<<<x->blocks, 64, x->lds_bytes, s>>>(a, x->cfg);
+    auto* okey_a = (uint32_t*)(b + L.okey_a);
+    auto* okey_b = (uint32_t*)(b + L.okey_b);
+    auto* orun_a = (int32_t*)(b + L.orun_a);
+    auto* orun_b = (int32_t*)(b + L.orun_b);
+    order_keys_kernel<<<grid, 256, 0, s>>>(n_runs, uniq, counts, n, okey_a, orun_a);
+    PMT_TRY(hipGetLastError());
+    cub = L.cub_bytes;
+    PMT_TRY(hipcub::DeviceRadixSort::SortPairs(b + L.cub, cub, okey_a, okey_b, orun_a, orun_b, (int)n, 0, 32, s));
+    ChanArgs a{key_b, idx_b, tq, hflag, uniq, counts, offs, n_runs, orun_b, next, capacity, seed, *out};
+    DevCfg cfg = x->cfg;
+    void* kargs[] = {&a, &cfg};
+    PMT_TRY(hipLaunchKernel(x->kernel, dim3(x->blocks), dim3(64), kargs, x->lds_bytes, s));
     PMT_TRY(hipGetLastError());
     PMT_TRY(hipEventRecord(x->last, s));
     return NT_OK;
