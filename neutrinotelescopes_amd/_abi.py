@@ -204,7 +204,7 @@ class ntpmt_config(C.Structure):
         ("nnls_max_iter", C.c_int64),
         ("decim_taps", C.POINTER(C.c_double)), ("n_decim_taps", C.c_int32),
         ("filt_pulse", C.POINTER(C.c_double)), ("n_filt_pulse", C.c_int32),
-        ("filt_pulse_t0", C.c_double), ("filt_pulse_dt", C.c_double)]
+        ("filt_pulse_t0", C.c_double), ("filt_pulse_dt", C.c_double), ("lds_factor_rows", C.c_int32)]
 
 
 class ntpmt_pulses(C.Structure):

@@ -55,6 +55,7 @@ constexpr int kMaxPos = 2 * kLanes;                  // active-set positions per
 constexpr double kDegenerate = 1e-14;                // a column with d² ≤ this · ‖a‖² is numerically dependent
 constexpr int kTaps = 16;                            // rows a column reaches in the register-window gradient pass
 constexpr int kPadLo = 16, kPadHi = 48;              // zero rows around the residual (padded LDS array)
+constexpr int kLdsRows = 53;                         // factor rows kept in LDS by default; later rows in HBM
 
 struct DevCfg {
     double tt_d, tt_c, tt_scale, tt_subtract;
@@ -64,6 +65,7 @@ struct DevCfg {
     double b0, b1, a1, tol;
     int64_t max_iter;
     int n_wave, n_adc, n_pulse, R, Q, J, m0, n_taps, chunk, max_outer;
+    int lds_rows;            // factor rows kept in LDS
     const double* taps;      // device [n_taps]
     const double* table;     // device [J][Q]
 };
@@ -80,6 +82,8 @@ struct HitIn {
 };
 
 struct ChanArgs {
+    double* spill;             // [gridDim.x][spill_stride] factor rows beyond the LDS ones
+    int64_t spill_stride;
     const uint64_t* key;       // sorted keys
     const int32_t* idx;        // hit index of each sorted key
     const double* tq;          // [n][2] (t', charge) by hit index
@@ -234,7 +238,9 @@ __device__ __forceinline__ int tri(int i) { return i * (i + 1) / 2; }
 
 struct Lds {
     double* T;          // [J][Q]
-    double* buf;        // waveform [n_wave], then the Cholesky factor (packed rows)
+    double* buf;        // waveform [n_wave], then the Cholesky factor's rows 0 … K−1 (packed)
+    double* glb;        // the factor's rows K … (packed, this workgroup's HBM region)
+    int K, triK;        // rows in LDS, tri(K)
     double* b;          // [n_adc] digitised waveform
     double* r;          // [n_adc] residual b − A x, inside kPadLo / kPadHi zeros
     int2* pos_qp;       // [kMaxPos] (q, p) of the active positions (residual)
@@ -265,6 +271,20 @@ __device__ __forceinline__ double gram(const Lds& s, const DevCfg& c, int qi, in
 
 __device__ __forceinline__ bool bit(const uint32_t* m, int i) { return (m[i >> 5] >> (i & 31)) & 1u; }
 
+// Factor entry (i, j), i ≥ j.  Rows below K sit in LDS; the rare rows beyond
+// (more than K active columns) in the workgroup's HBM region.  kSpill = false
+// is the all-LDS form the callers use while every row involved is below K.
+template <bool kSpill>
+__device__ __forceinline__ double fac_ld(const Lds& s, int i, int j) {
+    if (!kSpill || i < s.K) return s.buf[tri(i) + j];
+    return s.glb[tri(i) - s.triK + j];
+}
+template <bool kSpill>
+__device__ __forceinline__ void fac_st(const Lds& s, int i, int j, double v) {
+    if (!kSpill || i < s.K) s.buf[tri(i) + j] = v;
+    else s.glb[tri(i) - s.triK + j] = v;
+}
+
 // Active set: positions 0 … P−1, position j in lane j mod 64, register A (j < 64) or B.
 struct Active {
     int P = 0;
@@ -278,6 +298,7 @@ struct Active {
 // (forward substitution with the Gram row of the active columns) and z by z_t.
 // Returns false (nothing written) if the column is numerically dependent on
 // positions 0 … t−1.
+template <bool kSpill>
 __device__ __forceinline__ bool chol_append(const Lds& s, const DevCfg& c, Active& a, int t, int col, double atb_col,
                                             int lane) {
     const int cq = col / c.Q, cp = col - cq * c.Q;
@@ -287,14 +308,13 @@ __device__ __forceinline__ bool chol_append(const Lds& s, const DevCfg& c, Activ
     // column-oriented forward substitution; each lane's factor entry of the
     // next step is loaded a step ahead (the LDS latency stays off the chain
     // readlane → multiply → fma)
-    const double* rowA = s.buf + tri(lane);
-    const double* rowB = s.buf + tri(lane + kLanes);
-    double eA = (lane > 0 && lane < t) ? rowA[0] : 0.0;
-    double eB = (lane + kLanes < t) ? rowB[0] : 0.0;
+    const int iA = lane, iB = lane + kLanes;
+    double eA = (iA > 0 && iA < t) ? fac_ld<kSpill>(s, iA, 0) : 0.0;
+    double eB = (iB < t) ? fac_ld<kSpill>(s, iB, 0) : 0.0;
     for (int j = 0; j < t; ++j) {
         const int jn = j + 1;
-        const double nA = (lane > jn && lane < t) ? rowA[jn] : 0.0;
-        const double nB = (lane + kLanes > jn && lane + kLanes < t) ? rowB[jn] : 0.0;
+        const double nA = (iA > jn && iA < t) ? fac_ld<kSpill>(s, iA, jn) : 0.0;
+        const double nB = (iB > jn && iB < t) ? fac_ld<kSpill>(s, iB, jn) : 0.0;
         const double lj = getp(gA, gB, j) * getp(a.dA, a.dB, j);
         setp(lA, lB, j, lj, lane);
         gA = fma(-eA, lj, gA);                     // eA = 0 on the lanes this step leaves alone
@@ -307,9 +327,9 @@ __device__ __forceinline__ bool chol_append(const Lds& s, const DevCfg& c, Activ
     if (!(d2 > kDegenerate * gcc)) return false;
     const double d = sqrt(d2), id = 1.0 / d;
     const double zt = (atb_col - wsum((lane < t ? lA * a.zA : 0.0) + (lane + kLanes < t ? lB * a.zB : 0.0))) * id;
-    if (lane < t) s.buf[tri(t) + lane] = lA;
-    if (lane + kLanes < t) s.buf[tri(t) + lane + kLanes] = lB;
-    if (lane == 0) s.buf[tri(t) + t] = d;
+    if (lane < t) fac_st<kSpill>(s, t, lane, lA);
+    if (lane + kLanes < t) fac_st<kSpill>(s, t, lane + kLanes, lB);
+    if (lane == 0) fac_st<kSpill>(s, t, t, d);
     const bool own_a = t < kLanes && lane == t, own_b = t >= kLanes && lane == t - kLanes;
     a.colA = own_a ? col : a.colA;
     a.qA = own_a ? cq : a.qA;
@@ -329,17 +349,18 @@ __device__ __forceinline__ bool chol_append(const Lds& s, const DevCfg& c, Activ
 
 // s = L⁻ᵀ z over positions 0 … P−1 (0 beyond); row j of the factor is
 // loaded one step ahead
-__device__ __forceinline__ void back_solve(const Lds& s, Active& a, int lane) {
+template <bool kSpill>
+__device__ __forceinline__ void back_solve_t(const Lds& s, Active& a, int lane) {
     double zA = a.zA, zB = a.zB;
     a.sA = 0.0;
     a.sB = 0.0;
     int j = a.P - 1;
-    double eA = (lane < j) ? s.buf[tri(j) + lane] : 0.0;
-    double eB = (lane + kLanes < j) ? s.buf[tri(j) + lane + kLanes] : 0.0;
+    double eA = (lane < j) ? fac_ld<kSpill>(s, j, lane) : 0.0;
+    double eB = (lane + kLanes < j) ? fac_ld<kSpill>(s, j, lane + kLanes) : 0.0;
     for (; j >= 0; --j) {
         const int jn = j - 1;
-        const double nA = (lane < jn) ? s.buf[tri(jn) + lane] : 0.0;
-        const double nB = (lane + kLanes < jn) ? s.buf[tri(jn) + lane + kLanes] : 0.0;
+        const double nA = (lane < jn) ? fac_ld<kSpill>(s, jn, lane) : 0.0;
+        const double nB = (lane + kLanes < jn) ? fac_ld<kSpill>(s, jn, lane + kLanes) : 0.0;
         const double sj = getp(zA, zB, j) * getp(a.dA, a.dB, j);
         setp(a.sA, a.sB, j, sj, lane);
         zA = fma(-eA, sj, zA);
@@ -364,25 +385,25 @@ __device__ __forceinline__ void shift_down(T& A, T& B, int r, int lane) {
 // column pairs (k, k+1), k = r … P−2, remove them (L Q Qᵀ Lᵀ = L Lᵀ) and the
 // same rotations carry z (L z = Aᵀb stays true).  Then the later rows move up
 // one slot and the per-position registers shift down one position.
-__device__ __forceinline__ void chol_delete(const Lds& s, Active& a, int r, int lane) {
+template <bool kSpill>
+__device__ __forceinline__ void chol_delete_t(const Lds& s, Active& a, int r, int lane) {
     const int P = a.P;
     for (int k = r; k < P - 1; ++k) {
-        const int base = tri(k + 1);                     // old row k + 1
-        const double av = s.buf[base + k], bv = s.buf[base + k + 1];
+        const double av = fac_ld<kSpill>(s, k + 1, k), bv = fac_ld<kSpill>(s, k + 1, k + 1);   // old row k + 1
         const double rr = sqrt(av * av + bv * bv), ir = 1.0 / rr;
         const double cs = av * ir, sn = bv * ir;
         for (int i = k + 2 + lane; i < P; i += kLanes) {
-            const double x = s.buf[tri(i) + k], y = s.buf[tri(i) + k + 1];
-            s.buf[tri(i) + k] = cs * x + sn * y;
-            s.buf[tri(i) + k + 1] = cs * y - sn * x;
+            const double x = fac_ld<kSpill>(s, i, k), y = fac_ld<kSpill>(s, i, k + 1);
+            fac_st<kSpill>(s, i, k, cs * x + sn * y);
+            fac_st<kSpill>(s, i, k + 1, cs * y - sn * x);
         }
         const double zk = getp(a.zA, a.zB, k), zk1 = getp(a.zA, a.zB, k + 1);
         setp(a.zA, a.zB, k, cs * zk + sn * zk1, lane);
         setp(a.zA, a.zB, k + 1, cs * zk1 - sn * zk, lane);
         setp(a.dA, a.dB, k, ir, lane);
         if (lane == 0) {
-            s.buf[base + k] = rr;
-            s.buf[base + k + 1] = 0.0;
+            fac_st<kSpill>(s, k + 1, k, rr);
+            fac_st<kSpill>(s, k + 1, k + 1, 0.0);
         }
         __syncthreads();
     }
@@ -390,7 +411,7 @@ __device__ __forceinline__ void chol_delete(const Lds& s, Active& a, int r, int 
     // same columns of every row, so a row's source is read before the next
     // row's copy overwrites it, by the same lane
     for (int i = r + 1; i < P; ++i)
-        for (int col = lane; col < i; col += kLanes) s.buf[tri(i - 1) + col] = s.buf[tri(i) + col];
+        for (int col = lane; col < i; col += kLanes) fac_st<kSpill>(s, i - 1, col, fac_ld<kSpill>(s, i, col));
     __syncthreads();
     shift_down(a.colA, a.colB, r, lane);
     shift_down(a.qA, a.qB, r, lane);
@@ -399,6 +420,19 @@ __device__ __forceinline__ void chol_delete(const Lds& s, Active& a, int r, int 
     shift_down(a.atbA, a.atbB, r, lane);
     a.P = P - 1;
     setp(a.zA, a.zB, a.P, 0.0, lane);
+}
+
+// the all-LDS forms while every row involved is below K
+__device__ __forceinline__ void back_solve(const Lds& s, Active& a, int lane) {
+    if (a.P > s.K) back_solve_t<true>(s, a, lane); else back_solve_t<false>(s, a, lane);
+}
+__device__ __forceinline__ void chol_delete(const Lds& s, Active& a, int r, int lane) {
+    if (a.P > s.K) chol_delete_t<true>(s, a, r, lane); else chol_delete_t<false>(s, a, r, lane);
+}
+__device__ __forceinline__ bool chol_add(const Lds& s, const DevCfg& c, Active& a, int t, int col, double atb_col,
+                                         int lane) {
+    return t >= s.K ? chol_append<true>(s, c, a, t, col, atb_col, lane)
+                    : chol_append<false>(s, c, a, t, col, atb_col, lane);
 }
 
 // r = b − A x: the positions' (q, p, x) are published in LDS, then each lane
@@ -486,7 +520,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
     Lds s;
     s.T = smem;
     s.buf = s.T + c.J * c.Q;
-    const int nbuf = max(c.n_wave, tri(c.n_adc));
+    s.K = c.lds_rows;
+    s.triK = tri(s.K);
+    s.glb = g.spill + (int64_t)blockIdx.x * g.spill_stride;
+    const int nbuf = max(c.n_wave, s.triK);
     s.b = s.buf + nbuf;
     s.r = s.b + c.n_adc + kPadLo;
     s.pos_x = s.r + c.n_adc + kPadHi;
@@ -621,7 +658,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             if (++outer > c.max_outer) { status |= NTPMT_ST_ITER_CAP; break; }
             if (a.P >= c.n_adc) { status |= NTPMT_ST_RANK_CAP; break; }
             const double atb_col = adot(s, c, bi / c.Q, bi - (bi / c.Q) * c.Q, s.b);
-            if (!chol_append(s, c, a, a.P, bi, atb_col, lane)) {
+            if (!chol_add(s, c, a, a.P, bi, atb_col, lane)) {
                 if (lane == 0) s.excl[bi >> 5] |= 1u << (bi & 31);
                 status |= NTPMT_ST_DEGENERATE;
                 __syncthreads();
@@ -717,6 +754,8 @@ struct ntpmt_ctx {
     size_t lds_bytes = 0;
     int blocks = 0;
     const void* kernel = nullptr;       // channel_kernel instance for the configuration's pattern length
+    double* d_spill = nullptr;          // factor rows beyond the LDS ones, per workgroup
+    int64_t spill_stride = 0;
     char* d_ws = nullptr;
     size_t ws_cap = 0;
     hipEvent_t last = nullptr;
@@ -928,7 +967,10 @@ int32_t ntpmt_create(int32_t device, const ntpmt_config* c, ntpmt_ctx** out) {
     d.taps = x->d_taps;
     d.table = x->d_table;
     const size_t nw32 = (size_t)(n_pulse + 31) / 32;
-    x->lds_bytes = sizeof(double) * ((size_t)J_trim * Q + std::max<size_t>(n_wave, (size_t)n_adc * (n_adc + 1) / 2) +
+    if (c->lds_factor_rows < 0) return cleanup(arg_error("lds_factor_rows >= 0"));
+    const size_t K = (size_t)std::min<int64_t>(n_adc, c->lds_factor_rows > 0 ? c->lds_factor_rows : kLdsRows);
+    d.lds_rows = (int)K;
+    x->lds_bytes = sizeof(double) * ((size_t)J_trim * Q + std::max<size_t>(n_wave, K * (K + 1) / 2) +
                                      2 * (size_t)n_adc + kPadLo + kPadHi + 2 * kMaxPos) +
                    sizeof(uint32_t) * 2 * nw32;
     if (x->lds_bytes > 64 * 1024) return cleanup(arg_error("configuration needs more than 64 KB of LDS per channel"));
@@ -937,6 +979,10 @@ int32_t ntpmt_create(int32_t device, const ntpmt_config* c, ntpmt_ctx** out) {
     if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, x->kernel, 64, x->lds_bytes)) != hipSuccess)
         return cleanup(fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor"));
     x->blocks = std::max(1, per_cu) * x->num_cus;
+    x->spill_stride = (int64_t)(n_adc * (n_adc + 1) / 2) - (int64_t)(K * (K + 1) / 2);
+    if (x->spill_stride > 0 &&
+        (e = hipMalloc(&x->d_spill, sizeof(double) * (size_t)x->spill_stride * x->blocks)) != hipSuccess)
+        return cleanup(fail(e, "hipMalloc"));
     *out = x;
     return NT_OK;
 }
@@ -948,6 +994,7 @@ void ntpmt_destroy(ntpmt_ctx* x) {
     if (x->d_ws) (void)hipFree(x->d_ws);
     if (x->d_taps) (void)hipFree(x->d_taps);
     if (x->d_table) (void)hipFree(x->d_table);
+    if (x->d_spill) (void)hipFree(x->d_spill);
     if (x->last) (void)hipEventDestroy(x->last);
     if (x->stream) (void)hipStreamDestroy(x->stream);
     delete x;
@@ -1029,7 +1076,8 @@ int32_t ntpmt_make_reco_pulses(ntpmt_ctx* x, const nt_hits* h, const uint8_t* d_
     PMT_TRY(hipGetLastError());
     cub = L.cub_bytes;
     PMT_TRY(hipcub::DeviceRadixSort::SortPairs(b + L.cub, cub, okey_a, okey_b, orun_a, orun_b, (int)n, 0, 32, s));
-    ChanArgs a{key_b, idx_b, tq, hflag, uniq, counts, offs, n_runs, orun_b, next, capacity, seed, *out};
+    ChanArgs a{x->d_spill, x->spill_stride, key_b, idx_b, tq, hflag, uniq, counts, offs, n_runs, orun_b, next,
+               capacity, seed, *out};
     DevCfg cfg = x->cfg;
     void* kargs[] = {&a, &cfg};
     PMT_TRY(hipLaunchKernel(x->kernel, dim3(x->blocks), dim3(64), kargs, x->lds_bytes, s));

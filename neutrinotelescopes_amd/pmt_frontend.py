@@ -364,6 +364,9 @@ class PMTConfig:
     lead_time: float = 50.0
     min_charge: float = 0.2
     nnls_tol: float = 1e-8
+    # unfolding factor rows kept in LDS on the device (0: the library's choice);
+    # a tuning knob that does not change results
+    lds_factor_rows: int = 0
 
     @classmethod
     def make(cls, st: ExponTruncNormalSPE, pm: PDFPulseTemplate, snr_db: float, sampling_freq: float,
@@ -412,6 +415,7 @@ class PMTConfig:
         c.filt_pulse = fp.ctypes.data_as(C.POINTER(C.c_double))
         c.n_filt_pulse = len(fp)
         c.filt_pulse_t0, c.filt_pulse_dt = self.pulse_model_filt.t0, self.pulse_model_filt.dt
+        c.lds_factor_rows = int(self.lds_factor_rows)
         return c, (taps, fp)
 
 

@@ -135,6 +135,23 @@ def test_order_independent_and_selection(fe):
     np.testing.assert_allclose(e.t0, a.t0 - 1.0, atol=1e-5)
 
 
+def test_factor_rows_in_hbm(fe):
+    """With only 8 rows of the unfolding's Cholesky factor in LDS, every
+    channel's larger active sets keep their later rows in HBM: the pulses are
+    those of the all-LDS run and of the oracle."""
+    import dataclasses
+    cfg = dataclasses.replace(fe.config, lds_factor_rows=8)
+    small = pf.PMTFrontEnd(cfg, 0)
+    h = synthetic_hits(n_modules=3, seed=7)
+    got = small.make_reco_pulses(h, seed=13, waveforms=True)
+    ref = fe.make_reco_pulses(h, seed=13, waveforms=True)
+    small.close()
+    np.testing.assert_array_equal(got.n_pulses, ref.n_pulses)
+    np.testing.assert_allclose(got.charge, ref.charge, rtol=1e-9, atol=1e-12)
+    np.testing.assert_array_equal(got.time, ref.time)
+    compare(fe.config, got, oracle_reco(fe.config, h, 13))
+
+
 def test_empty_and_no_selection(fe):
     h = synthetic_hits(n_modules=1, seed=1)
     r = fe.make_reco_pulses(h, seed=1, select=np.zeros(len(h.time), bool))
