@@ -37,8 +37,10 @@
  *   charge:                  (0x73000000, …) → component and exponential
  *                            uniforms; (0x73000001 + t, …) → two normals per trial
  *   noise, sample i:         (0x74000000, channel, i / 2, 0) → normal i mod 2
- * with channel = module_id · 17 + min(pmt_id, 16) and 53-bit uniforms.  The
- * reference draws from Julia's default RNG (not reproducible).
+ * with channel = module_id · 17 + min(pmt_id, 16) and 53-bit uniforms (the
+ * noise depends on the seed and the channel only: give separate events
+ * separate seeds).  The reference draws from Julia's default RNG (not
+ * reproducible).
  *
  * Tables (host arrays, copied at ntpmt_create):
  *   decim_taps   the resampling filter phase used at the ratio 1/R (odd
