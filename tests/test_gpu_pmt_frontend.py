@@ -152,6 +152,23 @@ def test_factor_rows_in_hbm(fe):
     compare(fe.config, got, oracle_reco(fe.config, h, 13))
 
 
+def test_other_digitiser_configuration(engine):
+    """A 0.5 GHz digitiser over a 150 ns window (301 waveform samples, R = 4,
+    76 digitiser samples, 1501 unfolding columns in 20 patterns): the pulse
+    spans more digitiser rows than the register-window gradient pass holds, so
+    the general path runs; same bar against the oracle."""
+    import dataclasses
+    base = pf.std_pmt_config()
+    cfg = dataclasses.replace(base, adc_freq=0.5, time_range=150.0, decim_taps=pf.decimation_taps(0.5 / 2.0))
+    fe2 = pf.PMTFrontEnd(cfg, 0)
+    assert (fe2.n_wave, fe2.n_adc, fe2.n_pulse) == (301, 76, 1501)
+    h = synthetic_hits(n_modules=3, seed=11)
+    h.time = (h.time * 0.5).astype(np.float32)           # inside the shorter window
+    got = fe2.make_reco_pulses(h, seed=5, waveforms=True)
+    fe2.close()
+    compare(cfg, got, oracle_reco(cfg, h, 5))
+
+
 def test_empty_and_no_selection(fe):
     h = synthetic_hits(n_modules=1, seed=1)
     r = fe.make_reco_pulses(h, seed=1, select=np.zeros(len(h.time), bool))
