@@ -357,7 +357,10 @@ def surrogate_benchmark():
             "holdout_loss_final", "error_pairs", "count_err_median", "count_err_mean", "count_bias",
             "error_pairs_50hits", "count_err_median_50hits", "count_err_mean_50hits", "count_bias_50hits",
             "count_err_median_untrained", "count_err_median_scale_only", "count_err_median_50hits_scale_only",
-            "truth_rel_stat_err_median", "truth_rel_stat_err_median_50hits", "time_kl_median", "time_kl_median_untrained", "events",
+            "truth_rel_stat_err_median", "truth_rel_stat_err_median_50hits", "truth_rel_stat_err_weighted_median",
+            "truth_rel_stat_err_weighted_median_50hits", "count_err_rms", "count_err_rms_intrinsic",
+            "count_err_rms_50hits", "count_err_rms_intrinsic_50hits", "time_kl_median", "time_kl_median_untrained",
+            "events",
             "llh_seconds", "llh_flow_seconds", "llh_fallback_seconds", "llh_pairs", "llh_hits", "fallback_pairs",
             "surrogate_events_per_s", "direct_llh_events", "direct_llh_photons_per_event", "direct_llh_seconds",
             "direct_events_per_s", "direct_llh_finite", "speedup", "direct_photons_per_s")

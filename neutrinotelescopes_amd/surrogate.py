@@ -592,16 +592,29 @@ def direct_propagation(engine, medium, spectrum, casc: Cascades, modules: np.nda
         "photons": sim, "seconds": time.perf_counter() - t0}
 
 
-def module_errors(lam_sur: np.ndarray, lam_dir: np.ndarray, n_sim_hits: np.ndarray, min_hits: int = 400):
+def module_errors(lam_sur: np.ndarray, lam_dir: np.ndarray, n_sim_hits: np.ndarray, min_hits: int = 400,
+                  dir_var: np.ndarray | None = None):
     """Per-module expected-count error of the surrogate against direct
     propagation (Σ over the module's PMTs), over cascade-module pairs whose
     direct estimate rests on ≥ min_hits simulated hits (≤ 5 % statistical
-    error)."""
+    error).  With `dir_var` (the direct estimate's variance per pair, Σw² of
+    its weighted hits) also the direct estimate's median relative statistical
+    error and the surrogate's own RMS error with that variance taken out: for
+    δ = (s − d)/s and a truth d = μ(1 + η) independent of the surrogate
+    s = μ(1 + ε), E[δ²] − E[Var d / s²] = E[ε²/(1 + ε)²] ≈ ε_rms² (dividing by
+    s rather than by the noisy d keeps the subtraction unbiased)."""
     s = lam_sur.sum(-1)
     d = lam_dir.sum(-1)
     sel = n_sim_hits >= min_hits
     rel = (s[sel] - d[sel]) / d[sel]
-    return {"pairs": int(sel.sum()), "median_abs_rel_err": float(np.median(np.abs(rel))) if rel.size else None,
+    extra = {}
+    if dir_var is not None and rel.size:
+        v, ss, dd = dir_var[sel], s[sel], d[sel]
+        delta = (ss - dd) / ss
+        extra = {"truth_rel_stat_err_weighted_median": float(np.median(np.sqrt(v) / dd)),
+                 "rms_rel_err": float(np.sqrt(np.mean(rel ** 2))),
+                 "rms_rel_err_intrinsic": float(np.sqrt(max(0.0, np.mean(delta ** 2) - np.mean(v / ss ** 2))))}
+    return {**extra, "pairs": int(sel.sum()), "median_abs_rel_err": float(np.median(np.abs(rel))) if rel.size else None,
             "mean_abs_rel_err": float(np.mean(np.abs(rel))) if rel.size else None,
             "bias": float(np.sum(s[sel]) / np.sum(d[sel]) - 1) if rel.size else None,
             "rel_err": rel}
@@ -884,11 +897,14 @@ def surrogate_study(n_runs: int = 128, photons_per_run: int = 2 * 10 ** 8, train
     np.add.at(nh, (hits_sel[:, 0].astype(int), hits_sel[:, 1].astype(int)), 1)
     nh_truth = np.zeros((ne, len(modules)))
     np.add.at(nh_truth, (hits_dir[:, 0].astype(int), hits_dir[:, 1].astype(int)), 1)
+    # the truth's variance per pair: Σw² of its weighted hits
+    truth_var = np.zeros((ne, len(modules)))
+    np.add.at(truth_var, (hits_dir[:, 0].astype(int), hits_dir[:, 1].astype(int)), hits_dir[:, 4] ** 2)
     dist = np.linalg.norm(modules[None] - ev.pos[:, None], axis=-1)
     nh_far = np.where(dist >= FALLBACK_DISTANCE, nh, 0)
     lam_sur = surrogate_expectations(sur, ev, modules)
-    err = module_errors(lam_sur, lam_dir, nh_far)
-    err50 = module_errors(lam_sur, lam_dir, nh_far, min_hits=50)
+    err = module_errors(lam_sur, lam_dir, nh_far, dir_var=truth_var)
+    err50 = module_errors(lam_sur, lam_dir, nh_far, min_hits=50, dir_var=truth_var)
     base = scale_only_surrogate(sur)
     lam_base = surrogate_expectations(base, ev, modules)
     base.model.close()
@@ -964,6 +980,11 @@ def surrogate_study(n_runs: int = 128, photons_per_run: int = 2 * 10 ** 8, train
         "count_err_median_scale_only": err_base["median_abs_rel_err"],
         "count_err_median_50hits_scale_only": err50_base["median_abs_rel_err"],
         "truth_rel_stat_err_median": truth_stat, "truth_rel_stat_err_median_50hits": truth_stat50,
+        "truth_rel_stat_err_weighted_median": err.get("truth_rel_stat_err_weighted_median"),
+        "truth_rel_stat_err_weighted_median_50hits": err50.get("truth_rel_stat_err_weighted_median"),
+        "count_err_rms": err.get("rms_rel_err"), "count_err_rms_intrinsic": err.get("rms_rel_err_intrinsic"),
+        "count_err_rms_50hits": err50.get("rms_rel_err"),
+        "count_err_rms_intrinsic_50hits": err50.get("rms_rel_err_intrinsic"),
         "time_kl_median": float(np.median(kl)) if kl.size else None,
         "time_kl_median_untrained": float(np.median(kl0)) if kl0.size else None,
         "events": n_events, "llh_seconds": linfo["seconds"], "llh_flow_seconds": linfo["flow_seconds"],

@@ -175,3 +175,19 @@ def test_resample_simulation_counts_and_subsets():
                                                np.random.default_rng(s))[0] for s in range(2000)])
     assert abs((hits == 0).mean() * len(hits) / 2000 - 0.394) < 0.04
     assert abs((hits == 1).mean() * len(hits) / 2000 - 0.135) < 0.03
+
+
+def test_module_errors_separates_truth_noise():
+    """The noise-subtracted RMS error recovers a known surrogate error when
+    the truth carries a known, larger statistical error."""
+    rng = np.random.default_rng(3)
+    n = 20000
+    true = rng.uniform(50, 500, (n, 1, 16)) / 16
+    var = rng.uniform(0.004, 0.02, n)                        # truth relative variance (6-14 % error)
+    truth = true * (1 + rng.normal(0, np.sqrt(var))[:, None, None])
+    sur = true * (1 + rng.normal(0, 0.04, n)[:, None, None])  # 4 % surrogate error
+    mu = true.sum(-1)
+    e = S.module_errors(sur, truth, np.full((n, 1), 1000), dir_var=var[:, None] * mu ** 2)
+    assert abs(e["rms_rel_err_intrinsic"] - 0.04) < 0.004
+    assert e["rms_rel_err"] > 0.09
+    assert abs(e["truth_rel_stat_err_weighted_median"] - np.median(np.sqrt(var))) < 0.005
