@@ -9,21 +9,27 @@
 //      contiguous and in time order;
 //   3. channel_kernel: one 64-lane workgroup per channel, taken from an atomic
 //      work counter over the channels sorted by decreasing hit count (bright
-//      channels cost more: they go first, so none is left alone at the end).  In LDS: the waveform
-//      (n_wave FP64 samples, each a sum over the hits within the pulse's
-//      support, found by binary search in the sorted keys, plus noise), the
-//      first-order low-pass as a parallel scan of affine maps over the lanes,
-//      the decimation by R (one FIR phase, FP64), then the unfolding.
+//      channels cost more: they go first, so none is left alone at the end).
+//      In LDS: the waveform (n_wave FP64 samples, each a sum over the hits
+//      within the pulse's support, found by binary search in the sorted keys,
+//      plus noise), the first-order low-pass as a parallel scan of affine maps
+//      over the lanes, the decimation by R (one FIR phase, FP64), then the
+//      unfolding.
 // The unfolding is the fnnls active-set method (Bro & de Jong 1997) in FP64
 // on the implicit matrix A[k, i] = filtered pulse at t_k − τ_i: with
 // τ_i = t0 + i·res and Q = (1/adc_freq)/res an integer, column i = Q·q + p is
-// pattern p shifted down by q rows, so A is J·Q numbers (a 16 × 40 table for
-// the standard configuration) kept in LDS.  The active set's Gram matrix
-// A_Pᵀ A_P is kept as a Cholesky factor (packed, LDS; the waveform's space)
+// pattern p shifted down by q rows, so A is J·Q numbers (14 × 40 for the
+// standard configuration, rows zero for every pattern dropped) kept in LDS.
+// The active set's Gram matrix A_Pᵀ A_P is kept as a packed Cholesky factor
 // that grows by one row when a column enters and loses one by Givens
-// rotations when a column leaves; solves are lane-parallel column sweeps
-// with the reciprocal diagonal kept beside the factor.  Per outer iteration the gradient Aᵀ(b − A x) over all n_pulse
-// columns is one pass of J fused multiply-adds per column.
+// rotations when a column leaves; its first K rows (53 by default) sit in LDS
+// in the waveform's space, the rest in a per-workgroup HBM region, so that
+// eight channels share a CU.  Solves are lane-parallel column sweeps (a
+// readlane broadcast, a multiply, one fma per lane, the next row loaded a
+// step ahead) with the reciprocal diagonal in registers.  Per outer iteration
+// the gradient Aᵀ(b − A x) over all n_pulse columns runs with one lane per
+// pattern: its taps in registers and a window of residual rows rotating
+// through registers.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
