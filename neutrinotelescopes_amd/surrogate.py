@@ -680,6 +680,20 @@ def observe(lam: np.ndarray, direct_hits: np.ndarray, rng) -> Observation:
                        np.column_stack([ev, m, p, direct_hits[order[pos], 3]]))
 
 
+def _bin_index(bins: np.ndarray, x: np.ndarray) -> np.ndarray:
+    """np.searchsorted(bins, x, side="right") − 1 clipped to [0, len(bins) − 2];
+    for evenly spaced bins by arithmetic (one floor per value, then one step
+    of correction at the edges, so the result equals the search exactly)."""
+    nb = bins.size - 1
+    d = np.diff(bins)
+    if nb > 0 and np.all(d == d[0]):
+        i = np.clip(np.floor((x - bins[0]) / d[0]), -1, nb).astype(np.int64)
+        i = np.where((i + 1 <= nb) & (bins[np.clip(i + 1, 0, nb)] <= x), i + 1, i)
+        i = np.where((i >= 0) & (bins[np.clip(i, 0, nb)] > x), i - 1, i)
+        return np.clip(i, 0, nb - 1)
+    return np.clip(np.searchsorted(bins, x, side="right") - 1, 0, nb - 1)
+
+
 def fallback_terms(engine, medium, spectrum, casc: Cascades, modules: np.ndarray, counts: np.ndarray,
                    near: np.ndarray, near_hits: np.ndarray, photons: int = 10 ** 7, seed: int = 0,
                    bins=np.arange(-20.0, 2001.0, 2.0)) -> np.ndarray:
@@ -726,11 +740,13 @@ def fallback_terms(engine, medium, spectrum, casc: Cascades, modules: np.ndarray
     dist = np.linalg.norm(casc.pos[ev] - modules[m], axis=1)
     tres = h.time[keep].astype(np.float64) - (dist - MODULE_RADIUS) / (C_VAC_M_NS / n800)
     n_mod = len(modules)
-    key = (ev * n_mod + m) * 16 + k
     pairs = np.argwhere(near)
     all_keys = ((pairs[:, 0] * n_mod + pairs[:, 1])[:, None] * 16 + np.arange(16)).reshape(-1)
+    # (event, module) → index of the pair among the near pairs, then × 16 + PMT
+    pair_of = np.full(near.shape, -1, np.int64)
+    pair_of[pairs[:, 0], pairs[:, 1]] = np.arange(len(pairs))
     lam = np.zeros(all_keys.size)
-    pos = np.searchsorted(all_keys, key)
+    pos = pair_of[ev, m] * 16 + k
     np.add.at(lam, pos, w)
     n_obs = counts.reshape(-1)[all_keys]
     with np.errstate(divide="ignore"):
@@ -741,12 +757,13 @@ def fallback_terms(engine, medium, spectrum, casc: Cascades, modules: np.ndarray
     # times only (sparse: one entry per occupied (PMT, bin))
     if len(near_hits):
         nb = bins.size - 1
-        hb = np.clip(np.searchsorted(bins, tres, side="right") - 1, 0, nb - 1)
+        hb = _bin_index(bins, tres)
         cell = pos * nb + hb
         cells, inv = np.unique(cell, return_inverse=True)
         cell_w = np.bincount(inv, weights=w)
-        ob = np.clip(np.searchsorted(bins, near_hits[:, 2], side="right") - 1, 0, nb - 1)
-        op = np.searchsorted(all_keys, near_hits[:, 0].astype(np.int64))
+        ob = _bin_index(bins, near_hits[:, 2])
+        hk = near_hits[:, 0].astype(np.int64)
+        op = pair_of[hk // (16 * n_mod), (hk // 16) % n_mod] * 16 + hk % 16
         oc = op * nb + ob
         if cells.size:
             j = np.clip(np.searchsorted(cells, oc), 0, cells.size - 1)

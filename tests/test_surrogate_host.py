@@ -191,3 +191,17 @@ def test_module_errors_separates_truth_noise():
     assert abs(e["rms_rel_err_intrinsic"] - 0.04) < 0.004
     assert e["rms_rel_err"] > 0.09
     assert abs(e["truth_rel_stat_err_weighted_median"] - np.median(np.sqrt(var))) < 0.005
+
+
+def test_bin_index_equals_search():
+    """The arithmetic binning of the fallback's time histograms equals
+    searchsorted exactly, on and next to every edge."""
+    rng = np.random.default_rng(0)
+    bins = np.arange(-20.0, 2001.0, 2.0)
+    x = np.concatenate([rng.uniform(-50, 2100, 200000), bins, np.nextafter(bins, -np.inf),
+                        np.nextafter(bins, np.inf)])
+    ref = np.clip(np.searchsorted(bins, x, side="right") - 1, 0, bins.size - 2)
+    assert np.array_equal(S._bin_index(bins, x), ref)
+    uneven = np.array([0.0, 1.0, 3.0, 7.0])
+    y = rng.uniform(-1, 8, 1000)
+    assert np.array_equal(S._bin_index(uneven, y), np.clip(np.searchsorted(uneven, y, side="right") - 1, 0, 2))
