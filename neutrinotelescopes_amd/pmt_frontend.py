@@ -532,7 +532,10 @@ class PMTFrontEnd:
         """make_reco_pulses (pipeline.jl:146-160) for the host hits `hits`
         (a PhotonHits with module_id, pmt_id, source_index, photon_id and the
         time column), restricted to `select` (bool per hit, e.g. the detected
-        flags of PropagationEngine.resample; None: all hits)."""
+        flags of PropagationEngine.resample; None: all hits).  Every
+        (module_id, pmt_id) present is a channel, pmt_id 0 (hits between the
+        PMTs) included: leave those out with `select`, as the reference's
+        per-PMT loops over PMTs 1-16 do."""
         import torch
         dev = torch.device(f"cuda:{self.device}")
         n = len(hits.time)
