@@ -163,6 +163,24 @@ __device__ __forceinline__ bool sphere_hit(float3 pos, float3 dir, float step, f
     return false;
 }
 
+// Kernel arguments used only off the step loop (photon initialisation, hit
+// flush), and the target-list bounds of first_hit, are re-read from the
+// kernarg segment where they are used, through a pointer the compiler cannot
+// match with the loads at kernel entry: their SGPRs are then live only there,
+// not across the whole persistent loop (held across it they spill to VGPR
+// lanes and cost a readlane per use).  Only kernels whose first argument is
+// KParams (propagate_kernel, serve_kernel) may call code that uses kargs().
+typedef const __attribute__((address_space(4))) KParams* const_kparams_ptr;
+__device__ __forceinline__ const_kparams_ptr kargs() {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const_kparams_ptr k = (const_kparams_ptr)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(k));
+    return k;
+#else
+    return nullptr;   // host pass: never called
+#endif
+}
+
 // First listed target hit by the step [pos, pos + step·dir]: returns its list
 // index or -1.  Brute force over the list (n_clusters == 0) or through the
 // cluster bounds; `tests` counts the sphere tests behind the bounds (the
@@ -176,9 +194,10 @@ __device__ __forceinline__ bool sphere_hit(float3 pos, float3 dir, float step, f
 __device__ __forceinline__ int first_hit(const KParams& p, float3 pos, float3 dir, float step, float& dt,
                                          uint32_t& tests) {
     int best = 0x7fffffff;
-    if (p.n_clusters == 0) {
+    const int n_clusters = kargs()->n_clusters, n_targets = kargs()->n_targets;
+    if (n_clusters == 0) {
         const const_f4_ptr tgt = (const_f4_ptr)(const void*)p.targets;
-        for (int t = 0; t < p.n_targets; ++t) {
+        for (int t = 0; t < n_targets; ++t) {
             const f4 tv = tgt[t];
             float d;
             if (sphere_hit(pos, dir, step, make_float4(tv.x, tv.y, tv.z, tv.w), d) && best == 0x7fffffff) {
@@ -197,12 +216,12 @@ __device__ __forceinline__ int first_hit(const KParams& p, float3 pos, float3 di
     const float inv_dd = dd > 0.0f ? __builtin_amdgcn_rcpf(dd) : 0.0f;
     const float re = __builtin_fmaf(step, 0x1p-10f, p.cluster_reach);
     const float re2 = re * re;
-    for (int c = 0; c < p.n_clusters; ++c) {
+    for (int c = 0; c < n_clusters; ++c) {
         const f4 c0 = cl[2 * c];
         const f4 c1 = cl[2 * c + 1];
         const int first = __float_as_int(c1.y);
         const int count = __float_as_int(c1.z);
-        if (first < 0 || count < 0 || first + count > p.n_targets) continue;  // corrupt table guard
+        if (first < 0 || count < 0 || first + count > n_targets) continue;  // corrupt table guard
         // distance of the step's xy projection from the string axis first:
         // for strings long compared with a step it is the selective test
         const float wx = c0.x - pos.x, wy = c0.y - pos.y;
@@ -248,21 +267,6 @@ __device__ __forceinline__ int first_hit(const KParams& p, float3 pos, float3 di
         }
     }
     return best == 0x7fffffff ? -1 : best;
-}
-
-// Kernel arguments used only off the step loop (photon initialisation, hit
-// flush) are re-read from the kernarg segment where they are used, through a
-// pointer the compiler cannot match with the loads at kernel entry: their
-// SGPRs are then live only there, not across the whole persistent loop.
-typedef const __attribute__((address_space(4))) KParams* const_kparams_ptr;
-__device__ __forceinline__ const_kparams_ptr kargs() {
-#if defined(__HIP_DEVICE_COMPILE__)
-    const_kparams_ptr k = (const_kparams_ptr)__builtin_amdgcn_kernarg_segment_ptr();
-    asm volatile("" : "+s"(k));
-    return k;
-#else
-    return nullptr;   // host pass: never called
-#endif
 }
 
 __device__ __forceinline__ Medium load_medium(const_kparams_ptr k) {
