@@ -164,21 +164,27 @@ def cpu_baseline(workload: str, target_s: float = 12.0):
         if dt > 1.0:
             break
         n = int(n * min(50.0, 1.5 / max(dt, 1e-3)))
-    n_final = int(n * max(1.0, target_s / max(dt, 1e-6)))
+    # the sizing runs above are the warm-up; then the median of five timed
+    # runs of a fifth of the target time each
+    n_final = int(n * max(1.0, target_s / 5.0 / max(dt, 1e-6)))
     s = build_workload(workload, n_final)
-    t0 = time.perf_counter()
-    _oracle.propagate(s, seed=2, threads=threads, capacity=1 << 22, lib=lib)
-    dt = time.perf_counter() - t0
+    times = []
+    for rep in range(5):
+        t0 = time.perf_counter()
+        _oracle.propagate(s, seed=2 + rep, threads=threads, capacity=1 << 22, lib=lib)
+        times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
     rate = n_final / dt
     # the same workload on one core, a ~4 s sample
     n1 = max(10000, int(rate / threads * 4.0))
     s1 = build_workload(workload, n1)
     t0 = time.perf_counter()
-    _oracle.propagate(s1, seed=3, threads=1, capacity=1 << 22, lib=lib)
+    _oracle.propagate(s1, seed=7, threads=1, capacity=1 << 22, lib=lib)
     dt1 = time.perf_counter() - t0
     return {"value": rate, "unit": "photons/s", "cores": threads, "kind": "port",
-            "sample": f"{n_final} photons of the {workload} workload (FP64 direction arithmetic), {dt:.1f} s on "
-                      f"{threads} threads (CPU oracle, {build}, OpenMP)",
+            "sample": f"{n_final} photons of the {workload} workload (FP64 direction arithmetic) on {threads} threads, "
+                      f"median of 5 runs ({min(times):.2f}-{max(times):.2f} s) after warm-up (CPU oracle, {build}, "
+                      f"OpenMP)",
             "one_core_value": n1 / dt1, "one_core_sample": f"{n1} photons, {dt1:.1f} s on 1 thread",
             "host": info}
 
