@@ -121,6 +121,39 @@ class PhotonHits:
         return pd.DataFrame(d)
 
 
+class DeviceHitColumns:
+    """Hit columns in device memory (torch tensors) for `PropagationEngine.launch`
+    (nt_launch): the PhotonHit columns, source and photon index, and with
+    `epilogue` the PMT id, weight and residual; `counters` (device int64[4]:
+    hits found, photons simulated, sphere tests, string-bound tests) is
+    cleared by each launch.  Hits beyond `capacity` are counted, not stored."""
+
+    def __init__(self, capacity: int, device: int = 0, epilogue: bool = True):
+        import torch
+        dev = torch.device(f"cuda:{device}")
+        n = int(capacity)
+        f = lambda *shape: torch.empty(*shape, dtype=torch.float32, device=dev)
+        self.capacity = n
+        self.cols = dict(position=f(n, 3), direction=f(n, 3), initial_direction=f(n, 3), wavelength=f(n),
+                         time=f(n), dist_travelled=f(n),
+                         module_id=torch.empty(n, dtype=torch.int16, device=dev),        # uint16 bits
+                         source_index=torch.empty(n, dtype=torch.int32, device=dev),     # uint32 bits
+                         photon_id=torch.empty(n, dtype=torch.int64, device=dev))        # uint64 bits
+        if epilogue:
+            self.cols.update(pmt_id=torch.empty(n, dtype=torch.uint8, device=dev), total_weight=f(n), tres=f(n))
+        self.counters = torch.zeros(4, dtype=torch.int64, device=dev)
+        self.abi = _abi.nt_hits()
+        for k, v in self.cols.items():
+            setattr(self.abi, k, v.data_ptr())
+        self.abi.capacity = n
+
+    def __getattr__(self, name):
+        cols = self.__dict__.get("cols", {})
+        if name in cols:
+            return cols[name]
+        raise AttributeError(name)
+
+
 class PropagationEngine:
     """A libntprop context: device-resident medium, spectrum, targets and sources."""
 
@@ -231,6 +264,21 @@ class PropagationEngine:
         out.n_simulated = ns.value
         out.n_dropped = nd.value
         return out
+
+    def launch(self, seed: int, out: DeviceHitColumns, stream=None):
+        """Asynchronous propagation into device columns (nt_launch) on `stream`
+        (a torch stream; default: the current stream of the engine's
+        device).  The hit count is out.counters[0] once the stream reaches
+        this point; with the epilogue enabled (set_epilogue) the launch
+        fills the PMT id, weight and residual columns too."""
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream(torch.device(f"cuda:{self.device}"))
+        if self._epilogue and out.cols.get("pmt_id") is None:
+            raise ValueError("the epilogue is enabled: the columns need pmt_id, total_weight and tres")
+        _abi.check(self.lib.nt_launch(self.handle, C.c_uint64(seed & (2**64 - 1)), C.byref(out.abi),
+                                      C.c_void_p(out.counters.data_ptr()), C.c_void_p(stream.cuda_stream)),
+                   "nt_launch")
 
     def resample(self, hits: PhotonHits, seed: int, downsample: float = 1.0, per_pmt: bool = True) -> np.ndarray:
         """resample_simulation (pipeline.jl:86-116) of host hits on the device
