@@ -97,9 +97,10 @@ typedef struct ntpmt_config {
     int32_t n_filt_pulse;
     double filt_pulse_t0, filt_pulse_dt;
     int32_t lds_factor_rows;         /* rows of the unfolding's Cholesky factor kept in LDS, the rest
-                                        in HBM (0: the library's choice, 53); fewer rows fit more
-                                        channels on a CU but send more rows to HBM; results do not
-                                        depend on it */
+                                        in HBM (0: the library's choice, the most that let twelve
+                                        channels share a CU: 35 for STD_PMT_CONFIG); fewer rows fit
+                                        more channels on a CU but send more rows to HBM; results do
+                                        not depend on it */
 } ntpmt_config;
 
 typedef struct ntpmt_ctx ntpmt_ctx;

@@ -57,11 +57,12 @@ constexpr int kTtTrials = 64;
 constexpr int kSpeTrials = 16;
 constexpr uint32_t kNoChannel = (1u << 21) - 1;      // key of hits that are not selected
 constexpr int kLanes = 64;
-constexpr int kMaxPos = 2 * kLanes;                  // active-set positions per workgroup (≥ NTPMT_MAX_ADC)
+static_assert(NTPMT_MAX_ADC <= 2 * kLanes, "active-set positions live in two registers per lane");
 constexpr double kDegenerate = 1e-14;                // a column with d² ≤ this · ‖a‖² is numerically dependent
 constexpr int kTaps = 16;                            // rows a column reaches in the register-window gradient pass
 constexpr int kPadLo = 16, kPadHi = 48;              // zero rows around the residual (padded LDS array)
-constexpr int kLdsRows = 53;                         // factor rows kept in LDS by default; later rows in HBM
+constexpr size_t kLdsPerCU = 160 * 1024;             // LDS of a CU
+constexpr int kChannelsPerCU = 12;                   // one-wave workgroups a CU runs at once (3 per SIMD)
 
 struct DevCfg {
     double tt_d, tt_c, tt_scale, tt_subtract;
@@ -249,8 +250,8 @@ struct Lds {
     int K, triK;        // rows in LDS, tri(K)
     double* b;          // [n_adc] digitised waveform
     double* r;          // [n_adc] residual b − A x, inside kPadLo / kPadHi zeros
-    int2* pos_qp;       // [kMaxPos] (q, p) of the active positions (residual)
-    double* pos_x;      // [kMaxPos] x of the active positions (residual)
+    int2* pos_qp;       // [n_adc] (q, p) of the active positions (residual)
+    double* pos_x;      // [n_adc] x of the active positions (residual)
     uint32_t* inP;      // [n_pulse / 32] active set
     uint32_t* excl;     // [n_pulse / 32] columns skipped as dependent
 };
@@ -518,34 +519,69 @@ __device__ __forceinline__ void w_argmax(const Lds& s, const DevCfg& c, int lane
     }
 }
 
+struct ChanKArgs {
+    DevCfg c;
+    ChanArgs g;
+};
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef const __attribute__((address_space(4))) ChanKArgs* const_kargs_ptr;
+#else
+typedef const ChanKArgs* const_kargs_ptr;       // host pass: never called
+#endif
+// the kernel argument through a pointer the compiler cannot match with the
+// loads at kernel entry: what is read through it per channel is live only
+// while that channel is processed
+__device__ __forceinline__ const_kargs_ptr chan_kargs() {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const_kargs_ptr k = (const_kargs_ptr)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(k));
+    return k;
+#else
+    return nullptr;
+#endif
+}
+
+// Three waves per SIMD (168 VGPRs, no spills; twelve channels per CU with the
+// default LDS split): the solve is a chain of dependent steps, and what hides
+// their latency is the number of channels a CU runs at once.
 template <int kJ>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void channel_kernel(ChanArgs g, DevCfg c) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void channel_kernel(ChanKArgs ka) {
     extern __shared__ double smem[];
-    const int lane = threadIdx.x;
-    const int nw32 = (c.n_pulse + 31) / 32;
-    Lds s;
-    s.T = smem;
-    s.buf = s.T + c.J * c.Q;
-    s.K = c.lds_rows;
-    s.triK = tri(s.K);
-    s.glb = g.spill + (int64_t)blockIdx.x * g.spill_stride;
-    const int nbuf = max(c.n_wave, s.triK);
-    s.b = s.buf + nbuf;
-    s.r = s.b + c.n_adc + kPadLo;
-    s.pos_x = s.r + c.n_adc + kPadHi;
-    s.pos_qp = (int2*)(s.pos_x + kMaxPos);
-    s.inP = (uint32_t*)(s.pos_qp + kMaxPos);
-    s.excl = s.inP + nw32;
-    for (int i = lane; i < c.J * c.Q; i += kLanes) s.T[i] = c.table[i];
+    const int lane0 = threadIdx.x;
+    {
+        const const_kargs_ptr kp = chan_kargs();
+        for (int i = lane0; i < kp->c.J * kp->c.Q; i += kLanes) smem[i] = kp->c.table[i];
+    }
     __syncthreads();
-    const uint32_t k0 = (uint32_t)g.seed, k1 = (uint32_t)(g.seed >> 32);
-    const int n_runs = *g.n_runs;
 
     for (;;) {
+        const const_kargs_ptr kp = chan_kargs();
+        auto& g = kp->g;
         int wi = 0;
-        if (lane == 0) wi = atomicAdd(g.next, 1);
+        if (lane0 == 0) wi = atomicAdd(g.next, 1);
         wi = __shfl(wi, 0);
-        if (wi >= n_runs) break;
+        if (wi >= *g.n_runs) break;
+        // per channel: the configuration, the lane index and the seed words
+        // afresh, so that nothing derived from them stays live across channels
+        const DevCfg c = kp->c;
+        int lane = lane0;
+        asm volatile("" : "+v"(lane));
+        uint32_t k0 = (uint32_t)g.seed, k1 = (uint32_t)(g.seed >> 32);
+        asm volatile("" : "+s"(k0), "+s"(k1));
+        const int nw32 = (c.n_pulse + 31) / 32;
+        Lds s;
+        s.T = smem;
+        s.buf = s.T + c.J * c.Q;
+        s.K = c.lds_rows;
+        s.triK = tri(s.K);
+        s.glb = g.spill + (int64_t)blockIdx.x * g.spill_stride;
+        const int nbuf = max(c.n_wave, s.triK);
+        s.b = s.buf + nbuf;
+        s.r = s.b + c.n_adc + kPadLo;
+        s.pos_x = s.r + c.n_adc + kPadHi;
+        s.pos_qp = (int2*)(s.pos_x + c.n_adc);
+        s.inP = (uint32_t*)(s.pos_qp + c.n_adc);
+        s.excl = s.inP + nw32;
         const int ci = g.order[wi];
         const uint32_t ch = g.unique[ci];
         if (ch == kNoChannel || ci >= g.capacity) continue;
@@ -657,6 +693,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         int64_t iter = 0;
         int outer = 0;
         for (;;) {
+            // lane-derived addresses are recomputed per iteration, not held
+            // in registers across the solve
+            asm volatile("" : "+v"(lane));
             double best;
             int bi;
             w_argmax<kJ>(s, c, lane, best, bi);
@@ -974,11 +1013,22 @@ int32_t ntpmt_create(int32_t device, const ntpmt_config* c, ntpmt_ctx** out) {
     d.table = x->d_table;
     const size_t nw32 = (size_t)(n_pulse + 31) / 32;
     if (c->lds_factor_rows < 0) return cleanup(arg_error("lds_factor_rows >= 0"));
-    const size_t K = (size_t)std::min<int64_t>(n_adc, c->lds_factor_rows > 0 ? c->lds_factor_rows : kLdsRows);
+    // LDS per channel with the factor's first K rows in it; by default the
+    // largest K that leaves room for kChannelsPerCU channels on a CU
+    auto lds_for = [&](size_t K) {
+        return sizeof(double) * ((size_t)J_trim * Q + std::max<size_t>(n_wave, K * (K + 1) / 2) + 2 * (size_t)n_adc +
+                                 kPadLo + kPadHi + 2 * (size_t)n_adc) +
+               sizeof(uint32_t) * 2 * nw32;
+    };
+    size_t K = (size_t)n_adc;
+    if (c->lds_factor_rows > 0) {
+        K = std::min<size_t>(K, (size_t)c->lds_factor_rows);
+    } else {
+        const size_t budget = (kLdsPerCU / kChannelsPerCU) & ~(size_t)511;
+        while (K > 1 && lds_for(K) > budget) --K;
+    }
     d.lds_rows = (int)K;
-    x->lds_bytes = sizeof(double) * ((size_t)J_trim * Q + std::max<size_t>(n_wave, K * (K + 1) / 2) +
-                                     2 * (size_t)n_adc + kPadLo + kPadHi + 2 * kMaxPos) +
-                   sizeof(uint32_t) * 2 * nw32;
+    x->lds_bytes = lds_for(K);
     if (x->lds_bytes > 64 * 1024) return cleanup(arg_error("configuration needs more than 64 KB of LDS per channel"));
     x->kernel = J_trim <= 14 ? (const void*)channel_kernel<14> : (const void*)channel_kernel<16>;
     int per_cu = 0;
@@ -1085,7 +1135,8 @@ int32_t ntpmt_make_reco_pulses(ntpmt_ctx* x, const nt_hits* h, const uint8_t* d_
     ChanArgs a{x->d_spill, x->spill_stride, key_b, idx_b, tq, hflag, uniq, counts, offs, n_runs, orun_b, next,
                capacity, seed, *out};
     DevCfg cfg = x->cfg;
-    void* kargs[] = {&a, &cfg};
+    ChanKArgs ka{cfg, a};
+    void* kargs[] = {&ka};
     PMT_TRY(hipLaunchKernel(x->kernel, dim3(x->blocks), dim3(64), kargs, x->lds_bytes, s));
     PMT_TRY(hipGetLastError());
     PMT_TRY(hipEventRecord(x->last, s));
