@@ -86,6 +86,16 @@ int32_t ntf_train_step(ntf_ctx* ctx, const float* d_x, const float* d_t, const f
                        const uint8_t* d_flags, const float* d_aux, int64_t n, float lr, double* d_loss,
                        void* stream);
 
+/* The same step on a batch drawn from larger arrays: the batch is rows
+   d_idx[0 … batch−1] (device int64, each in [0, n_rows); an index outside
+   makes its row count as absent) of the n_rows-row arrays, read by the kernel
+   in place of a gathered copy (the reference's DataLoader batches,
+   extended_cascade_model.jl:179-195, 237-295).  The result equals
+   ntf_train_step on the gathered rows. */
+int32_t ntf_train_step_gather(ntf_ctx* ctx, const float* d_x, const float* d_t, const float* d_count,
+                              const uint8_t* d_flags, const float* d_aux, int64_t n_rows, const int64_t* d_idx,
+                              int64_t batch, float lr, double* d_loss, void* stream);
+
 /* ---- event likelihood -------------------------------------------------
    Rows of every (source, module, PMT) pair of an event set, built on the
    device (the reference's calc_flow_inputs, extended_cascade_model.jl:

@@ -268,6 +268,8 @@ SIGNATURES = {
                                   C.c_int64, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p]),
     "ntf_train_step": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.c_int64, C.c_float, C.c_void_p, C.c_void_p]),
+    "ntf_train_step_gather": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_int64, C.c_void_p, C.c_int64, C.c_float, C.c_void_p, C.c_void_p]),
     "ntf_pair_rows": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p,
                                   C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_float, C.c_void_p,
                                   C.c_void_p, C.c_void_p]),

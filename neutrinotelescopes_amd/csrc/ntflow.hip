@@ -490,6 +490,15 @@ struct Rows {
     const uint8_t* flags;
     const float* aux;          // [n][2] (log λ offset, row weight) or NULL
     int64_t n;
+    const int64_t* idx = nullptr;   // row i is row idx[i] of the arrays (n_src rows); NULL: row i
+    int64_t n_src = 0;
+
+    // the arrays' row of row i, or -1 for an index outside them
+    __device__ __forceinline__ int64_t src(int64_t i) const {
+        if (!idx) return i;
+        const int64_t j = idx[i];
+        return (j >= 0 && j < n_src) ? j : -1;
+    }
 };
 
 // global → LDS copy of kHalfs halfwords: every load of a thread in flight at once
@@ -519,7 +528,8 @@ __device__ __forceinline__ void load_bias(float* b, const float* __restrict__ w,
 // Input fragment of row `row` (B operand, K = 16 features): X[row][4g .. 4g+3]
 __device__ __forceinline__ s16x4 x_frag(const Rows& rows, int64_t row, int g, uint16_t* tx, int r) {
     f32x4 v = {0.0f, 0.0f, 0.0f, 0.0f};
-    if (row < rows.n) v = *reinterpret_cast<const f32x4*>(rows.x + row * kIn + 4 * g);
+    const int64_t sr = row < rows.n ? rows.src(row) : -1;
+    if (sr >= 0) v = *reinterpret_cast<const f32x4*>(rows.x + sr * kIn + 4 * g);
     s16x4 f;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -638,13 +648,14 @@ __global__ __launch_bounds__(kThreads, 1) void flow_grad_kernel(const uint16_t* 
         const int64_t gr = row0 + r;
         float t = 0.0f, cnt = 0.0f, off = 0.0f, wgt = 1.0f;
         int flags = 0;
-        if (gr < rows.n) {
-            t = rows.t[gr];
-            cnt = rows.cnt[gr];
-            flags = rows.flags[gr];
+        const int64_t sr = gr < rows.n ? rows.src(gr) : -1;
+        if (sr >= 0) {
+            t = rows.t[sr];
+            cnt = rows.cnt[sr];
+            flags = rows.flags[sr];
             if (rows.aux) {
-                off = rows.aux[2 * gr];
-                wgt = rows.aux[2 * gr + 1];
+                off = rows.aux[2 * sr];
+                wgt = rows.aux[2 * sr + 1];
             }
         }
         const float loss = quad_row_loss<true>(P + r * kPLd, s, t, cnt, flags, off, wgt, scale, tdp, r);
@@ -1109,13 +1120,27 @@ bool rows_ok(const float* x, const float* t, const float* cnt, const uint8_t* fl
     return n == 0 || (x && t && cnt && flags);
 }
 
-int32_t run_grad(ntf_ctx* c, const float* x, const float* t, const float* cnt, const uint8_t* flags,
-                 const float* aux, int64_t n, float scale, hipStream_t s) {
-    const int64_t blocks = (n + kRows - 1) / kRows;
+int32_t run_grad(ntf_ctx* c, const Rows& rows, float scale, hipStream_t s) {
+    const int64_t blocks = (rows.n + kRows - 1) / kRows;
     const int32_t st = ensure_partials(c, blocks);
     if (st != NT_OK) return st;
-    hipLaunchKernelGGL(flow_grad_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, c->d_img, c->d_w,
-                       Rows{x, t, cnt, flags, aux, n}, scale, c->d_partial, c->d_ploss);
+    hipLaunchKernelGGL(flow_grad_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, c->d_img, c->d_w, rows, scale,
+                       c->d_partial, c->d_ploss);
+    return hip_status(hipGetLastError());
+}
+
+// the gradient of the mean loss of n rows (run_grad's partial records) into
+// one Adam step
+int32_t adam_update(ntf_ctx* c, int64_t n, float lr, double* d_loss, hipStream_t s) {
+    c->step += 1;
+    const float c1 = (float)(1.0 - pow(0.9, (double)c->step));
+    const float c2 = (float)(1.0 - pow(0.999, (double)c->step));
+    const int64_t blocks = (n + kRows - 1) / kRows;
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3((kNW + 255) / 256, kSplits), dim3(256), 0, s, c->d_partial,
+                       (int)blocks, c->d_tmp);
+    hipLaunchKernelGGL(reduce_kernel, dim3((kNW + 255) / 256), dim3(256), 0, s, c->d_tmp, (int)blocks,
+                       c->d_ploss, 1.0 / (double)n, d_loss, (float*)nullptr, c->d_w, c->d_m, c->d_v, c->d_img, lr,
+                       c1, c2);
     return hip_status(hipGetLastError());
 }
 
@@ -1205,7 +1230,7 @@ int32_t ntf_loss_grad(ntf_ctx* c, const float* x, const float* t, const float* c
     if (!c || !d_grad || !rows_ok(x, t, cnt, flags, n) || n == 0) return NT_ERR_ARG;
     if (hipSetDevice(c->device) != hipSuccess) return NT_ERR_DEVICE;
     hipStream_t s = (hipStream_t)stream;
-    int32_t st = run_grad(c, x, t, cnt, flags, aux, n, scale, s);
+    int32_t st = run_grad(c, Rows{x, t, cnt, flags, aux, n}, scale, s);
     if (st != NT_OK) return st;
     const int64_t blocks = (n + kRows - 1) / kRows;
     hipLaunchKernelGGL(reduce_partials_kernel, dim3((kNW + 255) / 256, kSplits), dim3(256), 0, s, c->d_partial,
@@ -1220,18 +1245,25 @@ int32_t ntf_train_step(ntf_ctx* c, const float* x, const float* t, const float* 
     if (!c || !rows_ok(x, t, cnt, flags, n) || n == 0 || !(lr >= 0.0f)) return NT_ERR_ARG;
     if (hipSetDevice(c->device) != hipSuccess) return NT_ERR_DEVICE;
     hipStream_t s = (hipStream_t)stream;
-    int32_t st = run_grad(c, x, t, cnt, flags, aux, n, 1.0f / (float)n, s);
+    const int32_t st = run_grad(c, Rows{x, t, cnt, flags, aux, n}, 1.0f / (float)n, s);
     if (st != NT_OK) return st;
-    c->step += 1;
-    const float c1 = (float)(1.0 - pow(0.9, (double)c->step));
-    const float c2 = (float)(1.0 - pow(0.999, (double)c->step));
-    const int64_t blocks = (n + kRows - 1) / kRows;
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3((kNW + 255) / 256, kSplits), dim3(256), 0, s, c->d_partial,
-                       (int)blocks, c->d_tmp);
-    hipLaunchKernelGGL(reduce_kernel, dim3((kNW + 255) / 256), dim3(256), 0, s, c->d_tmp, (int)blocks,
-                       c->d_ploss, 1.0 / (double)n, d_loss, (float*)nullptr, c->d_w, c->d_m, c->d_v, c->d_img, lr,
-                       c1, c2);
-    return hip_status(hipGetLastError());
+    return adam_update(c, n, lr, d_loss, s);
+}
+
+int32_t ntf_train_step_gather(ntf_ctx* c, const float* x, const float* t, const float* cnt, const uint8_t* flags,
+                              const float* aux, int64_t n_rows, const int64_t* d_idx, int64_t batch, float lr,
+                              double* d_loss, void* stream) {
+    if (!c || !rows_ok(x, t, cnt, flags, n_rows) || n_rows == 0 || !d_idx || batch <= 0 ||
+        batch > (int64_t)1 << 40 || !(lr >= 0.0f))
+        return NT_ERR_ARG;
+    if (hipSetDevice(c->device) != hipSuccess) return NT_ERR_DEVICE;
+    hipStream_t s = (hipStream_t)stream;
+    Rows rows{x, t, cnt, flags, aux, batch};
+    rows.idx = d_idx;
+    rows.n_src = n_rows;
+    const int32_t st = run_grad(c, rows, 1.0f / (float)batch, s);
+    if (st != NT_OK) return st;
+    return adam_update(c, batch, lr, d_loss, s);
 }
 
 int32_t ntf_pair_rows(ntf_ctx* c, const float* d_sources, int32_t n_sources, const float* d_modules,

@@ -230,3 +230,15 @@ class FlowModel:
         _abi.check(self.lib.ntf_train_step(self.handle, *rows.args(), C.c_float(lr),
                                            C.c_void_p(loss_out.data_ptr()) if loss_out is not None else None,
                                            self._stream()), "ntf_train_step")
+
+    def train_step_gather(self, rows: FlowRows, idx, lr: float, loss_out=None):
+        """train_step on the batch rows.slice(idx) (idx: device int64 tensor of
+        row indices) without building it: the kernel reads the rows in place."""
+        import torch
+        if idx.dtype != torch.int64 or not idx.is_contiguous() or idx.device != self.dev:
+            raise ValueError("idx: a contiguous int64 tensor on the model's device")
+        args = rows.args()
+        _abi.check(self.lib.ntf_train_step_gather(self.handle, *args[:5], args[5], C.c_void_p(idx.data_ptr()),
+                                                  C.c_int64(idx.numel()), C.c_float(lr),
+                                                  C.c_void_p(loss_out.data_ptr()) if loss_out is not None else None,
+                                                  self._stream()), "ntf_train_step_gather")

@@ -474,7 +474,7 @@ def train_surrogate(tables: TableSet, steps: int = 3000, batch: int = 16384, lr:
         decayed = frac >= 2 / 3
         lr_s = lr * 0.1 if decayed else lr
         idx = torch.randint(0, rows.n, (min(batch, rows.n),), generator=gen, device=dev)
-        model.train_step(rows.slice(idx), lr_s, loss_buf)
+        model.train_step_gather(rows, idx, lr_s, loss_buf)
         if average_every and decayed and (steps - 1 - step) % average_every == 0:
             w = model.get_weights().astype(np.float64)
             avg = w if avg is None else avg + w

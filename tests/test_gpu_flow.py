@@ -225,3 +225,34 @@ def test_pair_rows_and_time_logpdf(flow):
     bad = flow.time_logpdf(p, torch.tensor([-1, n_pairs], dtype=torch.int32, device=dev),
                            torch.zeros(2, device=dev)).cpu().numpy()
     assert np.all(np.isneginf(bad))
+
+
+def test_train_step_gather_equals_gathered_rows(flow):
+    """ntf_train_step_gather (the kernel reads the batch rows in place) gives
+    bit for bit the weights and losses of ntf_train_step on the gathered
+    batch, over several Adam steps; an index outside the arrays counts as an
+    absent row (the same as a row of no kind)."""
+    import torch
+    from neutrinotelescopes_amd.flow import FlowModel, FlowRows, init_weights
+    x, t, cnt, flags = _rows(5000, 21)
+    rows = FlowRows.from_numpy(x, t, cnt, flags)
+    a, b = FlowModel(0, seed=1), FlowModel(0, seed=1)
+    try:
+        a.set_weights(init_weights(22))
+        b.set_weights(init_weights(22))
+        la = torch.zeros(1, dtype=torch.float64, device=rows.t.device)
+        lb = torch.zeros_like(la)
+        g = torch.Generator(device="cpu").manual_seed(3)
+        for step in range(4):
+            idx = torch.randint(0, rows.n, (1000,), generator=g).to(rows.t.device)
+            sl = rows.slice(idx)
+            if step == 3:                       # one index outside the arrays
+                idx[5] = rows.n
+                sl.flags[5] = 0
+            a.train_step(sl, 1e-3, la)
+            b.train_step_gather(rows, idx, 1e-3, lb)
+            assert float(la.cpu()[0]) == float(lb.cpu()[0])
+        assert np.array_equal(a.get_weights(), b.get_weights())
+    finally:
+        a.close()
+        b.close()
