@@ -152,6 +152,11 @@ def test_near_fallback_on_device_equals_host_terms(trained):
     for _ in range(2):
         got = S.event_llh(sur, ev, nf, seed=64).cpu().numpy()
         np.testing.assert_allclose(got, ref, rtol=1e-9)
+    # device columns too small for the hits: the launch is repeated with room for all
+    small = S.NearFallback(eng, med, casc, modules, ev, capacity=64)
+    small.launch(64, st)
+    np.testing.assert_allclose(small.finish(st).cpu().numpy(), dev, rtol=1e-12)
+    assert small.cols.capacity > 64
 
 
 def test_full_size_likelihood_1024_cascades(trained):
