@@ -1014,7 +1014,8 @@ int32_t ntpmt_create(int32_t device, const ntpmt_config* c, ntpmt_ctx** out) {
     const size_t nw32 = (size_t)(n_pulse + 31) / 32;
     if (c->lds_factor_rows < 0) return cleanup(arg_error("lds_factor_rows >= 0"));
     // LDS per channel with the factor's first K rows in it; by default the
-    // largest K that leaves room for kChannelsPerCU channels on a CU
+    // largest K that leaves room for kChannelsPerCU channels on a CU, but not
+    // fewer rows than fit in the waveform's space (they cost no LDS)
     auto lds_for = [&](size_t K) {
         return sizeof(double) * ((size_t)J_trim * Q + std::max<size_t>(n_wave, K * (K + 1) / 2) + 2 * (size_t)n_adc +
                                  kPadLo + kPadHi + 2 * (size_t)n_adc) +
@@ -1025,7 +1026,7 @@ int32_t ntpmt_create(int32_t device, const ntpmt_config* c, ntpmt_ctx** out) {
         K = std::min<size_t>(K, (size_t)c->lds_factor_rows);
     } else {
         const size_t budget = (kLdsPerCU / kChannelsPerCU) & ~(size_t)511;
-        while (K > 1 && lds_for(K) > budget) --K;
+        while (K > 1 && lds_for(K) > budget && K * (K + 1) / 2 > (size_t)n_wave) --K;
     }
     d.lds_rows = (int)K;
     x->lds_bytes = lds_for(K);
