@@ -456,6 +456,9 @@ struct PhotonOut {
     bool hit;
 };
 
+constexpr int kDirMode = 1;       // DIRM bit: NT_DIR_FP64 (0) or NT_DIR_FP32_REF (1)
+constexpr int kDirSL = 2;         // DIRM bit: the medium has a Simplified-Liu share
+
 // One photon of source S from initialisation to its hit or its last step
 // (photon_prop_cuda.jl:133-188, 435-506).
 template <int KIND, int DIRM, class SrcPtr>
@@ -486,7 +489,7 @@ __device__ __forceinline__ void simulate_photon(const KParams& p, const_kparams_
             time = S->t0 + u01(r0.w) * (S->t1 - S->t0);
             break;
         case NT_SRC_AXICONE:
-            dir = rot_from_ez_mode(sdir, sph_to_cart(S->angle, 0.0f + u01(r0.z) * kTwoPi), DIRM);
+            dir = rot_from_ez_mode(sdir, sph_to_cart(S->angle, 0.0f + u01(r0.z) * kTwoPi), DIRM & kDirMode);
             break;
         case NT_SRC_PENCIL:
             dir = sdir;
@@ -495,13 +498,13 @@ __device__ __forceinline__ void simulate_photon(const KParams& p, const_kparams_
             const float lp = rand_gamma(S->gamma_d, S->gamma_c, S->gamma_scale, rng, nullptr) * S->lrad;
             pos = make_float3(pos.x + lp * sdir.x, pos.y + lp * sdir.y, pos.z + lp * sdir.z);
             time = S->t0 + lp / kCVacMNs;
-            dir = DIRM == NT_DIR_FP32_REF ? cherenkov_dir32(wl, u01(r0.y), u01(r0.z), u01(r0.w), sdir, mi)
-                                          : cherenkov_dir(wl, u01(r0.y), u01(r0.z), u01(r0.w), sdir, mi);
+            dir = (DIRM & kDirMode) == NT_DIR_FP32_REF ? cherenkov_dir32(wl, u01(r0.y), u01(r0.z), u01(r0.w), sdir, mi)
+                                                       : cherenkov_dir(wl, u01(r0.y), u01(r0.z), u01(r0.w), sdir, mi);
             break;
         }
         default:  // NT_SRC_POINTLIKE_CHERENKOV
-            dir = DIRM == NT_DIR_FP32_REF ? cherenkov_dir32(wl, u01(r0.y), u01(r0.z), u01(r0.w), sdir, mi)
-                                          : cherenkov_dir(wl, u01(r0.y), u01(r0.z), u01(r0.w), sdir, mi);
+            dir = (DIRM & kDirMode) == NT_DIR_FP32_REF ? cherenkov_dir32(wl, u01(r0.y), u01(r0.z), u01(r0.w), sdir, mi)
+                                                       : cherenkov_dir(wl, u01(r0.y), u01(r0.z), u01(r0.w), sdir, mi);
             break;
     }
     idir = dir;
@@ -525,18 +528,20 @@ __device__ __forceinline__ void simulate_photon(const KParams& p, const_kparams_
         if (hit_t < 0) {
             pos = make_float3(pos.x + dir.x * step, pos.y + dir.y * step, pos.z + dir.z * step);
             dist += step;
-            // the mixture uniform matters only with a Simplified-Liu share
-            // (a wave-uniform medium field): with none, u = 1 never selects it
+            // the mixture uniform matters only with a Simplified-Liu share:
+            // with none, u = 1 never selects it, and the instances for media
+            // without one (kDirSL clear) leave the uniform and the test out
+            constexpr bool kSL = (DIRM & kDirSL) != 0;
             float u_mix = 1.0f;
-            if (m.sl_fraction > 0.0f)
+            if (kSL && m.sl_fraction > 0.0f)
                 u_mix = u01(((wx & 0xFFu) << 8) | ((wy & 0xFFu) << 16) | ((wz & 0xFFu) << 24));
-            if (DIRM == NT_DIR_FP32_REF) {
-                dir = update_direction32(dir, u01(wy), u01(wz), u_mix, m);
+            if ((DIRM & kDirMode) == NT_DIR_FP32_REF) {
+                dir = update_direction32<kSL>(dir, u01(wy), u01(wz), u_mix, m);
             } else {
                 const const_kparams_ptr kh = kargs();
                 const HgConst hg{kh->medium.hg.omg, kh->medium.hg.twog, kh->medium.hg.omg2, kh->medium.hg.k0,
                                  kh->medium.hg.inv2g};
-                dir = update_direction(dir, u01(wy), u01(wz), u_mix, m, hg);
+                dir = update_direction<kSL>(dir, u01(wy), u01(wz), u_mix, m, hg);
             }
             return false;
         }
@@ -603,7 +608,9 @@ __device__ __forceinline__ void stage_write(uint32_t* st, int staged, uint64_t m
 
 // KIND >= 0: every source of the launch has that kind (smaller code, fewer
 // registers); KIND < 0: mixed kinds, dispatched per photon.  DIRM: the
-// medium's direction arithmetic (NT_DIR_FP64 or NT_DIR_FP32_REF).
+// medium's direction arithmetic (NT_DIR_FP64 or NT_DIR_FP32_REF) in bit 0,
+// kDirSL when the medium has a Simplified-Liu share (without it the step loop
+// carries no mixture test: +5 % on the cascade).
 template <int KIND, int DIRM>
 __global__ __launch_bounds__(kThreads, NTP_MIN_WAVES_PER_SIMD) void propagate_kernel(KParams p) {
     extern __shared__ float lds[];
@@ -1052,13 +1059,18 @@ PropagateFn propagate_variant_m(int v) {
     }
 }
 
-PropagateFn propagate_variant(int v, int dir_mode) {
-    return dir_mode == NT_DIR_FP32_REF ? propagate_variant_m<NT_DIR_FP32_REF>(v) : propagate_variant_m<NT_DIR_FP64>(v);
+// the instance for a source kind, the direction arithmetic and whether the
+// medium has a Simplified-Liu share (the instances without one leave its
+// mixture uniform and test out of the step loop)
+PropagateFn propagate_variant(int v, int dir_mode, bool sl) {
+    if (dir_mode == NT_DIR_FP32_REF)
+        return sl ? propagate_variant_m<NT_DIR_FP32_REF | kDirSL>(v) : propagate_variant_m<NT_DIR_FP32_REF>(v);
+    return sl ? propagate_variant_m<NT_DIR_FP64 | kDirSL>(v) : propagate_variant_m<NT_DIR_FP64>(v);
 }
 
 typedef void (*ServeFn)(KParams, SrvParams);
 ServeFn serve_variant(int dir_mode) {
-    return dir_mode == NT_DIR_FP32_REF ? serve_kernel<NT_DIR_FP32_REF> : serve_kernel<NT_DIR_FP64>;
+    return dir_mode == NT_DIR_FP32_REF ? serve_kernel<NT_DIR_FP32_REF | kDirSL> : serve_kernel<NT_DIR_FP64 | kDirSL>;
 }
 
 __global__ void clear_counters(int64_t* c) {
@@ -1368,7 +1380,8 @@ int32_t launch_plan(nt_ctx* c, uint64_t seed, const LaunchPlan& lp, hipStream_t 
     const int64_t need_blocks = (lp.total + kThreads - 1) / kThreads;
     const int dm = c->medium.dir_mode == NT_DIR_FP32_REF ? 1 : 0;
     const int blocks = (int)std::min<int64_t>(c->blocks_per_variant[dm][lp.variant], need_blocks);
-    hipLaunchKernelGGL(propagate_variant(lp.variant, c->medium.dir_mode), dim3(blocks), dim3(kThreads),
+    hipLaunchKernelGGL(propagate_variant(lp.variant, c->medium.dir_mode, c->medium.sl_fraction > 0.0f), dim3(blocks),
+                       dim3(kThreads),
                        lds_bytes(c->spectrum.n), s, kp);
     HIP_TRY(hipGetLastError());
     return NT_OK;
@@ -1692,7 +1705,7 @@ int32_t nt_create(int32_t device, nt_ctx** out) {
     for (int dm = 0; dm < 2; ++dm)
         for (int v = 0; v < kKernelVariants; ++v) {
             int per_cu = 0;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, propagate_variant(v, dm ? NT_DIR_FP32_REF : NT_DIR_FP64),
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, propagate_variant(v, dm ? NT_DIR_FP32_REF : NT_DIR_FP64, true),
                                                              kThreads, lds_bytes(NT_MAX_KNOTS)) != hipSuccess ||
                 per_cu < 1)
                 per_cu = 1;

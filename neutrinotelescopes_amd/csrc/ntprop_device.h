@@ -465,10 +465,11 @@ __device__ __forceinline__ float sl_cos(float g, float xi) {
 // and the rotation onto the current direction in FP64 (see above)
 // `hg` is passed apart from the medium so that the kernel can read it where
 // it is used rather than hold its ten scalar registers across the loop.
+template <bool kSL = true>
 __device__ __forceinline__ float3 update_direction(float3 dir, float eta, float u_phi, float u_mix,
                                                    const Medium& m, const HgConst& hg) {
     double c = hg_cos64(hg, eta);
-    if (m.sl_fraction > 0.0f && u_mix < m.sl_fraction)      // the first test is wave-uniform
+    if (kSL && m.sl_fraction > 0.0f && u_mix < m.sl_fraction)      // the second test is wave-uniform
         c = (double)sl_cos(m.g, eta);
     const double s = sin_from_cos64(c);
     float sp, cp;
@@ -477,9 +478,10 @@ __device__ __forceinline__ float3 update_direction(float3 dir, float eta, float 
 }
 
 // update_direction in FP32 with the reference's operation order (NT_DIR_FP32_REF)
+template <bool kSL = true>
 __device__ __forceinline__ float3 update_direction32(float3 dir, float eta, float u_phi, float u_mix,
                                                      const Medium& m) {
-    const float c = (u_mix < m.sl_fraction) ? sl_cos(m.g, eta) : hg_cos32(m.g, eta);
+    const float c = (kSL && u_mix < m.sl_fraction) ? sl_cos(m.g, eta) : hg_cos32(m.g, eta);
     // v_sqrt_f32 (1 ulp): the argument lies in [0, 1]
     const float s = __builtin_amdgcn_sqrtf(__builtin_fmaf(-c, c, 1.0f));
     float sp, cp;
