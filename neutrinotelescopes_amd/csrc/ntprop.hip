@@ -277,6 +277,8 @@ __device__ __forceinline__ Medium load_medium(const_kparams_ptr k) {
     m.dispersion_mode = k->medium.dispersion_mode; m.max_steps = k->medium.max_steps;
     m.dir_mode = k->medium.dir_mode;
     m.cherenkov_angle = k->medium.cherenkov_angle;
+    m.cherenkov_cos = k->medium.cherenkov_cos;
+    m.cherenkov_sin = k->medium.cherenkov_sin;
     m.hg.omg = k->medium.hg.omg; m.hg.twog = k->medium.hg.twog; m.hg.omg2 = k->medium.hg.omg2;
     m.hg.k0 = k->medium.hg.k0; m.hg.inv2g = k->medium.hg.inv2g;
     return m;
@@ -2154,6 +2156,9 @@ int32_t nt_set_option(nt_ctx* c, int32_t option, int64_t value) {
             HIP_TRY(hipSetDevice(c->device));
             if (int32_t w = wait_last_launch(c)) return w;
             c->medium.cherenkov_angle = (double)value * 1e-6;
+            // cos and sin once here (the C library's, as the oracle's), not per photon
+            c->medium.cherenkov_cos = value ? std::cos(c->medium.cherenkov_angle) : 0.0;
+            c->medium.cherenkov_sin = value ? std::sin(c->medium.cherenkov_angle) : 0.0;
             return NT_OK;
         default:
             snprintf(g_err, sizeof g_err, "unknown option %d", option);

@@ -154,6 +154,7 @@ struct Medium {
     int max_steps;
     int dir_mode;             // NT_DIR_FP64 or NT_DIR_FP32_REF
     double cherenkov_angle;   // > 0: fixed emission angle (rad); 0: acos(1/n(λ)) as the reference
+    double cherenkov_cos, cherenkov_sin;   // cos and sin of the fixed angle (computed on the host)
 };
 
 // refractive_index_fry, medium.jl:234-244
@@ -518,8 +519,8 @@ __device__ __forceinline__ float3 cherenkov_dir(float wl, float u_phi, float u_t
                                                 float3 src_dir, const Medium& m) {
     double ctc, stc;
     if (m.cherenkov_angle > 0.0) {                 // fixed emission angle (configuration option)
-        ctc = cos(m.cherenkov_angle);
-        stc = sin(m.cherenkov_angle);
+        ctc = m.cherenkov_cos;
+        stc = m.cherenkov_sin;
     } else {
         const double x = rcp64((double)wl), x2 = x * x;
         const double n = __builtin_fma(x, (double)m.a2, (double)m.a01) +
@@ -547,7 +548,7 @@ __device__ __forceinline__ float3 track_dir32(float u_tcos, float u_tphi) {
 __device__ __forceinline__ float3 cherenkov_dir32(float wl, float u_phi, float u_tcos, float u_tphi,
                                                   float3 src_dir, const Medium& m) {
     // θc = acos(1/n), or the fixed angle of the configuration option
-    const float ct = m.cherenkov_angle > 0.0 ? (float)cos(m.cherenkov_angle) : 1.0f / refractive_index(wl, m);
+    const float ct = m.cherenkov_angle > 0.0 ? (float)m.cherenkov_cos : 1.0f / refractive_index(wl, m);
     const float3 ph = sph_to_cart_cos(ct, u_phi);
     return rot_from_ez32(src_dir, rot_from_ez32(track_dir32(u_tcos, u_tphi), ph));
 }
