@@ -1071,8 +1071,10 @@ PropagateFn propagate_variant(int v, int dir_mode, bool sl) {
 }
 
 typedef void (*ServeFn)(KParams, SrvParams);
-ServeFn serve_variant(int dir_mode) {
-    return dir_mode == NT_DIR_FP32_REF ? serve_kernel<NT_DIR_FP32_REF | kDirSL> : serve_kernel<NT_DIR_FP64 | kDirSL>;
+ServeFn serve_variant(int dir_mode, bool sl) {
+    if (dir_mode == NT_DIR_FP32_REF)
+        return sl ? serve_kernel<NT_DIR_FP32_REF | kDirSL> : serve_kernel<NT_DIR_FP32_REF>;
+    return sl ? serve_kernel<NT_DIR_FP64 | kDirSL> : serve_kernel<NT_DIR_FP64>;
 }
 
 __global__ void clear_counters(int64_t* c) {
@@ -2742,7 +2744,8 @@ int32_t worklist_launch(nt_server* sv, int i) {
     sp.idle_ticks = kIdleTicks;
     {
         std::lock_guard<std::mutex> lk(c->mu);
-        hipLaunchKernelGGL(serve_variant(c->medium.dir_mode), dim3(sv->srv_blocks), dim3(kThreads),
+        hipLaunchKernelGGL(serve_variant(c->medium.dir_mode, c->medium.sl_fraction > 0.0f), dim3(sv->srv_blocks),
+                           dim3(kThreads),
                            lds_bytes(c->spectrum.n), sv->kstream[i], kp, sp);
         HIP_TRY(hipGetLastError());
     }
@@ -2972,7 +2975,7 @@ int32_t nt_server_create(nt_ctx* c, const nt_server_config* cfg, nt_server** out
                 return fail(e, "event");
         }
         int per_cu = 0;
-        if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, serve_variant(c->medium.dir_mode), kThreads,
+        if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, serve_variant(c->medium.dir_mode, true), kThreads,
                                                              lds_bytes(NT_MAX_KNOTS))) != hipSuccess)
             return fail(e, "occupancy");
         sv->srv_blocks = std::max(1, per_cu) * c->num_cus;
