@@ -216,6 +216,7 @@ __device__ __forceinline__ int first_hit(const KParams& p, float3 pos, float3 di
     const float inv_dd = dd > 0.0f ? __builtin_amdgcn_rcpf(dd) : 0.0f;
     const float re = __builtin_fmaf(step, 0x1p-10f, p.cluster_reach);
     const float re2 = re * re;
+    const float re2dd = re2 * dd;
     for (int c = 0; c < n_clusters; ++c) {
         const f4 c0 = cl[2 * c];
         const f4 c1 = cl[2 * c + 1];
@@ -225,6 +226,10 @@ __device__ __forceinline__ int first_hit(const KParams& p, float3 pos, float3 di
         // distance of the step's xy projection from the string axis first:
         // for strings long compared with a step it is the selective test
         const float wx = c0.x - pos.x, wy = c0.y - pos.y;
+        // the step's infinite line first (|w × d|² ≤ re²·|d|²: four
+        // instructions, and a segment within re implies a line within re)
+        const float cr = __builtin_fmaf(wx, dy, -(wy * dx));
+        if (cr * cr > re2dd) continue;
         const float t = fminf(fmaxf(__builtin_fmaf(wx, dx, wy * dy) * inv_dd, 0.0f), 1.0f);
         const float ex = __builtin_fmaf(-t, dx, wx), ey = __builtin_fmaf(-t, dy, wy);
         if (__builtin_fmaf(ex, ex, ey * ey) > re2) continue;
