@@ -471,15 +471,16 @@ __device__ __forceinline__ void residual(const Lds& s, const DevCfg& c, const Ac
 }
 
 // argmax over the columns outside the active set of w = Aᵀ r (first index
-// on ties).  With J ≤ kTaps and −kPadLo ≤ m0 ≤ 0 (the standard
-// configuration): lane p < Q takes the columns q·Q + p, its pattern's taps in
-// registers and a window of kTaps residual rows rotating through registers
-// (one broadcast LDS read per column); otherwise one column per lane and step.
+// on ties).  With J ≤ kTaps, −kPadLo ≤ m0 ≤ 0 and at most one pattern per
+// lane (Q ≤ 64; the standard configuration): lane p < Q takes the columns
+// q·Q + p, its pattern's taps in registers and a window of kTaps residual rows
+// rotating through registers (one broadcast LDS read per column); otherwise
+// one column per lane and step.
 template <int kJ>
 __device__ __forceinline__ void w_argmax(const Lds& s, const DevCfg& c, int lane, double& best, int& bi) {
     best = -INFINITY;
     bi = 0x7fffffff;
-    if (c.J <= kJ && c.m0 >= -kPadLo && c.m0 <= 0) {
+    if (c.J <= kJ && c.m0 >= -kPadLo && c.m0 <= 0 && c.Q <= kLanes) {
         if (lane < c.Q) {
             const int p = lane;
             double t[kTaps], win[kTaps];

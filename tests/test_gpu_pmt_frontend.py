@@ -169,6 +169,23 @@ def test_other_digitiser_configuration(engine):
     compare(cfg, got, oracle_reco(cfg, h, 5))
 
 
+def test_long_window_configuration(engine):
+    """A 600 ns window at a 0.125 GHz digitiser (1201 waveform samples, R = 16,
+    76 digitiser samples, 6001 unfolding columns in 80 patterns): the waveform
+    needs more LDS than twelve channels per CU leave, so the factor keeps the
+    48 rows its space holds and fewer channels share a CU; same bar against
+    the oracle."""
+    import dataclasses
+    base = pf.std_pmt_config()
+    cfg = dataclasses.replace(base, adc_freq=0.125, time_range=600.0, decim_taps=pf.decimation_taps(0.125 / 2.0))
+    fe2 = pf.PMTFrontEnd(cfg, 0)
+    assert (fe2.n_wave, fe2.n_adc, fe2.n_pulse) == (1201, 76, 6001)
+    h = synthetic_hits(n_modules=1, seed=12)
+    got = fe2.make_reco_pulses(h, seed=6, waveforms=True)
+    fe2.close()
+    compare(cfg, got, oracle_reco(cfg, h, 6))
+
+
 def test_empty_and_no_selection(fe):
     h = synthetic_hits(n_modules=1, seed=1)
     r = fe.make_reco_pulses(h, seed=1, select=np.zeros(len(h.time), bool))
