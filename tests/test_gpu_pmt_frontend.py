@@ -186,6 +186,24 @@ def test_long_window_configuration(engine):
     compare(cfg, got, oracle_reco(cfg, h, 6))
 
 
+def test_fast_digitiser_long_patterns(engine):
+    """A 1 GHz digitiser over 90 ns (181 waveform samples, R = 2, 91
+    digitiser samples, 901 unfolding columns in 10 patterns): the filtered
+    pulse reaches far more digitiser rows than the register-window pass holds
+    (the 16-row kernel instance, one column per lane and step); same bar
+    against the oracle."""
+    import dataclasses
+    base = pf.std_pmt_config()
+    cfg = dataclasses.replace(base, adc_freq=1.0, time_range=90.0, decim_taps=pf.decimation_taps(1.0 / 2.0))
+    fe2 = pf.PMTFrontEnd(cfg, 0)
+    assert (fe2.n_wave, fe2.n_adc, fe2.n_pulse) == (181, 91, 901)
+    h = synthetic_hits(n_modules=1, seed=13)
+    h.time = (h.time * 0.5).astype(np.float32)           # inside the shorter window
+    got = fe2.make_reco_pulses(h, seed=7, waveforms=True)
+    fe2.close()
+    compare(cfg, got, oracle_reco(cfg, h, 7))
+
+
 def test_empty_and_no_selection(fe):
     h = synthetic_hits(n_modules=1, seed=1)
     r = fe.make_reco_pulses(h, seed=1, select=np.zeros(len(h.time), bool))
