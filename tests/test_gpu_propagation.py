@@ -188,6 +188,26 @@ def test_extensions_sl_invlambda2_quanfry(engine):
     assert_hits_match(g, o)
 
 
+@pytest.mark.parametrize("max_steps", [0, 1, 2, 3, 6])
+def test_step_caps_per_photon(engine, max_steps):
+    """The step loop is unrolled by four: caps that end it inside a group
+    (1, 2, 3, 6), and no scattering step at all (0), give the oracle's hits
+    photon by photon (HG medium; a 2 m sphere 3 m away so that photons hit
+    within the first steps)."""
+    med = nt.make_cascadia_medium_properties(0.95, max_steps=max_steps)
+    spec = nt.CherenkovSpectrum((300.0, 800.0), 30, med)
+    src = nt.PointlikeIsotropicEmitter(np.zeros(3, np.float32), 0.0, 400_000)
+    tgt = [nt.DetectionSphere(np.array([0, 0, 3.0], np.float32), 2.0, 16, PMT_AREA, 1)]
+    s = nt.PhotonPropSetup([src], tgt, med, spec)
+    g = run_gpu(engine, s, seed=3)
+    o = O.propagate(s, seed=3)
+    if max_steps == 0:
+        assert len(g) == len(o) == 0
+        return
+    assert len(o) > 1000
+    assert_hits_match(g, o)
+
+
 def test_no_targets_and_zero_photons(engine):
     med = nt.make_cascadia_medium_properties(0.95)
     s = nt.PhotonPropSetup([nt.PointlikeIsotropicEmitter(np.zeros(3), 0.0, 10000)], [], med,
