@@ -136,7 +136,7 @@ def test_cascade_string_module_counts_independent_seeds(engine):
     assert stats.ks_2samp(g.time, o.time).pvalue > 0.01
 
 
-@pytest.mark.parametrize("kind", ["time_range", "axicone", "pencil", "pointlike_cherenkov",
+@pytest.mark.parametrize("kind", ["time_range", "axicone", "pencil", "pencil_vertical", "pointlike_cherenkov",
                                   "extended_cherenkov_vertical"])
 def test_source_kinds(engine, kind):
     med = nt.make_cascadia_medium_properties(0.9)
@@ -148,6 +148,8 @@ def test_source_kinds(engine, kind):
         "time_range": nt.PointlikeTimeRangeEmitter(p, (5.0, 25.0), n),
         "axicone": nt.AxiconeEmitter(p, d, 1.0, n, 0.7),
         "pencil": nt.PencilEmitter(p, d, 2.0, n),
+        # every photon's first scattering rotates about exactly +e_z (Q2 in the step loop)
+        "pencil_vertical": nt.PencilEmitter(p, np.array([0, 0, 1.0], np.float32), 2.0, n),
         "pointlike_cherenkov": nt.PointlikeCherenkovEmitter(p, d, 0.0, n),
         "extended_cherenkov_vertical": nt.ExtendedCherenkovEmitter(
             p, np.array([0, 0, 1.0], np.float32), 0.0, n, nt.LongitudinalParameterisation(6.0, 0.63, 0.36)),
