@@ -76,6 +76,29 @@ def test_results_identical_alone_and_in_mix(serving_engine, mode):
     assert total_hits > 100
 
 
+def test_results_identical_with_simplified_liu_share(mode):
+    """A medium with a Simplified-Liu share selects the kernel instances with
+    the mixture test; served hits still equal the same sources alone."""
+    med = nt.make_cascadia_medium_properties(0.95, sl_fraction=0.2)
+    eng = nt.PropagationEngine(0)
+    try:
+        eng.set_medium(med)
+        eng.set_spectrum(nt.CherenkovSpectrum((300.0, 800.0), 30, med))
+        eng.set_targets(cluster_targets(radius=2.0))
+        reqs = cascade_requests(4, med, np.random.default_rng(8), radius=60.0, height=200.0, e_range=(1e2, 1e4))
+        jobs = [(r.sources, r.seed, r.capacity) for r in reqs]
+        with PhotonServer(eng, device_budget=1 << 30, host_budget=1 << 29, round_photons=400_000, mode=mode) as srv:
+            got = [srv.wait(srv.submit(src, seed, cap)) for src, seed, cap in jobs]
+        hits = 0
+        for (sources, seed, cap), (h, _) in zip(jobs, got):
+            ref = alone(eng, sources, seed, cap)
+            assert_same(h.sorted(), ref)
+            hits += len(ref)
+        assert hits > 0
+    finally:
+        eng.close()
+
+
 def test_small_budget_admits_in_turn(serving_engine, mode):
     eng, med = serving_engine
     rng = np.random.default_rng(6)
