@@ -97,6 +97,11 @@ def test_device_likelihood_matches_host_terms(trained):
     print("llh", llh, "host", host)
     # pair features differ between the FP32 device and FP64 host paths only by rounding
     np.testing.assert_allclose(llh, host, rtol=2e-4, atol=1.0)
+    # event_llh with a fallback that has nothing to do (no pair within 5 m) is the surrogate alone
+    if not ev.near.any():
+        nf = S.NearFallback(eng, med, casc, modules, ev)
+        assert nf.evs.size == 0
+        assert np.array_equal(S.event_llh(sur, ev, nf).cpu().numpy(), llh)
 
 
 def test_fallback_near_modules(trained):
