@@ -177,9 +177,10 @@ def test_multi_source_ragged(engine):
     assert_hits_match(g, o)
 
 
-def test_extensions_sl_invlambda2_quanfry(engine):
+@pytest.mark.parametrize("arith", ["fp64", "fp32"])
+def test_extensions_sl_invlambda2_quanfry(engine, arith):
     med = nt.make_cascadia_medium_properties(0.95, sl_fraction=0.2, dispersion_mode="quan_fry",
-                                             max_steps=25)
+                                             max_steps=25, direction_arithmetic=arith)
     med.sca_len_scale = float(50.0 / nt.scattering_length(np.float32(400.0), med))
     spec = nt.InvLambda2Spectrum((300.0, 600.0))
     src = nt.PointlikeIsotropicEmitter(np.zeros(3, np.float32), 0.0, 500_000)
